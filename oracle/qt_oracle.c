@@ -1,0 +1,799 @@
+/*
+ * qt_oracle.c -- CPU FP64 restatement of the reference RGF path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): loaded by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ *
+ * Two flavours are built from this one file:
+ *   plain  -- the BLAS/LAPACK routines the reference calls (ZGEMM, ZGETRF,
+ *             ZGETRI) are restated below from their published algorithms;
+ *   mkl    -- -DQTO_USE_MKL links the vendor ZGEMM/ZGETRF/ZGETRI, as the
+ *             reference links -llapack -lblas (Makefile:5-6).  This is the
+ *             flavour timed as the CPU baseline.
+ * The RGF, Sancho-Rubio, Fermi and correlation code is shared by both.
+ */
+#include "qt_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define BOLTZ 8.61734e-05 /* eV/K, rgf_mod.f90:38 */
+
+/* ------------------------------------------------------------------ */
+/* BLAS / LAPACK layer                                                 */
+/* ------------------------------------------------------------------ */
+
+#ifdef QTO_USE_MKL
+extern void zgemm_(const char*, const char*, const int*, const int*, const int*,
+                   const void*, const void*, const int*, const void*, const int*,
+                   const void*, void*, const int*);
+extern void zgetrf_(const int*, const int*, void*, const int*, int*, int*);
+extern void zgetri_(const int*, void*, const int*, const int*, void*, const int*,
+                    int*);
+extern int MKL_Set_Threading_Layer(int);
+#define QTO_MKL_THREADING_SEQUENTIAL 1
+
+static const char* op_char(int op) { return op == QTO_OP_N ? "N" : (op == QTO_OP_T ? "T" : "C"); }
+
+void qto_zgemm(int opa, int opb, int m, int n, int k, qto_c alpha, const qto_c* A,
+               int lda, const qto_c* B, int ldb, qto_c beta, qto_c* C, int ldc) {
+    zgemm_(op_char(opa), op_char(opb), &m, &n, &k, &alpha, A, &lda, B, &ldb, &beta, C,
+           &ldc);
+}
+
+int qto_zgetrf(int n, qto_c* A, int lda, int* ipiv) {
+    int info = 0;
+    zgetrf_(&n, &n, A, &lda, ipiv, &info);
+    for (int i = 0; i < n; ++i) ipiv[i] -= 1; /* keep 0-based pivots internally */
+    return info;
+}
+
+int qto_zgetri(int n, qto_c* A, int lda, const int* ipiv) {
+    int info = 0, lwork = n * n;
+    int* ip1 = (int*)malloc(sizeof(int) * (size_t)n);
+    qto_c* work = (qto_c*)malloc(sizeof(qto_c) * (size_t)lwork);
+    for (int i = 0; i < n; ++i) ip1[i] = ipiv[i] + 1;
+    zgetri_(&n, A, &lda, ip1, work, &lwork, &info);
+    free(work);
+    free(ip1);
+    return info;
+}
+
+const char* qto_blas_flavour(void) { return "mkl"; }
+
+__attribute__((constructor)) static void qto_mkl_sequential(void) {
+    MKL_Set_Threading_Layer(QTO_MKL_THREADING_SEQUENTIAL);
+}
+
+#else /* plain restatement */
+
+static inline qto_c op_elem(const qto_c* A, int lda, int op, int i, int k) {
+    if (op == QTO_OP_N) return A[i + (size_t)k * lda];
+    if (op == QTO_OP_T) return A[k + (size_t)i * lda];
+    return conj(A[k + (size_t)i * lda]);
+}
+
+/* ZGEMM, reference BLAS loop order: column j of C, then the k-sum. */
+void qto_zgemm(int opa, int opb, int m, int n, int k, qto_c alpha, const qto_c* A,
+               int lda, const qto_c* B, int ldb, qto_c beta, qto_c* C, int ldc) {
+    for (int j = 0; j < n; ++j) {
+        qto_c* c = C + (size_t)j * ldc;
+        if (beta == 0.0) {
+            for (int i = 0; i < m; ++i) c[i] = 0.0;
+        } else if (beta != 1.0) {
+            for (int i = 0; i < m; ++i) c[i] *= beta;
+        }
+        if (opa == QTO_OP_N) {
+            for (int l = 0; l < k; ++l) {
+                qto_c t = alpha * op_elem(B, ldb, opb, l, j);
+                const qto_c* a = A + (size_t)l * lda;
+                for (int i = 0; i < m; ++i) c[i] += t * a[i];
+            }
+        } else {
+            for (int i = 0; i < m; ++i) {
+                const qto_c* a = A + (size_t)i * lda;
+                qto_c s = 0.0;
+                if (opa == QTO_OP_T)
+                    for (int l = 0; l < k; ++l) s += a[l] * op_elem(B, ldb, opb, l, j);
+                else
+                    for (int l = 0; l < k; ++l) s += conj(a[l]) * op_elem(B, ldb, opb, l, j);
+                c[i] += alpha * s;
+            }
+        }
+    }
+}
+
+static inline double cabs1(qto_c z) { return fabs(creal(z)) + fabs(cimag(z)); }
+
+/* ZGETRF (unblocked ZGETF2 form): pivot = first index of max |re|+|im|
+ * (IZAMAX), full-row interchange, column scaling, rank-1 update. */
+int qto_zgetrf(int n, qto_c* A, int lda, int* ipiv) {
+    int info = 0;
+    for (int j = 0; j < n; ++j) {
+        int p = j;
+        double best = cabs1(A[j + (size_t)j * lda]);
+        for (int i = j + 1; i < n; ++i) {
+            double v = cabs1(A[i + (size_t)j * lda]);
+            if (v > best) { best = v; p = i; }
+        }
+        ipiv[j] = p;
+        if (A[p + (size_t)j * lda] != 0.0) {
+            if (p != j)
+                for (int c = 0; c < n; ++c) {
+                    qto_c t = A[j + (size_t)c * lda];
+                    A[j + (size_t)c * lda] = A[p + (size_t)c * lda];
+                    A[p + (size_t)c * lda] = t;
+                }
+            qto_c r = 1.0 / A[j + (size_t)j * lda];
+            for (int i = j + 1; i < n; ++i) A[i + (size_t)j * lda] *= r;
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        for (int c = j + 1; c < n; ++c) {
+            qto_c u = A[j + (size_t)c * lda];
+            if (u == 0.0) continue;
+            qto_c* col = A + (size_t)c * lda;
+            const qto_c* lj = A + (size_t)j * lda;
+            for (int i = j + 1; i < n; ++i) col[i] -= lj[i] * u;
+        }
+    }
+    return info;
+}
+
+/* ZGETRI: inv(U) in place (ZTRTI2, upper, non-unit), then solve
+ * inv(A)*L = inv(U) column by column from the right, then undo the row
+ * interchanges as column interchanges in reverse order. */
+int qto_zgetri(int n, qto_c* A, int lda, const int* ipiv) {
+    for (int j = 0; j < n; ++j)
+        if (A[j + (size_t)j * lda] == 0.0) return j + 1;
+    for (int j = 0; j < n; ++j) {
+        qto_c* cj = A + (size_t)j * lda;
+        cj[j] = 1.0 / cj[j];
+        qto_c ajj = -cj[j];
+        /* x(0:j) := triu(A(0:j,0:j)) * x, x = A(0:j, j) (ZTRMV upper, no-trans) */
+        for (int l = 0; l < j; ++l) {
+            qto_c t = cj[l];
+            if (t == 0.0) continue;
+            const qto_c* cl = A + (size_t)l * lda;
+            for (int i = 0; i < l; ++i) cj[i] += t * cl[i];
+            cj[l] = t * cl[l];
+        }
+        for (int i = 0; i < j; ++i) cj[i] *= ajj;
+    }
+    qto_c* work = (qto_c*)malloc(sizeof(qto_c) * (size_t)n);
+    for (int j = n - 1; j >= 0; --j) {
+        qto_c* cj = A + (size_t)j * lda;
+        for (int i = j + 1; i < n; ++i) { work[i] = cj[i]; cj[i] = 0.0; }
+        for (int l = j + 1; l < n; ++l) {
+            qto_c t = work[l];
+            if (t == 0.0) continue;
+            const qto_c* cl = A + (size_t)l * lda;
+            for (int i = 0; i < n; ++i) cj[i] -= t * cl[i];
+        }
+    }
+    free(work);
+    for (int j = n - 2; j >= 0; --j) {
+        int jp = ipiv[j];
+        if (jp != j)
+            for (int i = 0; i < n; ++i) {
+                qto_c t = A[i + (size_t)j * lda];
+                A[i + (size_t)j * lda] = A[i + (size_t)jp * lda];
+                A[i + (size_t)jp * lda] = t;
+            }
+    }
+    return 0;
+}
+
+const char* qto_blas_flavour(void) { return "plain"; }
+#endif
+
+int qto_invert(qto_c* A, int n) {
+    int* ipiv = (int*)malloc(sizeof(int) * (size_t)n);
+    int info = qto_zgetrf(n, A, n, ipiv);
+    if (info == 0) info = qto_zgetri(n, A, n, ipiv);
+    if (info != 0) memset(A, 0, sizeof(qto_c) * (size_t)n * n);
+    free(ipiv);
+    return info;
+}
+
+double qto_ferm(double a) { return 1.0 / (1.0 + exp(a)); }
+
+/* ------------------------------------------------------------------ */
+/* small helpers mirroring matrix_c MUL_C / triMUL_C                   */
+/* ------------------------------------------------------------------ */
+
+typedef struct { qto_c* m; int r, c; } mat;
+
+static qto_c* zalloc(size_t n) { return (qto_c*)calloc(n ? n : 1, sizeof(qto_c)); }
+
+/* R = op(A)*op(B), matrix_c.f90:615-660; returns the shape of R. */
+static void mulc(const qto_c* A, int ra, int ca, int opa, const qto_c* B, int rb, int cb,
+                 int opb, qto_c* R, int* rm, int* rn) {
+    int m = opa == QTO_OP_N ? ra : ca, k = opa == QTO_OP_N ? ca : ra;
+    int n = opb == QTO_OP_N ? cb : rb;
+    qto_zgemm(opa, opb, m, n, k, 1.0, A, ra, B, rb, 0.0, R, m);
+    if (rm) *rm = m;
+    if (rn) *rn = n;
+}
+
+/* R = (op(A)*op(B))*op(C), matrix_c.f90:576-613 */
+static void trimulc(const qto_c* A, int ra, int ca, int opa, const qto_c* B, int rb,
+                    int cb, int opb, const qto_c* C, int rc, int cc, int opc, qto_c* R,
+                    qto_c* tmp) {
+    int tm, tn;
+    mulc(A, ra, ca, opa, B, rb, cb, opb, tmp, &tm, &tn);
+    mulc(tmp, tm, tn, QTO_OP_N, C, rc, cc, opc, R, NULL, NULL);
+}
+
+static void dagger(const qto_c* A, int r, int c, qto_c* R) { /* R (c x r) = A^H */
+    for (int j = 0; j < c; ++j)
+        for (int i = 0; i < r; ++i) R[j + (size_t)i * c] = conj(A[i + (size_t)j * r]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Sancho-Rubio, rgf_mod.f90:482-568 / gw_mod.f90:1378-1464            */
+/* ------------------------------------------------------------------ */
+
+int qto_sancho(int nm, double E, double eta, int nmax, double tol, int reset_on_fail,
+               const qto_c* S00, const qto_c* H00, const qto_c* H10, qto_c* G00,
+               qto_c* GBB, int* converged) {
+    size_t nn = (size_t)nm * nm;
+    qto_c *A = zalloc(nn), *B = zalloc(nn), *C = zalloc(nn);
+    qto_c *H_BB = zalloc(nn), *H_SS = zalloc(nn), *H_01 = zalloc(nn), *H_10 = zalloc(nn);
+    qto_c z = E + I * eta;
+    memcpy(H_BB, H00, nn * sizeof(qto_c));
+    memcpy(H_10, H10, nn * sizeof(qto_c));
+    dagger(H_10, nm, nm, H_01);
+    memcpy(H_SS, H00, nn * sizeof(qto_c));
+    int it, conv = 0;
+    for (it = 1; it <= nmax; ++it) {
+        for (size_t e = 0; e < nn; ++e) A[e] = z * S00[e] - H_BB[e];
+        qto_invert(A, nm);
+        qto_zgemm(QTO_OP_N, QTO_OP_N, nm, nm, nm, 1.0, A, nm, H_10, nm, 0.0, B, nm);
+        qto_zgemm(QTO_OP_N, QTO_OP_N, nm, nm, nm, 1.0, H_01, nm, B, nm, 0.0, C, nm);
+        for (size_t e = 0; e < nn; ++e) { H_SS[e] += C[e]; H_BB[e] += C[e]; }
+        qto_zgemm(QTO_OP_N, QTO_OP_N, nm, nm, nm, 1.0, H_10, nm, B, nm, 0.0, C, nm);
+        qto_zgemm(QTO_OP_N, QTO_OP_N, nm, nm, nm, 1.0, A, nm, H_01, nm, 0.0, B, nm);
+        qto_zgemm(QTO_OP_N, QTO_OP_N, nm, nm, nm, 1.0, H_10, nm, B, nm, 0.0, A, nm);
+        memcpy(H_10, C, nn * sizeof(qto_c));
+        for (size_t e = 0; e < nn; ++e) H_BB[e] += A[e];
+        qto_zgemm(QTO_OP_N, QTO_OP_N, nm, nm, nm, 1.0, H_01, nm, B, nm, 0.0, C, nm);
+        memcpy(H_01, C, nn * sizeof(qto_c));
+        double error = 0.0;
+        for (int k = 0; k < nm; ++k)
+            for (int j = 0; j < nm; ++j) {
+                qto_c v = C[k + (size_t)j * nm];
+                error += sqrt(cimag(v) * cimag(v) + creal(v) * creal(v));
+            }
+        if (fabs(error) < tol) { conv = 1; break; }
+        if (it == nmax && reset_on_fail) {
+            memcpy(H_SS, H00, nn * sizeof(qto_c));
+            memcpy(H_BB, H00, nn * sizeof(qto_c));
+        }
+    }
+    if (it > nmax) it = nmax;
+    for (size_t e = 0; e < nn; ++e) G00[e] = z * S00[e] - H_SS[e];
+    qto_invert(G00, nm);
+    if (GBB) {
+        for (size_t e = 0; e < nn; ++e) GBB[e] = z * S00[e] - H_BB[e];
+        qto_invert(GBB, nm);
+    }
+    free(A); free(B); free(C); free(H_BB); free(H_SS); free(H_01); free(H_10);
+    if (converged) *converged = conv;
+    return it;
+}
+
+/* ------------------------------------------------------------------ */
+/* RGF sweeps                                                          */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int M;
+    qto_c *A, *B, *C, *T, *H00, *H10, *G00, *sig, *sigL, *GN0, *ident;
+} work_t;
+
+static void work_init(work_t* w, int Mmax) {
+    size_t nn = (size_t)Mmax * Mmax;
+    w->M = Mmax;
+    w->A = zalloc(nn); w->B = zalloc(nn); w->C = zalloc(nn); w->T = zalloc(nn);
+    w->H00 = zalloc(nn); w->H10 = zalloc(nn); w->G00 = zalloc(nn); w->sig = zalloc(nn);
+    w->sigL = zalloc(nn); w->GN0 = zalloc(nn); w->ident = zalloc(nn);
+    for (int i = 0; i < Mmax; ++i) w->ident[i + (size_t)i * Mmax] = 1.0;
+}
+
+static void work_free(work_t* w) {
+    free(w->A); free(w->B); free(w->C); free(w->T); free(w->H00); free(w->H10);
+    free(w->G00); free(w->sig); free(w->sigL); free(w->GN0); free(w->ident);
+}
+
+/* identity of size M in a buffer laid out with leading dimension M */
+static const qto_c* ident_of(work_t* w, int M, qto_c* buf) {
+    memset(buf, 0, sizeof(qto_c) * (size_t)M * M);
+    for (int i = 0; i < M; ++i) buf[i + (size_t)i * M] = 1.0;
+    (void)w;
+    return buf;
+}
+
+/* H00 = Hii + sigma_r_ph*S  (rgf_mod.f90:77-79) */
+static void build_h00(const qto_point* p, int s, int M, const qto_c* S, qto_c* H00, qto_c* tmp) {
+    size_t nn = (size_t)M * M;
+    if (p->sigma_r_ph && p->sigma_r_ph[s]) {
+        mulc(p->sigma_r_ph[s], M, M, QTO_OP_N, S, M, M, QTO_OP_N, tmp, NULL, NULL);
+        for (size_t e = 0; e < nn; ++e) H00[e] = p->Hii[s][e] + tmp[e];
+    } else {
+        for (size_t e = 0; e < nn; ++e) H00[e] = p->Hii[s][e];
+    }
+}
+
+/* out += sigma_lesser_ph*S  (rgf_mod.f90:99,132) */
+static void add_sigl(const qto_point* p, int s, int M, const qto_c* S, qto_c* out, qto_c* tmp) {
+    if (p->sigma_lesser_ph && p->sigma_lesser_ph[s]) {
+        mulc(p->sigma_lesser_ph[s], M, M, QTO_OP_N, S, M, M, QTO_OP_N, tmp, NULL, NULL);
+        for (size_t e = 0; e < (size_t)M * M; ++e) out[e] += tmp[e];
+    }
+}
+
+static int uniform_sizes(const qto_point* p) {
+    for (int s = 1; s < p->nx; ++s)
+        if (p->nm[s] != p->nm[0]) return 0;
+    return 1;
+}
+
+static void note_info(qto_result* r, int info) {
+    if (info != 0 && r->info == 0) r->info = info;
+}
+
+int qto_rgf_forward(const qto_point* p, qto_result* r) {
+    const int nx = p->nx;
+    if (nx < 2) return -1;
+    if (!p->coupling_fix && !uniform_sizes(p)) return -2;
+    int Mmax = 0;
+    for (int s = 0; s < nx; ++s) Mmax = p->nm[s] > Mmax ? p->nm[s] : Mmax;
+    work_t w;
+    work_init(&w, Mmax);
+    qto_c* Sbuf = zalloc((size_t)Mmax * Mmax);
+    qto_c* tmp2 = zalloc((size_t)Mmax * Mmax);
+    const qto_c z = p->En + 0.0 * I; /* rgf_mod.f90:62 */
+    r->info = 0;
+
+    /* ---- left contact, rgf_mod.f90:63-104 */
+    int s = 0, M = p->nm[0];
+    size_t nn = (size_t)M * M;
+    const qto_c* S = p->Sii ? p->Sii[0] : ident_of(&w, M, Sbuf);
+    build_h00(p, 0, M, S, w.H00, w.B);
+    dagger(p->H1i[0], M, M, w.H10);
+    r->sancho_iters[0] = qto_sancho(M, p->En, p->lead_eta, p->lead_nmax, p->lead_tol, 0, S,
+                                    w.H00, w.H10, w.G00, NULL, NULL);
+    trimulc(p->H1i[0], M, M, QTO_OP_N, w.G00, M, M, QTO_OP_N, p->H1i[0], M, M, QTO_OP_C,
+            w.sigL, w.T);
+    for (size_t e = 0; e < nn; ++e) w.B[e] = z * S[e] - w.H00[e] - w.sigL[e];
+    note_info(r, qto_invert(w.B, M));
+    memcpy(r->Gl[0], w.B, nn * sizeof(qto_c));
+    const double f_first = qto_ferm((p->En - p->mur) / (BOLTZ * p->tempr));
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i)
+            w.sig[i + (size_t)j * M] =
+                -(w.sigL[i + (size_t)j * M] - conj(w.sigL[j + (size_t)i * M])) * f_first;
+    add_sigl(p, 0, M, S, w.sig, w.B);
+    trimulc(r->Gl[0], M, M, QTO_OP_N, w.sig, M, M, QTO_OP_N, r->Gl[0], M, M, QTO_OP_C,
+            r->Gln[0], w.T);
+
+    /* ---- first pass l -> r, rgf_mod.f90:113-136 */
+    for (s = 1; s <= nx - 2; ++s) {
+        M = p->nm[s];
+        int Mp = p->nm[s - 1];
+        nn = (size_t)M * M;
+        S = p->Sii ? p->Sii[s] : ident_of(&w, M, Sbuf);
+        build_h00(p, s, M, S, w.H00, w.B);
+        trimulc(p->H1i[s], M, Mp, QTO_OP_N, r->Gl[s - 1], Mp, Mp, QTO_OP_N, p->H1i[s], M, Mp,
+                QTO_OP_C, w.B, w.T);
+        for (size_t e = 0; e < nn; ++e) w.A[e] = z * S[e] - w.H00[e] - w.B[e];
+        note_info(r, qto_invert(w.A, M));
+        memcpy(r->Gl[s], w.A, nn * sizeof(qto_c));
+        trimulc(p->H1i[s], M, Mp, QTO_OP_N, r->Gln[s - 1], Mp, Mp, QTO_OP_N, p->H1i[s], M,
+                Mp, QTO_OP_C, w.B, w.T);
+        add_sigl(p, s, M, S, w.B, w.A);
+        trimulc(r->Gl[s], M, M, QTO_OP_N, w.B, M, M, QTO_OP_N, r->Gl[s], M, M, QTO_OP_C,
+                r->Gln[s], w.T);
+    }
+
+    /* ---- right contact, rgf_mod.f90:142-202 */
+    s = nx - 1;
+    M = p->nm[s];
+    int Mp = p->nm[s - 1];
+    nn = (size_t)M * M;
+    S = p->Sii ? p->Sii[s] : ident_of(&w, M, Sbuf);
+    build_h00(p, s, M, S, w.H00, w.B);
+    memcpy(w.H10, p->H1i[nx], nn * sizeof(qto_c));
+    r->sancho_iters[1] = qto_sancho(M, p->En, p->lead_eta, p->lead_nmax, p->lead_tol, 0, S,
+                                    w.H00, w.H10, w.G00, NULL, NULL);
+    qto_c* sigR = tmp2;
+    trimulc(w.H10, M, M, QTO_OP_C, w.G00, M, M, QTO_OP_N, w.H10, M, M, QTO_OP_N, sigR, w.T);
+    trimulc(p->H1i[s], M, Mp, QTO_OP_N, r->Gl[s - 1], Mp, Mp, QTO_OP_N, p->H1i[s], M, Mp,
+            QTO_OP_C, w.B, w.T);
+    for (size_t e = 0; e < nn; ++e) w.A[e] = z * S[e] - w.H00[e] - w.B[e] - sigR[e];
+    note_info(r, qto_invert(w.A, M));
+    qto_c* Gr_last = r->G_r && r->G_r[s] ? r->G_r[s] : w.GN0;
+    memcpy(Gr_last, w.A, nn * sizeof(qto_c));
+    memcpy(r->Gl[s], w.A, nn * sizeof(qto_c));
+    trimulc(p->H1i[s], M, Mp, QTO_OP_N, r->Gln[s - 1], Mp, Mp, QTO_OP_N, p->H1i[s], M, Mp,
+            QTO_OP_C, w.B, w.T);
+    memset(w.A, 0, nn * sizeof(qto_c));
+    add_sigl(p, s, M, S, w.A, w.C);
+    const double f_last = qto_ferm((p->En - p->mul) / (BOLTZ * p->templ));
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            w.sig[e] = -(sigR[e] - conj(sigR[j + (size_t)i * M])) * f_last;
+            w.sig[e] = w.sig[e] + w.A[e] + w.B[e];
+        }
+    /* G_lesser / G_greater of the last slab; they seed the second pass, so
+     * they live in the caller's lists (or in scratch when not requested). */
+    qto_c** GL = (qto_c**)calloc((size_t)nx, sizeof(qto_c*));
+    qto_c** GG = (qto_c**)calloc((size_t)nx, sizeof(qto_c*));
+    qto_c** GR = (qto_c**)calloc((size_t)nx, sizeof(qto_c*));
+    qto_c** own = (qto_c**)calloc((size_t)3 * nx, sizeof(qto_c*));
+    for (int t = 0; t < nx; ++t) {
+        size_t bt = (size_t)p->nm[t] * p->nm[t];
+        GR[t] = r->G_r && r->G_r[t] ? r->G_r[t] : (own[3 * t] = zalloc(bt));
+        GL[t] = r->G_lesser && r->G_lesser[t] ? r->G_lesser[t] : (own[3 * t + 1] = zalloc(bt));
+        GG[t] = r->G_greater && r->G_greater[t] ? r->G_greater[t] : (own[3 * t + 2] = zalloc(bt));
+    }
+    if (GR[s] != Gr_last) memcpy(GR[s], Gr_last, nn * sizeof(qto_c));
+    trimulc(GR[s], M, M, QTO_OP_N, w.sig, M, M, QTO_OP_N, GR[s], M, M, QTO_OP_C, GL[s], w.T);
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            GG[s][e] = GL[s][e] + (GR[s][e] - conj(GR[s][j + (size_t)i * M]));
+        }
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            w.A[e] = -(sigR[e] - conj(sigR[j + (size_t)i * M])) * f_last;
+        }
+    mulc(w.A, M, M, QTO_OP_N, GG[s], M, M, QTO_OP_N, w.B, NULL, NULL);
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            w.A[e] = -(sigR[e] - conj(sigR[j + (size_t)i * M])) * (f_last - 1.0);
+        }
+    mulc(w.A, M, M, QTO_OP_N, GL[s], M, M, QTO_OP_N, w.C, NULL, NULL);
+    double tim = 0.0;
+    for (int j = 0; j < M; ++j) tim += creal(w.B[j + (size_t)j * M] - w.C[j + (size_t)j * M]);
+    if (r->Jdens && r->Jdens[s])
+        for (size_t e = 0; e < nn; ++e) r->Jdens[s][e] = w.B[e] - w.C[e];
+    r->tr = tim;
+
+    /* ---- second pass r -> l, rgf_mod.f90:212-250 */
+    for (s = nx - 2; s >= 0; --s) {
+        M = p->nm[s];
+        int Mn = p->nm[s + 1];
+        nn = (size_t)M * M;
+        /* coupling between slab s and s+1: the reference uses H1i(ii) (the
+         * coupling of slab s to s-1), identical for uniform devices */
+        const qto_c* W = p->coupling_fix ? p->H1i[s + 1] : p->H1i[s];
+        qto_c* Gl = r->Gl[s];
+        qto_c* Gln = r->Gln[s];
+        /* A = G<(i+1) W Gl^H + G(i+1) W Gln  (Mn x M) */
+        trimulc(GL[s + 1], Mn, Mn, QTO_OP_N, W, Mn, M, QTO_OP_N, Gl, M, M, QTO_OP_C, w.A, w.T);
+        trimulc(GR[s + 1], Mn, Mn, QTO_OP_N, W, Mn, M, QTO_OP_N, Gln, M, M, QTO_OP_N, w.B, w.T);
+        for (size_t e = 0; e < (size_t)Mn * M; ++e) w.A[e] += w.B[e];
+        mulc(W, Mn, M, QTO_OP_C, w.A, Mn, M, QTO_OP_N, w.B, NULL, NULL);
+        if (r->Jdens && r->Jdens[s])
+            for (size_t e = 0; e < nn; ++e) r->Jdens[s][e] = -2.0 * w.B[e];
+        /* GN0 = Gl W^H G(i+1);  G = Gl + GN0 W Gl */
+        mulc(Gl, M, M, QTO_OP_N, W, Mn, M, QTO_OP_C, w.B, NULL, NULL);
+        mulc(w.B, M, Mn, QTO_OP_N, GR[s + 1], Mn, Mn, QTO_OP_N, w.GN0, NULL, NULL);
+        mulc(w.GN0, M, Mn, QTO_OP_N, W, Mn, M, QTO_OP_N, w.C, NULL, NULL);
+        mulc(w.C, M, M, QTO_OP_N, Gl, M, M, QTO_OP_N, w.A, NULL, NULL);
+        for (size_t e = 0; e < nn; ++e) GR[s][e] = Gl[e] + w.A[e];
+        /* G< = Gln + Gl W^H G<(i+1) W Gl^H */
+        mulc(Gl, M, M, QTO_OP_N, W, Mn, M, QTO_OP_C, w.B, NULL, NULL);
+        mulc(w.B, M, Mn, QTO_OP_N, GL[s + 1], Mn, Mn, QTO_OP_N, w.C, NULL, NULL);
+        mulc(w.C, M, Mn, QTO_OP_N, W, Mn, M, QTO_OP_N, w.A, NULL, NULL);
+        mulc(w.A, M, M, QTO_OP_N, Gl, M, M, QTO_OP_C, w.C, NULL, NULL);
+        for (size_t e = 0; e < nn; ++e) GL[s][e] = Gln[e] + w.C[e];
+        /* + GN0 W Gln */
+        mulc(w.GN0, M, Mn, QTO_OP_N, W, Mn, M, QTO_OP_N, w.B, NULL, NULL);
+        mulc(w.B, M, M, QTO_OP_N, Gln, M, M, QTO_OP_N, w.C, NULL, NULL);
+        for (size_t e = 0; e < nn; ++e) GL[s][e] += w.C[e];
+        /* + Gln W^H GN0^H */
+        mulc(Gln, M, M, QTO_OP_N, W, Mn, M, QTO_OP_C, w.B, NULL, NULL);
+        mulc(w.B, M, Mn, QTO_OP_N, w.GN0, M, Mn, QTO_OP_C, w.C, NULL, NULL);
+        for (size_t e = 0; e < nn; ++e) GL[s][e] += w.C[e];
+        for (int j = 0; j < M; ++j)
+            for (int i = 0; i < M; ++i) {
+                size_t e = i + (size_t)j * M;
+                GG[s][e] = GL[s][e] + (GR[s][e] - conj(GR[s][j + (size_t)i * M]));
+            }
+    }
+
+    /* ---- left contact current, rgf_mod.f90:256-266 */
+    M = p->nm[0];
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            w.A[e] = -(w.sigL[e] - conj(w.sigL[j + (size_t)i * M])) * f_first;
+        }
+    mulc(w.A, M, M, QTO_OP_N, GG[0], M, M, QTO_OP_N, w.B, NULL, NULL);
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            w.A[e] = -(w.sigL[e] - conj(w.sigL[j + (size_t)i * M])) * (f_first - 1.0);
+        }
+    mulc(w.A, M, M, QTO_OP_N, GL[0], M, M, QTO_OP_N, w.C, NULL, NULL);
+    tim = 0.0;
+    for (int j = 0; j < M; ++j) tim += creal(w.B[j + (size_t)j * M] - w.C[j + (size_t)j * M]);
+    r->tre = tim;
+
+    for (int t = 0; t < 3 * nx; ++t) free(own[t]);
+    free(own); free(GL); free(GG); free(GR);
+    free(Sbuf); free(tmp2);
+    work_free(&w);
+    return 0;
+}
+
+int qto_rgf_backward(const qto_point* p, qto_result* r) {
+    const int nx = p->nx;
+    if (nx < 2) return -1;
+    if (!uniform_sizes(p)) return -2;
+    const int M = p->nm[0];
+    const size_t nn = (size_t)M * M;
+    work_t w;
+    work_init(&w, M);
+    qto_c* Sbuf = zalloc(nn);
+    qto_c* sigR = zalloc(nn);
+    const qto_c z = p->En + 0.0 * I; /* rgf_mod.f90:287 */
+    r->info = 0;
+    qto_c** GL = (qto_c**)calloc((size_t)nx, sizeof(qto_c*));
+    qto_c** GG = (qto_c**)calloc((size_t)nx, sizeof(qto_c*));
+    qto_c** GR = (qto_c**)calloc((size_t)nx, sizeof(qto_c*));
+    qto_c** own = (qto_c**)calloc((size_t)3 * nx, sizeof(qto_c*));
+    for (int t = 0; t < nx; ++t) {
+        GR[t] = r->G_r && r->G_r[t] ? r->G_r[t] : (own[3 * t] = zalloc(nn));
+        GL[t] = r->G_lesser && r->G_lesser[t] ? r->G_lesser[t] : (own[3 * t + 1] = zalloc(nn));
+        GG[t] = r->G_greater && r->G_greater[t] ? r->G_greater[t] : (own[3 * t + 2] = zalloc(nn));
+    }
+
+    /* ---- right contact, rgf_mod.f90:289-325 */
+    int s = nx - 1;
+    const qto_c* S = p->Sii ? p->Sii[s] : ident_of(&w, M, Sbuf);
+    build_h00(p, s, M, S, w.H00, w.B);
+    r->sancho_iters[0] = qto_sancho(M, p->En, p->lead_eta, p->lead_nmax, p->lead_tol, 0, S,
+                                    w.H00, p->H1i[nx], w.G00, NULL, NULL);
+    trimulc(p->H1i[nx], M, M, QTO_OP_N, w.G00, M, M, QTO_OP_N, p->H1i[nx], M, M, QTO_OP_C,
+            sigR, w.T);
+    for (size_t e = 0; e < nn; ++e) w.B[e] = z * S[e] - w.H00[e] - sigR[e];
+    note_info(r, qto_invert(w.B, M));
+    memcpy(r->Gl[s], w.B, nn * sizeof(qto_c));
+    const double f_first = qto_ferm((p->En - p->mur) / (BOLTZ * p->tempr));
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i)
+            w.sig[i + (size_t)j * M] =
+                -(sigR[i + (size_t)j * M] - conj(sigR[j + (size_t)i * M])) * f_first;
+    add_sigl(p, s, M, S, w.sig, w.B);
+    trimulc(r->Gl[s], M, M, QTO_OP_N, w.sig, M, M, QTO_OP_N, r->Gl[s], M, M, QTO_OP_C,
+            r->Gln[s], w.T);
+
+    /* ---- first pass r -> l, rgf_mod.f90:330-353 */
+    for (s = nx - 2; s >= 1; --s) {
+        S = p->Sii ? p->Sii[s] : ident_of(&w, M, Sbuf);
+        build_h00(p, s, M, S, w.H00, w.B);
+        trimulc(p->H1i[s + 1], M, M, QTO_OP_N, r->Gl[s + 1], M, M, QTO_OP_N, p->H1i[s + 1],
+                M, M, QTO_OP_C, w.B, w.T);
+        for (size_t e = 0; e < nn; ++e) w.A[e] = z * S[e] - w.H00[e] - w.B[e];
+        note_info(r, qto_invert(w.A, M));
+        memcpy(r->Gl[s], w.A, nn * sizeof(qto_c));
+        trimulc(p->H1i[s + 1], M, M, QTO_OP_N, r->Gln[s + 1], M, M, QTO_OP_N, p->H1i[s + 1],
+                M, M, QTO_OP_C, w.B, w.T);
+        add_sigl(p, s, M, S, w.B, w.A);
+        trimulc(r->Gl[s], M, M, QTO_OP_N, w.B, M, M, QTO_OP_N, r->Gl[s], M, M, QTO_OP_C,
+                r->Gln[s], w.T);
+    }
+
+    /* ---- left contact, rgf_mod.f90:355-415 */
+    s = 0;
+    S = p->Sii ? p->Sii[0] : ident_of(&w, M, Sbuf);
+    build_h00(p, 0, M, S, w.H00, w.B);
+    dagger(p->H1i[0], M, M, w.H10);
+    r->sancho_iters[1] = qto_sancho(M, p->En, p->lead_eta, p->lead_nmax, p->lead_tol, 0, S,
+                                    w.H00, w.H10, w.G00, NULL, NULL);
+    trimulc(p->H1i[0], M, M, QTO_OP_C, w.G00, M, M, QTO_OP_N, p->H1i[0], M, M, QTO_OP_N,
+            w.sigL, w.T);
+    trimulc(p->H1i[1], M, M, QTO_OP_N, r->Gl[1], M, M, QTO_OP_N, p->H1i[1], M, M, QTO_OP_C,
+            w.B, w.T);
+    for (size_t e = 0; e < nn; ++e) w.A[e] = z * S[e] - w.H00[e] - w.B[e] - w.sigL[e];
+    note_info(r, qto_invert(w.A, M));
+    memcpy(GR[0], w.A, nn * sizeof(qto_c));
+    memcpy(r->Gl[0], w.A, nn * sizeof(qto_c));
+    trimulc(p->H1i[1], M, M, QTO_OP_N, r->Gln[1], M, M, QTO_OP_N, p->H1i[1], M, M, QTO_OP_C,
+            w.B, w.T);
+    memset(w.A, 0, nn * sizeof(qto_c));
+    add_sigl(p, 0, M, S, w.A, w.C);
+    const double f_last = qto_ferm((p->En - p->mul) / (BOLTZ * p->templ));
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            w.sig[e] = -(w.sigL[e] - conj(w.sigL[j + (size_t)i * M])) * f_last;
+            w.sig[e] = w.sig[e] + w.A[e] + w.B[e];
+        }
+    trimulc(GR[0], M, M, QTO_OP_N, w.sig, M, M, QTO_OP_N, GR[0], M, M, QTO_OP_C, GL[0], w.T);
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            GG[0][e] = GL[0][e] + (GR[0][e] - conj(GR[0][j + (size_t)i * M]));
+        }
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            w.A[e] = -(w.sigL[e] - conj(w.sigL[j + (size_t)i * M])) * f_last;
+        }
+    mulc(w.A, M, M, QTO_OP_N, GG[0], M, M, QTO_OP_N, w.B, NULL, NULL);
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            w.A[e] = -(w.sigL[e] - conj(w.sigL[j + (size_t)i * M])) * (f_last - 1.0);
+        }
+    mulc(w.A, M, M, QTO_OP_N, GL[0], M, M, QTO_OP_N, w.C, NULL, NULL);
+    if (r->Jdens && r->Jdens[0])
+        for (size_t e = 0; e < nn; ++e) r->Jdens[0][e] = w.B[e] - w.C[e];
+    double tim = 0.0;
+    for (int j = 0; j < M; ++j) tim += creal(w.B[j + (size_t)j * M] - w.C[j + (size_t)j * M]);
+    r->tre = tim;
+
+    /* ---- second pass l -> r, rgf_mod.f90:420-458 */
+    for (s = 1; s <= nx - 1; ++s) {
+        const qto_c* W = p->H1i[s];
+        qto_c* Gl = r->Gl[s];
+        qto_c* Gln = r->Gln[s];
+        trimulc(GL[s - 1], M, M, QTO_OP_N, W, M, M, QTO_OP_N, Gl, M, M, QTO_OP_C, w.A, w.T);
+        trimulc(GR[s - 1], M, M, QTO_OP_N, W, M, M, QTO_OP_N, Gln, M, M, QTO_OP_N, w.B, w.T);
+        for (size_t e = 0; e < nn; ++e) w.A[e] += w.B[e];
+        mulc(W, M, M, QTO_OP_C, w.A, M, M, QTO_OP_N, w.B, NULL, NULL);
+        if (r->Jdens && r->Jdens[s])
+            for (size_t e = 0; e < nn; ++e) r->Jdens[s][e] = -2.0 * w.B[e];
+        mulc(Gl, M, M, QTO_OP_N, W, M, M, QTO_OP_C, w.B, NULL, NULL);
+        mulc(w.B, M, M, QTO_OP_N, GR[s - 1], M, M, QTO_OP_N, w.GN0, NULL, NULL);
+        mulc(w.GN0, M, M, QTO_OP_N, W, M, M, QTO_OP_N, w.C, NULL, NULL);
+        mulc(w.C, M, M, QTO_OP_N, Gl, M, M, QTO_OP_N, w.A, NULL, NULL);
+        for (size_t e = 0; e < nn; ++e) GR[s][e] = Gl[e] + w.A[e];
+        mulc(Gl, M, M, QTO_OP_N, W, M, M, QTO_OP_C, w.B, NULL, NULL);
+        mulc(w.B, M, M, QTO_OP_N, GL[s - 1], M, M, QTO_OP_N, w.C, NULL, NULL);
+        mulc(w.C, M, M, QTO_OP_N, W, M, M, QTO_OP_N, w.A, NULL, NULL);
+        mulc(w.A, M, M, QTO_OP_N, Gl, M, M, QTO_OP_C, w.C, NULL, NULL);
+        for (size_t e = 0; e < nn; ++e) GL[s][e] = Gln[e] + w.C[e];
+        mulc(w.GN0, M, M, QTO_OP_N, W, M, M, QTO_OP_N, w.B, NULL, NULL);
+        mulc(w.B, M, M, QTO_OP_N, Gln, M, M, QTO_OP_N, w.C, NULL, NULL);
+        for (size_t e = 0; e < nn; ++e) GL[s][e] += w.C[e];
+        mulc(Gln, M, M, QTO_OP_N, W, M, M, QTO_OP_C, w.B, NULL, NULL);
+        mulc(w.B, M, M, QTO_OP_N, w.GN0, M, M, QTO_OP_C, w.C, NULL, NULL);
+        for (size_t e = 0; e < nn; ++e) GL[s][e] += w.C[e];
+        for (int j = 0; j < M; ++j)
+            for (int i = 0; i < M; ++i) {
+                size_t e = i + (size_t)j * M;
+                GG[s][e] = GL[s][e] + (GR[s][e] - conj(GR[s][j + (size_t)i * M]));
+            }
+    }
+
+    /* ---- right contact current, rgf_mod.f90:459-469 */
+    s = nx - 1;
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            w.A[e] = -(sigR[e] - conj(sigR[j + (size_t)i * M])) * f_first;
+        }
+    mulc(w.A, M, M, QTO_OP_N, GG[s], M, M, QTO_OP_N, w.B, NULL, NULL);
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            size_t e = i + (size_t)j * M;
+            w.A[e] = -(sigR[e] - conj(sigR[j + (size_t)i * M])) * (f_first - 1.0);
+        }
+    mulc(w.A, M, M, QTO_OP_N, GL[s], M, M, QTO_OP_N, w.C, NULL, NULL);
+    tim = 0.0;
+    for (int j = 0; j < M; ++j) tim += creal(w.B[j + (size_t)j * M] - w.C[j + (size_t)j * M]);
+    r->tr = tim;
+
+    for (int t = 0; t < 3 * nx; ++t) free(own[t]);
+    free(own); free(GL); free(GG); free(GR);
+    free(Sbuf); free(sigR);
+    work_free(&w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* batch driver (CPU baseline), negf_mod.f90:115-127                   */
+/* ------------------------------------------------------------------ */
+
+int qto_rgf_batch(int variant, int nx, const int* nm, int nk, const qto_c* const* Hii,
+                  const qto_c* const* H1i, const qto_c* const* Sii, int npts,
+                  const double* En, const int* kidx, double mul, double mur, double templ,
+                  double tempr, int coupling_fix, int nthreads, double* tr, double* tre,
+                  qto_c* gr_diag, qto_c* gl_diag, qto_c* gg_diag, int* sancho_iters) {
+    (void)nk;
+    size_t ndiag = 0;
+    for (int s = 0; s < nx; ++s) ndiag += (size_t)nm[s];
+    int status = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (int ip = 0; ip < npts; ++ip) {
+        int ik = kidx ? kidx[ip] : 0;
+        qto_c** Gl = (qto_c**)malloc(sizeof(qto_c*) * (size_t)nx);
+        qto_c** Gln = (qto_c**)malloc(sizeof(qto_c*) * (size_t)nx);
+        qto_c** GR = (qto_c**)malloc(sizeof(qto_c*) * (size_t)nx);
+        qto_c** GL = (qto_c**)malloc(sizeof(qto_c*) * (size_t)nx);
+        qto_c** GG = (qto_c**)malloc(sizeof(qto_c*) * (size_t)nx);
+        for (int s = 0; s < nx; ++s) {
+            size_t b = (size_t)nm[s] * nm[s];
+            Gl[s] = zalloc(b); Gln[s] = zalloc(b);
+            GR[s] = zalloc(b); GL[s] = zalloc(b); GG[s] = zalloc(b);
+        }
+        qto_point p;
+        memset(&p, 0, sizeof p);
+        p.nx = nx; p.nm = nm; p.En = En[ip];
+        p.mul = mul; p.mur = mur; p.templ = templ; p.tempr = tempr;
+        p.Hii = Hii + (size_t)ik * nx;
+        p.H1i = H1i + (size_t)ik * (nx + 1);
+        p.Sii = Sii ? Sii + (size_t)ik * nx : NULL;
+        p.lead_eta = 1e-5; p.lead_nmax = 50; p.lead_tol = 1e-100;
+        p.coupling_fix = coupling_fix;
+        qto_result r;
+        memset(&r, 0, sizeof r);
+        r.G_r = GR; r.G_lesser = GL; r.G_greater = GG; r.Gl = Gl; r.Gln = Gln;
+        int st = variant == 0 ? qto_rgf_forward(&p, &r) : qto_rgf_backward(&p, &r);
+        if (st != 0) status = st;
+        if (tr) tr[ip] = r.tr;
+        if (tre) tre[ip] = r.tre;
+        if (sancho_iters) { sancho_iters[2 * ip] = r.sancho_iters[0]; sancho_iters[2 * ip + 1] = r.sancho_iters[1]; }
+        size_t off = (size_t)ip * ndiag;
+        for (int s = 0; s < nx; ++s) {
+            for (int j = 0; j < nm[s]; ++j) {
+                size_t e = j + (size_t)j * nm[s];
+                if (gr_diag) gr_diag[off + j] = GR[s][e];
+                if (gl_diag) gl_diag[off + j] = GL[s][e];
+                if (gg_diag) gg_diag[off + j] = GG[s][e];
+            }
+            off += (size_t)nm[s];
+            free(Gl[s]); free(Gln[s]); free(GR[s]); free(GL[s]); free(GG[s]);
+        }
+        free(Gl); free(Gln); free(GR); free(GL); free(GG);
+    }
+    return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* energy-axis correlation / convolution, fft_mod.f90:58-168           */
+/* ------------------------------------------------------------------ */
+
+/* Z(i) = sum_e X(e) Y(e - l), l = i - n/2 (0-based; the lag-0 element sits
+ * at index n/2).  fft_mod.f90:77-89: X zero-padded, Y reversed, length
+ * 2n-1 circular product, middle n samples kept. */
+void qto_corr1d2(int n, const qto_c* X, const qto_c* Y, qto_c* Z) {
+    for (int i = 0; i < n; ++i) {
+        int l = i - n / 2;
+        qto_c s = 0.0;
+        for (int e = 0; e < n; ++e) {
+            int q = e - l;
+            if (q >= 0 && q < n) s += X[e] * Y[q];
+        }
+        Z[i] = s;
+    }
+}
+
+/* Z(e) = sum_{l=-n/2}^{n/2-1} X(e - l) Y(l + n/2), X = 0 outside [0, n).
+ * fft_mod.f90:153-166: Y rotated so lag 0 is sample 0, X zero-padded. */
+void qto_conv1d2(int n, const qto_c* X, const qto_c* Y, qto_c* Z) {
+    for (int e = 0; e < n; ++e) {
+        qto_c s = 0.0;
+        for (int l = -n / 2; l <= n / 2 - 1; ++l) {
+            int q = e - l;
+            if (q >= 0 && q < n) s += X[q] * Y[l + n / 2];
+        }
+        Z[e] = s;
+    }
+}

@@ -1,0 +1,334 @@
+"""Block-tridiagonal device Hamiltonians: Wannier ``ham_dat`` I/O, the
+``w90_MAT_DEF`` block builder and the deterministic synthetic generator.
+
+Host-side input preparation for the RGF path.  Block conventions follow the
+reference:
+
+* ``Hii[s]`` is the diagonal block of slab ``s`` (0-based here);
+* ``H1i[s]`` is H_{s,s-1}, the coupling of slab ``s`` to slab ``s-1``;
+  ``H1i[0]`` couples the left lead and ``H1i[nx]`` the right lead
+  (deviceHam_mod.f90:278-286, rgf_mod.f90:124-125,157);
+* ``Sii`` is the identity (deviceHam_mod.f90:281-284) and is represented
+  by ``None``.
+
+All blocks are complex128 numpy arrays in Fortran (column-major) order.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_MASK64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    """SplitMix64 finaliser (Steele, Lea, Flood 2014), vectorised."""
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+class SplitMix64:
+    """Counter-based SplitMix64 stream: draw i is mix(seed + (i+1)*gamma).
+
+    Sub-streams are derived with :meth:`stream` so that adding draws to one
+    part of the generator never shifts another part."""
+
+    def __init__(self, seed: int):
+        self.state = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+        self.count = 0
+
+    def stream(self, tag: int) -> "SplitMix64":
+        s = _mix64(np.array([self.state ^ np.uint64(tag & 0xFFFFFFFFFFFFFFFF)], dtype=np.uint64))[0]
+        return SplitMix64(int(s))
+
+    def u64(self, n: int) -> np.ndarray:
+        idx = np.arange(self.count + 1, self.count + n + 1, dtype=np.uint64)
+        self.count += n
+        with np.errstate(over="ignore"):
+            return _mix64(self.state + idx * _GAMMA)
+
+    def uniform(self, n: int) -> np.ndarray:
+        """Doubles in [0, 1) from the top 53 bits."""
+        return (self.u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    def normal(self, n: int) -> np.ndarray:
+        """Box-Muller, one normal per pair of uniforms (cosine branch)."""
+        u1 = self.uniform(n)
+        u2 = self.uniform(n)
+        return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
+
+    def integers(self, n: int, hi: int) -> np.ndarray:
+        return np.minimum((self.uniform(n) * hi).astype(np.int64), hi - 1)
+
+
+# ---------------------------------------------------------------------------
+# Wannier ham_dat
+# ---------------------------------------------------------------------------
+
+
+@dataclasses.dataclass
+class Wannier:
+    """Parsed ``ham_dat`` (wannierHam3d.f90:72-233)."""
+
+    nvb: int
+    spin_deg: int
+    cell: np.ndarray  # 3x3, columns alpha, beta, gamma
+    Hr: np.ndarray  # (nb, nb, nx, ny, nz) complex128
+    xmin: int
+    ymin: int
+    zmin: int
+    centres: np.ndarray  # (3, nb)
+
+    @property
+    def nb(self) -> int:
+        return self.Hr.shape[0]
+
+    @property
+    def xmax(self) -> int:
+        return self.xmin + self.Hr.shape[2] - 1
+
+    @property
+    def ymax(self) -> int:
+        return self.ymin + self.Hr.shape[3] - 1
+
+    @property
+    def zmax(self) -> int:
+        return self.zmin + self.Hr.shape[4] - 1
+
+    def _axes(self):
+        alpha, beta, gamm = self.cell[:, 0], self.cell[:, 1], self.cell[:, 2]
+        xhat = alpha / np.linalg.norm(alpha)
+        yhat = -np.cross(xhat, gamm)
+        yhat = yhat / np.linalg.norm(yhat)
+        zhat = gamm / np.linalg.norm(gamm)
+        return alpha, beta, gamm, xhat, yhat, zhat
+
+    @property
+    def Lx(self) -> float:
+        alpha, _, _, xhat, _, _ = self._axes()
+        return abs(float(np.dot(alpha, xhat)))
+
+    def mat_def(self, kx: float, ky: float, kz: float, ns: int) -> Tuple[np.ndarray, np.ndarray]:
+        """Diagonal and sub-diagonal slab blocks, wannierHam3d.f90:273-301.
+
+        Block (i,k) of Hii sums Hr(x=i-k) e^{-i r.k}; block (i,k) of H1i sums
+        Hr(x=i-k+ns) with the same phase r = (i-k) alpha + j beta + l gamma."""
+        alpha, beta, gamm, xhat, yhat, zhat = self._axes()
+        nb = self.nb
+        kv = kx * xhat + ky * yhat + kz * zhat
+        Hii = np.zeros((nb * ns, nb * ns), dtype=np.complex128, order="F")
+        H1i = np.zeros((nb * ns, nb * ns), dtype=np.complex128, order="F")
+        for i in range(ns):
+            for k in range(ns):
+                for j in range(self.ymin, self.ymax + 1):
+                    for l in range(self.zmin, self.zmax + 1):
+                        r = (i - k) * alpha + j * beta + l * gamm
+                        ph = np.exp(-1j * np.dot(r, kv))
+                        x0 = i - k
+                        if self.xmin <= x0 <= self.xmax:
+                            Hii[i * nb:(i + 1) * nb, k * nb:(k + 1) * nb] += (
+                                self.Hr[:, :, x0 - self.xmin, j - self.ymin, l - self.zmin] * ph)
+                        x1 = i - k + ns
+                        if self.xmin <= x1 <= self.xmax:
+                            H1i[i * nb:(i + 1) * nb, k * nb:(k + 1) * nb] += (
+                                self.Hr[:, :, x1 - self.xmin, j - self.ymin, l - self.zmin] * ph)
+        return Hii, H1i
+
+
+def read_ham_dat(path: str) -> Wannier:
+    """Free-format reader with the record order of wannierHam3d.f90:89-111,
+    147-152 and 204-208."""
+    with open(path, "r") as fh:
+        lines = [ln for ln in fh.read().splitlines()]
+    pos = 0
+
+    def nxt() -> str:
+        nonlocal pos
+        while not lines[pos].strip():
+            pos += 1
+        ln = lines[pos]
+        pos += 1
+        return ln
+
+    nvb, spin = (int(float(t)) for t in nxt().split()[:2])
+    nxt()  # comment
+    cell = np.zeros((3, 3))
+    for c in range(3):
+        cell[:, c] = [float(t) for t in nxt().split()[:3]]
+    nxt()  # comment
+    n = int(nxt().split()[0])
+    rows = np.array([[float(t) for t in nxt().split()[:7]] for _ in range(n)], dtype=np.float64)
+    xs, ys, zs = (np.floor(rows[:, c]).astype(np.int64) for c in range(3))
+    ms, ns_ = np.floor(rows[:, 3]).astype(np.int64), np.floor(rows[:, 4]).astype(np.int64)
+    nb = int(ms.max())
+    xmin, ymin, zmin = int(xs.min()), int(ys.min()), int(zs.min())
+    Hr = np.zeros((nb, nb, xs.max() - xmin + 1, ys.max() - ymin + 1, zs.max() - zmin + 1),
+                  dtype=np.complex128)
+    Hr[ms - 1, ns_ - 1, xs - xmin, ys - ymin, zs - zmin] = rows[:, 5] + 1j * rows[:, 6]
+    nxt()  # comment
+    centres = np.array([[float(t) for t in nxt().split()[:3]] for _ in range(nb)]).T
+    return Wannier(nvb, spin, cell, Hr, xmin, ymin, zmin, centres)
+
+
+def write_ham_dat(path: str, w: Wannier, fmt: str = "%.17g") -> None:
+    """Writes the ``ham_dat`` record layout; values at full precision."""
+    nb = w.nb
+    out = [f"{w.nvb} {w.spin_deg}", "cell"]
+    for c in range(3):
+        out.append(" ".join(fmt % v for v in w.cell[:, c]))
+    out.append("hamiltonian")
+    recs = []
+    Hr = w.Hr
+    for ix in range(Hr.shape[2]):
+        for iy in range(Hr.shape[3]):
+            for iz in range(Hr.shape[4]):
+                blk = Hr[:, :, ix, iy, iz]
+                nzm, nzn = np.nonzero(blk.T)  # column-major walk: n outer, m inner
+                for n_, m_ in zip(nzm, nzn):
+                    v = blk[m_, n_]
+                    recs.append(f"{ix + w.xmin} {iy + w.ymin} {iz + w.zmin} {m_ + 1} {n_ + 1} "
+                                f"{fmt % v.real} {fmt % v.imag}")
+    out.append(str(len(recs)))
+    out.extend(recs)
+    out.append("orbital_location")
+    for i in range(nb):
+        out.append(" ".join(fmt % v for v in w.centres[:, i]))
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+
+
+# ---------------------------------------------------------------------------
+# devH_build_fromWannierFile equivalent
+# ---------------------------------------------------------------------------
+
+
+@dataclasses.dataclass
+class Device:
+    """Block lists for nk k-points: Hii[ik][s], H1i[ik][s] (nx+1 blocks)."""
+
+    nm: List[int]
+    Hii: List[List[np.ndarray]]
+    H1i: List[List[np.ndarray]]
+    Sii: Optional[List[List[np.ndarray]]] = None
+    Lx: float = 1.0
+    nb: int = 0
+    ns: int = 1
+
+    @property
+    def nx(self) -> int:
+        return len(self.nm)
+
+    @property
+    def nk(self) -> int:
+        return len(self.Hii)
+
+
+def device_from_wannier(w: Wannier, nx: int, ns: int, kpts: Sequence[Tuple[float, float]]) -> Device:
+    """deviceHam_mod.f90:234-290: every slab gets the same H00/H10 at kx=0,
+    k = (ky, kz) per k-point; Sii is the identity."""
+    Hii, H1i = [], []
+    for ky, kz in kpts:
+        H00, H10 = w.mat_def(0.0, ky, kz, ns)
+        Hii.append([H00] * nx)
+        H1i.append([H10] * (nx + 1))
+    return Device([w.nb * ns] * nx, Hii, H1i, None, w.Lx, w.nb, ns)
+
+
+def energy_grid(emin: float, emax: float, nen: int) -> np.ndarray:
+    """negf_mod.f90:100-101."""
+    if nen == 1:
+        return np.array([emin])
+    dE = (emax - emin) / float(nen - 1)
+    return emin + np.arange(nen, dtype=np.float64) * dE
+
+
+# ---------------------------------------------------------------------------
+# synthetic generator
+# ---------------------------------------------------------------------------
+
+HOP_T = -2.7  # eV
+
+
+def _hop(rng: SplitMix64, n: int) -> np.ndarray:
+    return HOP_T * (1.0 + 0.1 * rng.normal(n))
+
+
+def _intra_block(rng: SplitMix64, nb: int, partners: int = 4) -> np.ndarray:
+    """Real symmetric block: diagonal U[-1,1] eV; each site draws
+    ``partners`` random partners (about 2*partners bonds per site), hopping
+    t = -2.7 (1 + 0.1 N(0,1)) eV."""
+    H = np.zeros((nb, nb), dtype=np.complex128)
+    H[np.arange(nb), np.arange(nb)] = 2.0 * rng.stream(1).uniform(nb) - 1.0
+    pr = rng.stream(2)
+    hv = rng.stream(3)
+    for i in range(nb):
+        js = pr.integers(partners, nb - 1)
+        js = js + (js >= i)  # skip self
+        ts = _hop(hv, partners)
+        for j, t in zip(js, ts):
+            H[i, j] = t
+            H[j, i] = t
+    return H
+
+
+def _sparse_block(rng: SplitMix64, nr: int, nc: int, density: float) -> np.ndarray:
+    mask = rng.stream(1).uniform(nr * nc).reshape(nc, nr).T < density
+    vals = _hop(rng.stream(2), nr * nc).reshape(nc, nr).T
+    return np.where(mask, vals, 0.0).astype(np.complex128)
+
+
+def synthetic_wannier(nb: int, seed: int, tz_density: float = 0.0,
+                      coupling_density: float = 0.05, a: float = 3.0) -> Wannier:
+    """Wannier model of the synthetic device (one cell per slab).
+
+    x in {-1,0,1}: Hr(x=0) = H_ii, Hr(x=-1) = H_{i,i+1} (random sparse at
+    ``coupling_density``), Hr(x=+1) = H_{i,i+1}^H.  With ``tz_density`` > 0 a
+    transverse z neighbour is added: Hr(z=-1) = T_z, Hr(z=+1) = T_z^H, so that
+    H_ii(kz) = H_ii + T_z e^{i kz} + h.c. (|gamma| = 1).  Orbital centres lie
+    on a 3 Angstrom grid."""
+    rng = SplitMix64(seed)
+    H0 = _intra_block(rng.stream(10), nb)
+    Hup = _sparse_block(rng.stream(20), nb, nb, coupling_density)
+    nz = 3 if tz_density > 0 else 1
+    Hr = np.zeros((nb, nb, 3, 1, nz), dtype=np.complex128)
+    zc = 1 if nz == 3 else 0
+    Hr[:, :, 1, 0, zc] = H0
+    Hr[:, :, 0, 0, zc] = Hup
+    Hr[:, :, 2, 0, zc] = Hup.conj().T
+    if nz == 3:
+        Tz = _sparse_block(rng.stream(30), nb, nb, tz_density)
+        Hr[:, :, 1, 0, 0] = Tz
+        Hr[:, :, 1, 0, 2] = Tz.conj().T
+    cell = np.diag([a, 40.0, 1.0])
+    side = int(math.ceil(nb ** (1.0 / 3.0)))
+    grid = np.array([(i % side, (i // side) % side, i // (side * side)) for i in range(nb)],
+                    dtype=np.float64).T * 3.0
+    return Wannier(0, 1, cell, Hr, -1, 0, -(nz // 2), grid)
+
+
+def synthetic_device(nb: int, nx: int, seed: int, kz: Sequence[float] = (0.0,),
+                     tz_density: float = 0.0, coupling_density: float = 0.05) -> Tuple[Wannier, Device]:
+    w = synthetic_wannier(nb, seed, tz_density=tz_density, coupling_density=coupling_density)
+    return w, device_from_wannier(w, nx, 1, [(0.0, k) for k in kz])
+
+
+def uneven_device(sizes: Sequence[int], seed: int, coupling_density: float = 0.05) -> Device:
+    """Device with slab sizes ``sizes`` built block by block (the direct /
+    COO route: the Wannier format cannot express uneven slabs).  The leads
+    repeat the first and last slab; H1i[s] is nm[s] x nm[s-1]."""
+    rng = SplitMix64(seed)
+    nx = len(sizes)
+    Hii = [_intra_block(rng.stream(100 + s), n) for s, n in enumerate(sizes)]
+    H1i = [_sparse_block(rng.stream(1000), sizes[0], sizes[0], coupling_density).conj().T]
+    for s in range(1, nx):
+        H1i.append(_sparse_block(rng.stream(1000 + s), sizes[s], sizes[s - 1], coupling_density))
+    H1i.append(_sparse_block(rng.stream(1000 + nx), sizes[-1], sizes[-1], coupling_density).conj().T)
+    Hii = [np.asfortranarray(h) for h in Hii]
+    H1i = [np.asfortranarray(h) for h in H1i]
+    return Device(list(sizes), [Hii], [H1i], None, 1.0, 0, 1)

@@ -1,0 +1,313 @@
+"""Pins the CPU oracle before it is trusted as the checker of the HIP path.
+
+Each restated routine is checked against something that does not repeat its
+own formula: the reference's golden outputs in tests/golden (test_cnt),
+LAPACK-level identities, closed forms, dense brute force and invariants."""
+import numpy as np
+import pytest
+
+from quantum_transport_amd import hamiltonian as hm
+from tests import _dense
+
+BOLTZ = 8.61734e-05
+
+
+@pytest.fixture(scope="module")
+def cnt(golden_dir):
+    w = hm.read_ham_dat(f"{golden_dir}/ham_dat")
+    return w
+
+
+# --------------------------------------------------------------- BLAS/LAPACK
+
+
+@pytest.mark.parametrize("opa", [0, 1, 2])
+@pytest.mark.parametrize("opb", [0, 1, 2])
+def test_zgemm_ops(oracle_mod, opa, opb):
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((7, 5)) + 1j * rng.standard_normal((7, 5))
+    B = rng.standard_normal((5, 6)) + 1j * rng.standard_normal((5, 6))
+    opf = [lambda x: x, lambda x: x.T, lambda x: x.conj().T]
+    Aop = A if opa == 0 else opf[opa](A.T.copy()).T.copy()  # stored so that op(stored) = A
+    Aop = {0: A, 1: A.T.copy(), 2: A.conj().T.copy()}[opa]
+    Bop = {0: B, 1: B.T.copy(), 2: B.conj().T.copy()}[opb]
+    for fl in ("plain", "mkl") if oracle_mod.available("mkl") else ("plain",):
+        R = oracle_mod.zgemm(opa, opb, Aop, Bop, flavour=fl)
+        np.testing.assert_allclose(R, A @ B, rtol=1e-13, atol=1e-13)
+
+
+def test_invert_identity_and_singular(oracle_mod):
+    rng = np.random.default_rng(5)
+    n = 37
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    Ai, info = oracle_mod.invert(A)
+    assert info == 0
+    np.testing.assert_allclose(A @ Ai, np.eye(n), atol=1e-11)
+    # a zero column makes ZGETRF report info = column index (1-based) and
+    # linalg.f90:50-53 zeroes the matrix
+    A[:, 4] = 0.0
+    Ai, info = oracle_mod.invert(A)
+    assert info == 5
+    assert np.all(Ai == 0)
+
+
+def test_invert_pivoting_needed(oracle_mod):
+    # zero leading pivot: only partial pivoting makes this solvable
+    A = np.array([[0, 1, 2], [1, 0, 3], [4, -3, 8]], dtype=np.complex128)
+    Ai, info = oracle_mod.invert(A)
+    assert info == 0
+    np.testing.assert_allclose(Ai, np.linalg.inv(A), rtol=1e-14, atol=1e-14)
+
+
+def test_ferm(oracle_mod):
+    assert oracle_mod.ferm(0.0) == 0.5
+    x = 3.7
+    assert abs(oracle_mod.ferm(x) + oracle_mod.ferm(-x) - 1.0) < 1e-15
+
+
+# --------------------------------------------------------------- Sancho-Rubio
+
+
+@pytest.mark.parametrize("E", [-1.3, 0.0, 0.7, 2.5, 3.1])
+def test_sancho_chain_closed_form(oracle_mod, E):
+    eps, t = 0.2, -1.0
+    z = E + 1e-5j
+    G00, GBB, it, conv = oracle_mod.sancho(None, np.array([[eps]]), np.array([[t]]), E)
+    assert conv and it < 50
+    g = _dense.chain_surface_g(z, eps, t)
+    assert abs(G00[0, 0] - g) <= 1e-7 * abs(g)
+
+
+def test_sancho_surface_equation(oracle_mod, cnt):
+    """g00 solves g = (z - H00 - H01 g H10)^-1 with H01 = H10^H."""
+    H00, H10 = cnt.mat_def(0, 0, 0, 3)
+    for E, H in [(-3.0, H10.conj().T), (-2.2, H10)]:
+        z = E + 1e-5j
+        G, _, it, conv = oracle_mod.sancho(None, H00, H, E)
+        H01 = H.conj().T
+        R = np.linalg.inv(z * np.eye(96) - H00 - H01 @ G @ H)
+        assert np.abs(R - G).max() <= 1e-8 * np.abs(G).max()
+
+
+def test_sancho_golden(oracle_mod, flavour, golden_dir, cnt):
+    """test_cnt/sancho_g00.dat and sancho_gbb.dat: E, lead tag, -Im Tr G at
+    full precision (rgf_mod.f90:82-89 left lead = tag 2 with H10^H passed,
+    :163-170 right lead = tag 1), nm = 96, eta = 1e-5, nmax = 50."""
+    H00, H10 = cnt.mat_def(0, 0, 0, 3)
+    g00 = np.loadtxt(f"{golden_dir}/sancho_g00.dat")
+    gbb = np.loadtxt(f"{golden_dir}/sancho_gbb.dat")
+    stride = 1 if flavour == "mkl" else 20
+    errs = []
+    for r0, rb in list(zip(g00, gbb))[::stride]:
+        E, tag = r0[0], int(r0[1])
+        assert rb[0] == E and int(rb[1]) == tag
+        H = H10.conj().T if tag == 2 else H10
+        G, GB, it, conv = oracle_mod.sancho(None, H00, H, E, flavour=flavour)
+        assert conv
+        errs.append(abs(-np.trace(G).imag - r0[2]) / abs(r0[2]))
+        errs.append(abs(-np.trace(GB).imag - rb[2]) / abs(rb[2]))
+    errs = np.array(errs)
+    # The reference's LAPACK is unknown.  At a few energies (e.g. E=-3.387)
+    # the 96x96 inversions are ill-conditioned at eta=1e-5 and two LAPACKs
+    # (this plain restatement vs the vendor one) already differ by ~5e-9
+    # there; the bulk agrees to ~1e-13.
+    assert errs.max() < 1e-8
+    assert np.percentile(errs, 90) < 1e-10
+    assert np.median(errs) < 1e-12
+
+
+# --------------------------------------------------------------- forward RGF
+
+
+def _cell_traces(diag, nx, ns, nb):
+    return diag.reshape(len(diag), nx, ns, nb).sum(-1).reshape(len(diag), nx * ns)
+
+
+def _sig4(a):
+    return np.array([float("%.3E" % v) for v in np.ravel(a)]).reshape(np.shape(a))
+
+
+def test_rgf_forward_golden_ldos(oracle_mod, flavour, golden_dir, cnt):
+    """test_cnt/{ldos,ndos,pdos}0000.dat: forward RGF, nx=5 slabs of ns=3
+    cells (nm=96), mu=-3.7 eV, T=300 K, 200 energies in [-6,-1]; written by
+    output_mod.f90:37-59 as (4E18.4) with coefficients (1,-2), (1,1), (1,-1)
+    (negf_mod.f90:143,146,149)."""
+    nx, ns = 5, 3
+    dev = hm.device_from_wannier(cnt, nx, ns, [(0.0, 0.0)])
+    E = hm.energy_grid(-6.0, -1.0, 200)
+    sel = np.arange(200) if flavour == "mkl" else np.array([0, 77, 150, 199])
+    res = oracle_mod.rgf_batch(dev, E[sel], None, -3.7, -3.7, flavour=flavour)
+    for name, key, coef in (("ldos", "gr_diag", -2.0), ("ndos", "gl_diag", 1.0),
+                            ("pdos", "gg_diag", -1.0)):
+        ref = np.loadtxt(f"{golden_dir}/{name}0000.dat").reshape(200, nx * ns, 4)[sel]
+        np.testing.assert_allclose(ref[:, :, 1], np.repeat(E[sel][:, None], nx * ns, 1), atol=1e-3)
+        np.testing.assert_allclose(ref[:, :, 0], np.tile(np.arange(1, 16) * cnt.Lx, (len(sel), 1)),
+                                   rtol=1e-3)
+        ours = _cell_traces(res[key], nx, ns, cnt.nb).imag * coef
+        scale = np.abs(ref[:, :, 3]).max()
+        big = np.abs(ref[:, :, 3]) > 1e-6 * scale
+        assert big.sum() > 0.3 * big.size
+        # every printed significant value matches at 4 digits
+        assert np.array_equal(_sig4(ours)[big], ref[:, :, 3][big]), name
+        # entries the reference prints as rounding noise are noise here too
+        assert np.all(np.abs(ours[~big]) < 1e-6 * scale)
+
+
+# --------------------------------------------------------------- brute force
+
+
+def _leads(oracle_mod, Hii0, H1i, E, lead_eta=1e-5):
+    M = Hii0.shape[0]
+    G_L, _, _, _ = oracle_mod.sancho(None, Hii0, H1i[0].conj().T, E, eta=lead_eta)
+    G_R, _, _, _ = oracle_mod.sancho(None, Hii0, H1i[-1], E, eta=lead_eta)
+    sigL = H1i[0] @ G_L @ H1i[0].conj().T
+    sigR = H1i[-1].conj().T @ G_R @ H1i[-1]
+    return sigL, sigR
+
+
+@pytest.mark.parametrize("E", [-0.22, -0.1])
+def test_rgf_forward_dense(oracle_mod, E):
+    """Uniform device: every diagonal block of G, G^<, G^> equals the dense
+    inverse of the whole device, with the contacts' self-energies; the
+    spectral currents tr/tre equal the Caroli transmission times the Fermi
+    window (and are opposite).  Energies sit inside the bias window so the
+    currents are not rounding noise."""
+    _, dev = hm.synthetic_device(8, 5, seed=11, coupling_density=0.4)
+    Hii, H1i = dev.Hii[0], dev.H1i[0]
+    mul, mur, T = 0.0, -0.3, 300.0
+    out = oracle_mod.rgf(Hii, H1i, E, mul, mur, T, T)
+    sigL, sigR = _leads(oracle_mod, Hii[0], H1i, E)
+    # rgf_mod.f90:100,182: the left contact is filled with f(E-mur, TEMPr),
+    # the right one with f(E-mul, TEMPl)
+    fL = _dense.ferm((E - mur) / (BOLTZ * T))
+    fR = _dense.ferm((E - mul) / (BOLTZ * T))
+    d = _dense.dense_negf(Hii, H1i, E, sigL, sigR, fL, fR)
+    for key in ("G_r", "G_lesser", "G_greater"):
+        for s in range(5):
+            # G^> = G^< + G - G^H cancels to rounding noise where f ~ 1: the
+            # error is measured on the scale of the spectral function
+            ref = d[key][s]
+            scale = max(np.abs(ref).max(), np.abs(d["G_r"][s]).max())
+            assert np.abs(out[key][s] - ref).max() <= 1e-10 * scale, (key, s)
+    Tw = d["T"] * (fL - fR)
+    assert abs(Tw) > 1e-3
+    assert abs(out["tre"] + out["tr"]) <= 1e-9 * abs(Tw)
+    assert abs(abs(out["tr"]) - abs(Tw)) <= 1e-9 * abs(Tw)
+    # G^< is anti-Hermitian, G^> - G^< = G - G^H
+    for s in range(5):
+        Gl = out["G_lesser"][s]
+        assert np.abs(Gl + Gl.conj().T).max() <= 1e-10 * np.abs(Gl).max()
+
+
+def test_rgf_forward_uneven_coupling_fix(oracle_mod):
+    """Uneven slabs need H1i(i+1) in the second pass (coupling_fix); the dense
+    device inverse pins every diagonal block."""
+    sizes = [6, 9, 5, 7, 6]
+    dev = hm.uneven_device(sizes, seed=3, coupling_density=0.5)
+    # leads repeat the first / last slab: make the outer couplings square
+    Hii, H1i = dev.Hii[0], dev.H1i[0]
+    E = 0.4
+    out = oracle_mod.rgf(Hii, H1i, E, 0.1, -0.2, 300.0, 300.0, coupling_fix=True)
+    G_L, _, _, _ = oracle_mod.sancho(None, Hii[0], H1i[0].conj().T, E)
+    G_R, _, _, _ = oracle_mod.sancho(None, Hii[-1], H1i[-1], E)
+    sigL = H1i[0] @ G_L @ H1i[0].conj().T
+    sigR = H1i[-1].conj().T @ G_R @ H1i[-1]
+    fL = _dense.ferm((E - -0.2) / (BOLTZ * 300.0))
+    fR = _dense.ferm((E - 0.1) / (BOLTZ * 300.0))
+    d = _dense.dense_negf(Hii, H1i, E, sigL, sigR, fL, fR)
+    for key in ("G_r", "G_lesser", "G_greater"):
+        for s in range(len(sizes)):
+            ref = d[key][s]
+            scale = max(np.abs(ref).max(), np.abs(d["G_r"][s]).max())
+            assert np.abs(out[key][s] - ref).max() <= 1e-10 * scale, (key, s)
+
+
+def test_rgf_reference_index_breaks_nonuniform(oracle_mod):
+    """The reference second pass uses H1i(ii) (rgf_mod.f90:215-245); with
+    non-identical couplings this differs from the device inverse, which is
+    why the GPU path exposes the corrected index as an option."""
+    _, dev = hm.synthetic_device(6, 4, seed=2, coupling_density=0.5)
+    Hii = dev.Hii[0]
+    H1i = [h.copy() for h in dev.H1i[0]]
+    H1i[2] = 1.7 * H1i[2]
+    E = 0.3
+    a = oracle_mod.rgf(Hii, H1i, E, 0.0, 0.0, coupling_fix=False)
+    b = oracle_mod.rgf(Hii, H1i, E, 0.0, 0.0, coupling_fix=True)
+    sigL, sigR = _leads(oracle_mod, Hii[0], H1i, E)
+    d = _dense.dense_negf(Hii, H1i, E, sigL, sigR, 0.5, 0.5)
+    assert np.abs(b["G_r"][1] - d["G_r"][1]).max() < 1e-10 * np.abs(d["G_r"][1]).max()
+    assert np.abs(a["G_r"][1] - d["G_r"][1]).max() > 1e-6 * np.abs(d["G_r"][1]).max()
+
+
+def test_rgf_backward_matches_forward_for_hermitian_coupling(oracle_mod):
+    """For a Hermitian inter-slab coupling both variants describe the same
+    device; the backward variant fills the right contact with (mur, TEMPr)
+    (rgf_mod.f90:321) and the forward one the left contact, so with equal
+    mu the diagonal blocks and currents agree."""
+    rng = np.random.default_rng(9)
+    n = 6
+    Hii0 = rng.standard_normal((n, n))
+    Hii0 = Hii0 + Hii0.T
+    T = rng.standard_normal((n, n)) * 0.6
+    T = (T + T.T).astype(np.complex128)
+    Hii = [Hii0.astype(np.complex128)] * 4
+    H1i = [T] * 5
+    E = 0.9
+    a = oracle_mod.rgf(Hii, H1i, E, -0.1, -0.1, variant="forward")
+    b = oracle_mod.rgf(Hii, H1i, E, -0.1, -0.1, variant="backward")
+    for key in ("G_r", "G_lesser", "G_greater"):
+        for s in range(4):
+            np.testing.assert_allclose(a[key][s], b[key][s], rtol=0, atol=1e-9 * np.abs(a[key][s]).max())
+
+
+def test_plain_and_vendor_lapack_agree(oracle_mod):
+    if not oracle_mod.available("mkl"):
+        pytest.skip("vendor LAPACK flavour not built")
+    _, dev = hm.synthetic_device(16, 6, seed=4, kz=(0.0, 0.8), tz_density=0.1)
+    E = np.array([-2.0, -0.5, 0.25, 1.5])
+    k = np.array([0, 1, 0, 1])
+    a = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour="plain")
+    b = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour="mkl")
+    # two LAPACKs on the same inputs differ by ~1e-10 here (rounding of
+    # the Schur-complement inversions, amplified by their condition): this is
+    # the floor any FP64 implementation is measured against
+    for key in ("gr_diag", "gl_diag", "gg_diag"):
+        assert np.abs(a[key] - b[key]).max() <= 1e-9 * np.abs(b[key]).max()
+    np.testing.assert_allclose(a["tr"], b["tr"], rtol=1e-8, atol=1e-12)
+
+
+# --------------------------------------------------------------- fft_mod
+
+
+def _fft_branch_corr(X, Y):
+    """fft_mod.f90:77-89 step by step with numpy's DFT."""
+    n = len(X)
+    L = 2 * n - 1
+    Xp = np.zeros(L, complex)
+    Yp = np.zeros(L, complex)
+    Xp[:n] = X
+    Yp[:n] = Y[::-1]
+    Zp = np.fft.ifft(np.fft.fft(Xp) * np.fft.fft(Yp))
+    return Zp[n - n // 2 - 1:n + n // 2 - 1]
+
+
+def _fft_branch_conv(X, Y):
+    """fft_mod.f90:153-166 step by step with numpy's DFT."""
+    n = len(X)
+    L = 2 * n - 1
+    Xp = np.zeros(L, complex)
+    Yp = np.zeros(L, complex)
+    Xp[:n] = X
+    Yp[:n // 2] = Y[n // 2:]
+    Yp[L - n // 2:] = Y[:n // 2]
+    Zp = np.fft.ifft(np.fft.fft(Yp) * np.fft.fft(Xp))
+    return Zp[:n]
+
+
+@pytest.mark.parametrize("n", [2, 8, 64])
+def test_corr_conv_match_fft_branch(oracle_mod, n):
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    Y = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    np.testing.assert_allclose(oracle_mod.corr1d2(X, Y), _fft_branch_corr(X, Y), atol=1e-12)
+    np.testing.assert_allclose(oracle_mod.conv1d2(X, Y), _fft_branch_conv(X, Y), atol=1e-12)
