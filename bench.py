@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""RGF throughput at the production shape on one MI355X.
+
+Workload: block-tridiagonal device of 100 identical slabs with nb=512
+orbitals each, 4 kz points (H_ii(kz) = H_ii + T_z e^{ikz} + h.c.), energies
+on the 1024-point grid in [-2, 2] eV, Fermi contacts mu_L=0, mu_R=-0.1 eV,
+T=300 K; both Sancho-Rubio leads and the full forward + backward sweep
+(G^R, G^<, G^> diagonals, contact currents) per energy point, complex128.
+
+A step is one device batch of --batch energy points (the (E, kz) pairs are
+taken in order from the 1024 x 4 grid).  Inputs (Hamiltonian blocks) are
+resident in HBM before the timed region; outputs stay on the device.
+
+The JSON line carries the roofline of the dominant kernel (the batched
+complex GEMM on the FP64 matrix cores: algorithmic 8mnk flops per launch over
+the launch durations measured with HIP events on the library stream) and a
+CPU baseline (the FP64 oracle on the vendor LAPACK, one energy point per
+thread).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RGF energy-points/s @nb=512×100 slabs: FP64 vs mixed+warm-start @≤1e-8 err"
+PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+
+
+def cpu_baseline(nb: int, slabs: int, threads: int, seed: int) -> dict:
+    """Times the oracle (vendor LAPACK flavour, one point per OpenMP thread)
+    on a bounded sample and extrapolates to the full slab count: the cost of
+    a point is lead + slabs * slab, with slab and lead separated from runs at
+    2 and 6 slabs."""
+    import oracle
+    from quantum_transport_amd import hamiltonian as hm
+
+    flavour = "mkl" if oracle.available("mkl") else "plain"
+    kz = [0.0, np.pi / 4, np.pi / 2, 3 * np.pi / 4]
+    w = hm.synthetic_wannier(nb, seed, tz_density=0.02)
+    E = hm.energy_grid(-2.0, 2.0, 1024)[:: max(1, 1024 // threads)][:threads]
+    k = np.arange(threads) % 4
+    times = {}
+    for nx in (2, 6):
+        dev = hm.device_from_wannier(w, nx, 1, [(0.0, q) for q in kz])
+        t0 = time.perf_counter()
+        oracle.rgf_batch(dev, E, k, 0.0, -0.1, nthreads=threads, flavour=flavour)
+        times[nx] = time.perf_counter() - t0
+    slab = max(times[6] - times[2], 0.0) / 4.0
+    lead = max(times[2] - 2 * slab, 0.0)
+    t_full = lead + slabs * slab  # wall time of `threads` points in parallel
+    return {
+        "value": threads / t_full,
+        "unit": "energy points/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{threads} energy points (one per thread, {flavour} ZGEMM/ZGETRF/ZGETRI) at "
+                   f"nb={nb}, timed at 2 and 6 slabs ({times[2]:.1f}s, {times[6]:.1f}s) and "
+                   f"extrapolated to {slabs} slabs (lead {lead:.1f}s + {slabs} x {slab:.2f}s)"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=64, help="energy points per step")
+    ap.add_argument("--nb", type=int, default=512)
+    ap.add_argument("--slabs", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=1002)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: the RGF path is measured per GPU "
+              "(energy points shard across GPUs with no communication but a final sum); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        return 2
+
+    from quantum_transport_amd import hamiltonian as hm
+    from quantum_transport_amd import rgf
+
+    nb, nx, B = args.nb, args.slabs, args.batch
+    kz = [0.0, np.pi / 4, np.pi / 2, 3 * np.pi / 4]
+    w = hm.synthetic_wannier(nb, args.seed, tz_density=0.02)
+    dev = hm.device_from_wannier(w, nx, 1, [(0.0, q) for q in kz])
+    grid = hm.energy_grid(-2.0, 2.0, 1024)
+    Eall = np.repeat(grid, 4)
+    Kall = np.tile(np.arange(4, dtype=np.int32), 1024)
+
+    rgf.init(0)
+    H = rgf.Hamiltonian.from_device(dev)
+    del dev
+    nd = nb * nx
+    bufs = {"tr": rgf.DeviceBuffer(8 * B), "tre": rgf.DeviceBuffer(8 * B)}
+    for key in ("gr_diag", "gl_diag", "gg_diag"):
+        bufs[key] = rgf.DeviceBuffer(16 * B * nd)
+
+    cursor = [0]
+
+    def step():
+        i0 = cursor[0] % len(Eall)
+        idx = (np.arange(B) + i0) % len(Eall)
+        cursor[0] += B
+        rgf.solve_batch_device(H, Eall[idx], Kall[idx], bufs, mul=0.0, mur=-0.1)
+
+    for _ in range(args.warmup):
+        step()
+    rgf.lib().qt_synchronize()
+    rgf.set_kernel_timing(True)
+    acc = {"gemm_ms": 0.0, "gemm_flops": 0.0, "inv_ms": 0.0, "inv_flops": 0.0, "gemm_launches": 0,
+           "lead_iterations": 0, "lead_ms": 0.0, "forward_ms": 0.0, "backward_ms": 0.0}
+    rgf.lib().qt_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        t = rgf.last_timing()
+        for k in acc:
+            acc[k] += t[k]
+    rgf.lib().qt_synchronize()
+    elapsed = time.perf_counter() - t0
+    rgf.set_kernel_timing(False)
+    points = B * args.steps
+    value = points / elapsed
+
+    gemm_tf = acc["gemm_flops"] / (acc["gemm_ms"] * 1e-3) / 1e12 if acc["gemm_ms"] > 0 else 0.0
+    inv_tf = acc["inv_flops"] / (acc["inv_ms"] * 1e-3) / 1e12 if acc["inv_ms"] > 0 else 0.0
+    K = acc["lead_iterations"] / max(1, 2 * points)
+    n3 = float(nb) ** 3
+    lead_flops = (K * 56 + 24) * n3  # per lead: K (INV + 6 GEMM) + INV + 2 GEMM
+    path_flops = nx * 120 * n3 + 2 * lead_flops
+    res = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "energy points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 generator)",
+        "config": {"workload": f"ballistic RGF, nb={nb} x {nx} slabs, 4 kz x 1024 E in [-2,2] eV, "
+                               f"G^R/G^</G^> diagonals + contact currents, plain FP64",
+                   "nb": nb, "slabs": nx, "kz": 4, "energies": 1024, "points_per_step": B,
+                   "precision": "fp64"},
+        "roofline": {"bound": "mfma", "achieved": round(gemm_tf, 3), "peak": PEAK_FP64,
+                     "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4), "traffic": None,
+                     "kernel": "zgemm_mfma_kernel (batched complex128 GEMM)",
+                     "launches": acc["gemm_launches"]},
+        "path": {"lead_iterations_avg": round(K, 2),
+                 "algorithmic_tflops": round(points * path_flops / elapsed / 1e12, 3),
+                 "inversion_tflops": round(inv_tf, 3),
+                 "gemm_share_of_wall": round(acc["gemm_ms"] / (1e3 * elapsed), 3),
+                 "inversion_share_of_wall": round(acc["inv_ms"] / (1e3 * elapsed), 3),
+                 "lead_ms": round(acc["lead_ms"], 1), "forward_ms": round(acc["forward_ms"], 1),
+                 "backward_ms": round(acc["backward_ms"], 1)},
+        "cpu_baseline": None,
+    }
+    for b in bufs.values():
+        b.free()
+    H.close()
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        res["cpu_baseline"] = cpu_baseline(nb, nx, threads, args.seed)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

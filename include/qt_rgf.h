@@ -1,0 +1,193 @@
+/*
+ * qt_rgf.h -- C ABI of the MI355X recursive Green's function (RGF) library.
+ *
+ * This is the drop-in boundary for the reference's RGF path.  Each entry
+ * point names the reference routine it replaces; the Fortran modules in
+ * quantum_transport_amd/fortran bind these symbols through iso_c_binding
+ * and keep the reference subroutine interfaces.
+ *
+ * Data conventions (binary-compatible with Fortran complex(8) arrays):
+ *   - every matrix block is column-major complex128, interleaved re/im,
+ *     leading dimension = its row count;
+ *   - block lists are arrays of pointers, one per block, the way the
+ *     reference passes type_matrix_complex lists;
+ *   - slab s is 0-based; H1i[s] is H_{s,s-1} (H1i[0]: left lead coupling,
+ *     H1i[nx]: right lead coupling), deviceHam_mod.f90:278-286;
+ *   - pointers are host pointers unless QT_F_DEVICE_OUTPUTS says otherwise.
+ * Every function returns a qt status code (0 = success).
+ * The library is re-entrant per call but serialises calls on one device
+ * internally (one HIP stream per device context).
+ */
+#ifndef QT_RGF_H
+#define QT_RGF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+    QT_OK = 0,
+    QT_ERR_INVALID = 1,      /* bad argument or shape */
+    QT_ERR_NOT_INIT = 2,     /* qt_init() not called */
+    QT_ERR_HIP = 3,          /* HIP runtime failure */
+    QT_ERR_NOMEM = 4,        /* device memory exhausted */
+    QT_ERR_SINGULAR = 5,     /* a block inversion met an exactly zero pivot;
+                                that block was zeroed as linalg.f90:50-58 does */
+    QT_ERR_UNSUPPORTED = 6,  /* shape / option combination not provided */
+    QT_ERR_NO_DEVICE = 7     /* no HIP device visible */
+};
+
+/* per-point info bits (qt_rgf_outputs.info) */
+#define QT_INFO_SINGULAR 0x1   /* some inversion of this point was singular */
+#define QT_INFO_LEAD_NMAX 0x2  /* a lead decimation reached nmax (warning,
+                                  rgf_mod.f90:549-551) */
+
+/* RGF variants */
+#define QT_RGF_FORWARD 0   /* rgf_variableblock_forward,  rgf_mod.f90:46-269 */
+#define QT_RGF_BACKWARD 1  /* rgf_variableblock_backward, rgf_mod.f90:272-472 */
+
+/* precision modes */
+#define QT_PREC_FP64 0     /* complex128 everywhere */
+
+/* option flags (qt_rgf_params.flags) */
+#define QT_F_COUPLING_FIX 0x1   /* forward variant: the second pass couples slab
+                                   s to s+1 through H1i[s+1] (the reference
+                                   uses H1i[s], rgf_mod.f90:215-245, equal for
+                                   uniform devices; required for uneven slabs) */
+#define QT_F_DEVICE_OUTPUTS 0x2 /* the per-point output arrays of
+                                   qt_rgf_outputs are device pointers */
+#define QT_F_LEAD_RESET 0x4     /* lead decimation resets H_SS=H_BB=H00 at nmax
+                                   (gw_mod.f90:1440-1445) */
+
+/* ---- context --------------------------------------------------------- */
+
+/* Replaces cuda_rgf_init (cuda_rgf_mod.f90:32-44): binds the calling
+ * process to HIP device `device`, creates its stream and workspace. */
+int qt_init(int device);
+/* Replaces cuda_rgf_finish (cuda_rgf_mod.f90:47-51): frees all device
+ * memory held by the library. */
+int qt_finish(void);
+const char* qt_status_string(int status);
+
+/* ---- device-resident Hamiltonian ------------------------------------- */
+
+typedef struct qt_ham qt_ham;
+
+/* Uploads the block lists of nk k-points (devH_build_fromWannierFile
+ * output, deviceHam_mod.f90:234-290): Hii[ik*nx + s], H1i[ik*(nx+1) + s],
+ * Sii[ik*nx + s] (Sii = NULL means identity blocks).  nm[s] is the size of
+ * slab s; H1i[s] is nm[s] x nm[s-1], H1i[0] and H1i[nx] are square. */
+int qt_ham_create(int32_t nx, const int32_t* nm, int32_t nk, const void* const* Hii,
+                  const void* const* H1i, const void* const* Sii, qt_ham** out);
+int qt_ham_destroy(qt_ham* h);
+
+/* ---- batched RGF ----------------------------------------------------- */
+
+typedef struct {
+    int32_t variant;    /* QT_RGF_FORWARD | QT_RGF_BACKWARD */
+    int32_t precision;  /* QT_PREC_FP64 */
+    int32_t flags;      /* QT_F_* */
+    int32_t lead_nmax;  /* Sancho-Rubio iteration cap (50, rgf_mod.f90:507) */
+    double mul, mur;    /* uniform mul / mur matrices (negf_mod.f90:82-83) */
+    double templ, tempr;/* uniform TEMPl / TEMPr matrices (negf_mod.f90:84-85) */
+    double lead_eta;    /* Im z in the lead decimation (1e-5, rgf_mod.f90:508) */
+    double lead_tol;    /* decimation stop: sum |H01| < tol (1e-100, :488) */
+} qt_rgf_params;
+
+typedef struct {
+    /* per point (length npts), each may be NULL */
+    double* tr;          /* spectral current at the second contact */
+    double* tre;         /* spectral current at the first contact */
+    int32_t* lead_iters; /* [npts][2] decimation iterations, first/second lead */
+    int32_t* info;       /* [npts] QT_INFO_* bits */
+    /* diagonals [npts][sum(nm)] complex128 of G^R, G^<, G^> per slab */
+    void* gr_diag;
+    void* gl_diag;
+    void* gg_diag;
+    /* full host blocks, [npts*nx] pointers each (point-major), may be NULL:
+     * the per-point outputs of rgf_variableblock_{forward,backward} */
+    void* const* G_r;
+    void* const* G_lesser;
+    void* const* G_greater;
+    void* const* Jdens;
+    void* const* Gl;
+    void* const* Gln;
+} qt_rgf_outputs;
+
+/* Replaces the E x k loop of negf_mod.f90:115-127 around
+ * rgf_variableblock_forward / _backward (rgf_mod.f90:46-472) and
+ * cuda_rgf_constblocksize (cuda_rgf_mod.f90:53-413): all points
+ * (En[ip], k-point kidx[ip]) are solved in batches on the device.
+ * sigma_lesser_ph / sigma_r_ph: NULL (zero) or [npts*nx] host block
+ * pointers (point-major), the sigma_*_ph(nx) lists of each point. */
+int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* En,
+                 const int32_t* kidx, const void* const* sigma_lesser_ph,
+                 const void* const* sigma_r_ph, qt_rgf_outputs* out);
+
+/* ---- lead self-energy ------------------------------------------------ */
+
+/* Batched Sancho-Rubio decimation, replaces sancho (rgf_mod.f90:482-568,
+ * cuda_rgf_mod.f90:657-765, gw_mod.f90:1378-1464): for every point ip,
+ * z = En[ip] + i*eta; H00[ip], H10[ip] are nm x nm host blocks (pointers may
+ * repeat), S00 one block or NULL (identity).  G00/GBB: [npts] host output
+ * blocks (GBB may be NULL).  iters/converged: [npts], may be NULL. */
+int qt_sancho(int32_t nm, int32_t npts, const double* En, double eta, int32_t nmax,
+              double tol, int32_t reset_on_fail, const void* S00, const void* const* H00,
+              const void* const* H10, void* const* G00, void* const* GBB, int32_t* iters,
+              int32_t* converged);
+
+/* ---- block primitives ------------------------------------------------ */
+
+/* Batched complex128 product R[b] = alpha*op(A[b])*op(B[b]) + beta*R[b],
+ * op in {0: N, 1: T, 2: C}; replaces MUL_C (matrix_c.f90:615-660), whose
+ * ZGEMM call is R = op(A) op(B).  Host buffers; member b of X starts at
+ * X + b*strideX (complex elements, 0 = shared operand). */
+int qt_zgemm_batched(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_t k, int32_t batch,
+                     double alpha, const void* A, int64_t lda, int64_t strideA, const void* B,
+                     int64_t ldb, int64_t strideB, double beta, void* R, int64_t ldr,
+                     int64_t strideR);
+
+/* Batched in-place inverse of n x n complex128 blocks (stride n*n), replaces
+ * invert (linalg.f90:42-62): partial pivoting; a block whose factorisation
+ * meets an exactly zero pivot is zeroed and info[b] = that column (1-based),
+ * else info[b] = 0.  Returns QT_ERR_SINGULAR when any block failed. */
+int qt_invert_batched(int32_t n, int32_t batch, void* A, int32_t* info);
+
+/* ---- device memory helpers (outputs kept resident) -------------------- */
+int qt_device_alloc(size_t bytes, void** ptr);
+int qt_device_free(void* ptr);
+int qt_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int qt_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int qt_synchronize(void);
+
+/* ---- measurement ----------------------------------------------------- */
+
+typedef struct {
+    double total_ms;     /* wall time of the last qt_rgf_solve (device events) */
+    double lead_ms;      /* lead decimation + self-energies */
+    double forward_ms;   /* first pass */
+    double backward_ms;  /* second pass + contact currents */
+    double gemm_ms;      /* summed duration of the complex GEMM launches */
+    double gemm_flops;   /* algorithmic flops of those launches (8 m n k per
+                            complex product, active batch members only) */
+    double inv_ms;       /* summed duration of the block inversions */
+    double inv_flops;    /* 8 n^3 per inverted block */
+    int64_t gemm_launches;
+    int64_t inv_launches;
+    int64_t lead_iterations; /* sum over points and leads of decimation steps */
+    int32_t batch;       /* points per device batch */
+} qt_timing;
+
+/* When enabled, every GEMM / inversion launch is bracketed by HIP events on
+ * the library stream and summed into qt_timing. */
+int qt_set_kernel_timing(int enable);
+int qt_last_timing(qt_timing* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QT_RGF_H */

@@ -52,8 +52,16 @@ def build() -> None:
 
 
 def available(flavour: str = "plain") -> bool:
+    """True when that flavour is built and loads (the vendor flavour needs
+    its BLAS/LAPACK library at run time)."""
     name = "libqtoracle.so" if flavour == "plain" else "libqtoracle_mkl.so"
-    return os.path.exists(os.path.join(BUILD, name))
+    if not os.path.exists(os.path.join(BUILD, name)):
+        return False
+    try:
+        lib(flavour)
+        return True
+    except OSError:
+        return False
 
 
 def lib(flavour: str = "plain") -> C.CDLL:

@@ -1,0 +1,231 @@
+// qt_gemm.hip -- batched complex128 GEMM on the gfx950 FP64 matrix cores.
+//
+// One workgroup (4 waves, 256 threads) computes a 64x64 complex tile of
+// C = alpha*op(A)op(B) + beta*D for one batch member.  K is staged through
+// LDS in chunks of 16 complex values, double buffered (2 x 32 KB).  Each
+// wave owns a 32x32 sub-tile = 2x2 MFMA tiles of 16x16; a complex product is
+// four v_mfma_f64_16x16x4 (re*re, -im*im into Re; re*im, im*re into Im).
+//
+// Operand order: the MFMA is fed (B-fragment, A-fragment), so that the
+// 16x16 result lands transposed in the lanes.  v_mfma_f64_16x16x4f64 leaves
+// element (r, c) of its product in lane c + 16*(r%4), register r/4 (measured
+// on gfx950); with the swapped operands lane l, register q holds
+// C[row = l%16][col = 4*q + l/16], so the 16 lanes of a quarter wave write
+// 16 consecutive rows of one column (256 B, coalesced for column-major C).
+//
+// Work mapping: the 1-D grid is ordered so that all tiles of one batch
+// member are dispatched to the same XCD (workgroups are dealt round-robin
+// over the 8 XCDs), keeping that member's operands in one L2.
+#include "qt_internal.h"
+
+namespace qt {
+
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int NXCD = 8;
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double2 cconj(double2 v) { return make_double2(v.x, -v.y); }
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int tiles_m, int tiles,
+                                                            int batch_pad) {
+    // XCD-aware decode: linear id L -> (member, tile), member%8 == L%8
+    const int L = blockIdx.x;
+    const int lane8 = L % NXCD;
+    const int rest = L / NXCD;
+    const int tile = rest % tiles;
+    const int b = lane8 + NXCD * (rest / tiles);
+    if (b >= a.batch) return;
+    if (a.active && !a.active[b]) return;
+    (void)batch_pad;
+
+    const int kz = a.kidx ? a.kidx[b >> a.kshift] : 0;
+    const double2* __restrict__ Ap = a.A.p + b * a.A.sb + kz * a.A.sk;
+    const double2* __restrict__ Bp = a.B.p + b * a.B.sb + kz * a.B.sk;
+    const int64_t lda = a.A.ld, ldb = a.B.ld;
+    const int M = a.M, N = a.N, K = a.K;
+    const int m0 = (tile % tiles_m) * BM;
+    const int n0 = (tile / tiles_m) * BN;
+
+    __shared__ double2 As[2][BK][BM];
+    __shared__ double2 Bs[2][BK][BN];
+
+    const int tid = threadIdx.x;
+    double2 ra[4], rb[4];
+
+    auto loadA = [&](int k0) {
+        if (OPA == OP_N) {
+            const int kk = tid >> 4, ii = (tid & 15) * 4;
+            const int gk = k0 + kk;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gi = m0 + ii + q;
+                ra[q] = (gi < M && gk < K) ? Ap[gi + gk * lda] : make_double2(0.0, 0.0);
+            }
+        } else {
+            const int ii = tid >> 2, kk = (tid & 3) * 4;
+            const int gi = m0 + ii;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gk = k0 + kk + q;
+                double2 v = (gi < M && gk < K) ? Ap[gk + gi * lda] : make_double2(0.0, 0.0);
+                ra[q] = OPA == OP_C ? cconj(v) : v;
+            }
+        }
+    };
+    auto storeA = [&](int buf) {
+        if (OPA == OP_N) {
+            const int kk = tid >> 4, ii = (tid & 15) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) As[buf][kk][ii + q] = ra[q];
+        } else {
+            const int ii = tid >> 2, kk = (tid & 3) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) As[buf][kk + q][ii] = ra[q];
+        }
+    };
+    auto loadB = [&](int k0) {
+        if (OPB == OP_N) {
+            const int jj = tid >> 2, kk = (tid & 3) * 4;
+            const int gj = n0 + jj;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gk = k0 + kk + q;
+                rb[q] = (gj < N && gk < K) ? Bp[gk + gj * ldb] : make_double2(0.0, 0.0);
+            }
+        } else {
+            const int kk = tid >> 4, jj = (tid & 15) * 4;
+            const int gk = k0 + kk;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gj = n0 + jj + q;
+                double2 v = (gj < N && gk < K) ? Bp[gj + gk * ldb] : make_double2(0.0, 0.0);
+                rb[q] = OPB == OP_C ? cconj(v) : v;
+            }
+        }
+    };
+    auto storeB = [&](int buf) {
+        if (OPB == OP_N) {
+            const int jj = tid >> 2, kk = (tid & 3) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Bs[buf][kk + q][jj] = rb[q];
+        } else {
+            const int kk = tid >> 4, jj = (tid & 15) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Bs[buf][kk][jj + q] = rb[q];
+        }
+    };
+
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    double4_t accR[2][2], accI[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            accR[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+            accI[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+        }
+
+    const int nk = (K + BK - 1) / BK;
+    loadA(0);
+    loadB(0);
+    storeA(0);
+    storeB(0);
+    __syncthreads();
+
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) {
+            loadA((kt + 1) * BK);
+            loadB((kt + 1) * BK);
+        }
+#pragma unroll
+        for (int kq = 0; kq < BK / 4; ++kq) {
+            const int k = kq * 4 + lk;
+            double2 av[2], bv[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) av[i] = As[buf][k][wm * 32 + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bv[j] = Bs[buf][k][wn * 32 + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
+                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[j].y, av[i].y, accR[i][j], 0, 0, 0);
+                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].x, accI[i][j], 0, 0, 0);
+                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].y, accI[i][j], 0, 0, 0);
+                }
+        }
+        if (kt + 1 < nk) {
+            storeA(buf ^ 1);
+            storeB(buf ^ 1);
+        }
+        __syncthreads();
+    }
+
+    // epilogue
+    double2* __restrict__ Cp = a.C + b * a.sbc + kz * a.skc;
+    const double2* __restrict__ Dp = a.D.p ? a.D.p + b * a.D.sb + kz * a.D.sk : nullptr;
+    const double alpha = a.alpha, beta = a.beta;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int gi = m0 + wm * 32 + i * 16 + li;
+        if (gi >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gj = n0 + wn * 32 + j * 16 + 4 * q + lk;
+                if (gj >= N || (gj >= a.skip_lo && gj < a.skip_hi)) continue;
+                double re = alpha * accR[i][j][q], im = alpha * accI[i][j][q];
+                if (Dp) {
+                    const double2 d = Dp[gi + gj * a.D.ld];
+                    re += beta * d.x;
+                    im += beta * d.y;
+                }
+                Cp[gi + gj * a.ldc] = make_double2(re, im);
+            }
+        }
+    }
+}
+
+template <int OPA, int OPB>
+void launch(hipStream_t s, const ZGemmArgs& a) {
+    const int tiles_m = (a.M + BM - 1) / BM;
+    const int tiles_n = (a.N + BN - 1) / BN;
+    const int tiles = tiles_m * tiles_n;
+    const int batch_pad = (a.batch + NXCD - 1) / NXCD * NXCD;
+    const int64_t blocks = (int64_t)tiles * batch_pad;
+    if (blocks > 0x7fffffff) throw hip_error("zgemm: grid too large", 1);
+    hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB>), dim3((unsigned)blocks), dim3(256), 0, s, a,
+                       tiles_m, tiles, batch_pad);
+    QT_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void zgemm(hipStream_t s, int opa, int opb, const ZGemmArgs& a, Timer* t) {
+    if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return;
+    if (t && t->enabled) t->begin(s, 0);
+    const int key = opa * 3 + opb;
+    switch (key) {
+        case 0: launch<OP_N, OP_N>(s, a); break;
+        case 2: launch<OP_N, OP_C>(s, a); break;
+        case 6: launch<OP_C, OP_N>(s, a); break;
+        case 8: launch<OP_C, OP_C>(s, a); break;
+        case 1: launch<OP_N, OP_T>(s, a); break;
+        case 3: launch<OP_T, OP_N>(s, a); break;
+        default: throw hip_error("zgemm: unsupported op pair", 6);
+    }
+    if (t && t->enabled)
+        t->end(s, 0, 8.0 * a.M * a.N * (double)a.K * (a.flop_batch >= 0 ? a.flop_batch : a.batch));
+}
+
+}  // namespace qt

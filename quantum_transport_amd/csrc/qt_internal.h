@@ -1,0 +1,180 @@
+// qt_internal.h -- shared declarations of the gfx950 RGF library.
+//
+// Device data layout: column-major complex128 blocks stored as double2
+// (x = re, y = im), the layout of Fortran complex(8).  Batched operands are
+// addressed as  base + b*sb + kidx[b >> kshift]*sk  so one launch can mix
+// per-point buffers (sb != 0) with per-k-point Hamiltonian blocks (sk != 0)
+// without pointer arrays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace qt {
+
+enum Op { OP_N = 0, OP_T = 1, OP_C = 2 };
+
+struct hip_error : std::runtime_error {
+    int code;
+    hip_error(const std::string& s, int c) : std::runtime_error(s), code(c) {}
+};
+
+#define QT_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess)                                                          \
+            throw ::qt::hip_error(std::string(#call) + ": " + hipGetErrorString(_e),   \
+                                  _e == hipErrorOutOfMemory ? 4 : 3);                  \
+    } while (0)
+
+// Batched operand: element (i, j) of member b is
+//   p[b*sb + kidx[b>>kshift]*sk + i + j*ld]
+struct ZOp {
+    const double2* p = nullptr;
+    int64_t ld = 0, sb = 0, sk = 0;
+};
+
+struct ZGemmArgs {
+    int M = 0, N = 0, K = 0, batch = 0;
+    ZOp A, B;
+    double2* C = nullptr;
+    int64_t ldc = 0, sbc = 0, skc = 0;
+    ZOp D;                // optional addend, C = alpha*op(A)op(B) + beta*D
+    double alpha = 1.0, beta = 0.0;
+    const int* kidx = nullptr;
+    int kshift = 0;
+    const int* active = nullptr;   // skip members with active[b] == 0
+    int skip_lo = 0, skip_hi = 0;  // output columns [skip_lo, skip_hi) not written
+    int flop_batch = -1;           // members counted by the timer (-1: batch)
+};
+
+struct Timer;  // forward
+
+// C = alpha * op(A) op(B) + beta * D, complex128, MFMA f64.
+void zgemm(hipStream_t s, int opa, int opb, const ZGemmArgs& a, Timer* t);
+
+// Batched in-place-free inversion: Ainv[b] = inv(A[b]); A is destroyed.
+// piv: n ints per member, info: one int per member (0 or first zero pivot
+// column + 1), R: NB*n scratch per member.
+struct InvArgs {
+    int n = 0, batch = 0;
+    double2* A = nullptr;       // work matrices, lda = n, stride sa
+    int64_t sa = 0;
+    double2* X = nullptr;       // output inverse, ldx = n, stride sx
+    int64_t sx = 0;
+    int* piv = nullptr;         // n per member
+    int* info = nullptr;        // 1 per member
+    double2* R = nullptr;       // scratch, 16*n per member
+    const int* active = nullptr;
+    int flop_batch = -1;        // members counted by the timer (-1: batch)
+};
+void zinv(hipStream_t s, const InvArgs& a, Timer* t);
+
+// ---- elementwise kernels (qt_elem.hip) ----
+
+// out[b] = z_b*S + c1*M1[b] + c2*M2[b] + c3*M3[b]   (n x n blocks)
+// z_b = zre[b] + i*zim ; S == nullptr means identity.
+struct ShiftArgs {
+    int n = 0, batch = 0;
+    double2* out = nullptr;
+    int64_t sout = 0;
+    const double* zre = nullptr;   // per member (device)
+    double zim = 0.0;
+    ZOp S;                         // may be null (identity)
+    ZOp M1, M2, M3;
+    double c1 = 0, c2 = 0, c3 = 0;
+    const int* kidx = nullptr;
+    int kshift = 0;
+    const int* active = nullptr;
+};
+void shifted(hipStream_t s, const ShiftArgs& a);
+
+// out[b] = a_b*(X[b] - X[b]^H) + c1*Y[b] + c2*Z[b]  (n x n)
+// a_b = afac ? afac[b] : aconst.
+struct AntiArgs {
+    int n = 0, batch = 0;
+    double2* out = nullptr;
+    int64_t sout = 0;
+    ZOp X, Y, Z;
+    double c1 = 0, c2 = 0;
+    const double* afac = nullptr;
+    double aconst = 1.0;
+    const int* kidx = nullptr;
+    int kshift = 0;
+    const int* active = nullptr;
+};
+void antiherm(hipStream_t s, const AntiArgs& a);
+
+// Contact spectral current
+//   cur[b] = Re Tr[ Om (f_b G^> - (f_b - 1) G^<) ],  Om = -(Sig - Sig^H),
+//   G^> = G^< + G - G^H   (rgf_mod.f90:191-202, 258-266)
+struct CurrentArgs {
+    int n = 0, batch = 0;
+    ZOp Sig, G, Gl;
+    const double* f = nullptr;     // per member
+    double* partial = nullptr;     // scratch [batch][tiles]
+    double* out = nullptr;         // [batch] (device)
+    const int* active = nullptr;
+};
+void contact_current(hipStream_t s, const CurrentArgs& a);
+
+// diag outputs: gr[b*ld + off + j] = G[j,j], gl = Gl[j,j],
+// gg = Gl[j,j] + G[j,j] - conj(G[j,j])
+struct DiagArgs {
+    int n = 0, batch = 0;
+    ZOp G, Gl;
+    double2 *gr = nullptr, *gl = nullptr, *gg = nullptr;
+    int64_t ld = 0, off = 0;
+    const int* active = nullptr;
+};
+void diag_extract(hipStream_t s, const DiagArgs& a);
+
+// Sancho-Rubio step update (rgf_mod.f90:526-541):
+//   H_SS += C; H_BB += C + Ap; err_b = sum |H01n|; converged when err < tol
+struct SanchoUpdArgs {
+    int n = 0, batch = 0;
+    double2 *HSS = nullptr, *HBB = nullptr;
+    const double2 *Cm = nullptr, *Ap = nullptr, *H01n = nullptr;
+    int64_t s = 0;                 // common batch stride of all five
+    double* partial = nullptr;     // [batch][blocks]
+    int* active = nullptr;         // cleared on convergence
+    int* iters = nullptr;          // iteration count per member
+    int iter = 0;
+    double tol = 0.0;
+    int reset = 0;                 // last iteration with reset_on_fail
+    const double2* H00 = nullptr;  // for the reset
+    int64_t sh00 = 0, skh00 = 0;
+    const int* kidx = nullptr;
+};
+void sancho_update(hipStream_t s, const SanchoUpdArgs& a);
+
+// copies: dst[b] = src[b] (n x n), optional conjugate transpose
+void copy_blocks(hipStream_t s, int n, int batch, double2* dst, int64_t sdst, ZOp src,
+                 const int* kidx, int kshift, bool dagger);
+
+// count active members (device) -> *count
+void count_active(hipStream_t s, const int* active, int batch, int* count);
+
+// fill ints
+void fill_int(hipStream_t s, int* p, int n, int v);
+
+// Per-launch timing accumulator (HIP events on the library stream)
+struct Timer {
+    bool enabled = false;
+    hipEvent_t ev[2][4096];
+    int n[2] = {0, 0};          // 0: gemm, 1: inversion
+    double flops[2] = {0, 0};
+    int64_t launches[2] = {0, 0};
+    double ms[2] = {0, 0};
+    bool created = false;
+    void begin(hipStream_t s, int kind);
+    void end(hipStream_t s, int kind, double flops_);
+    void flush();               // synchronise and sum pending events
+    void reset();
+};
+
+}  // namespace qt

@@ -1,0 +1,1305 @@
+// qt_rgf.hip -- batched RGF sweeps and the C ABI (include/qt_rgf.h).
+//
+// A solve is cut into device batches of B energy points.  For one batch the
+// work is a fixed sequence of batched launches (GEMM / inversion /
+// elementwise) over all B points at once:
+//   leads:   Sancho-Rubio for both contacts as one batch of 2B members,
+//            then the contact self-energies;
+//   pass 1:  left-connected (forward variant) or right-connected (backward
+//            variant) Gl, Gln for every slab, kept resident in HBM;
+//   pass 2:  G^R, G^<, G^> of every slab from the neighbour's result,
+//            diagonals written out, contact spectral currents.
+// The arithmetic of each step is that of rgf_mod.f90 (cited per step).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/qt_rgf.h"
+#include "qt_internal.h"
+
+struct qt_ham {
+    int nx = 0, nk = 0;
+    std::vector<int> nm;
+    std::vector<int64_t> hoff;  // offset of slab s inside one k set (elements)
+    std::vector<int64_t> coff;  // offset of coupling s (0..nx)
+    int64_t hset = 0, cset = 0;
+    double2* H = nullptr;       // [nk][hset]
+    double2* C = nullptr;       // [nk][cset]
+    double2* S = nullptr;       // [nk][hset] or null (identity)
+    bool uniform = true;
+    int nmax = 0;
+};
+
+namespace qt {
+
+static const double BOLTZ = 8.61734e-05;  // eV/K, rgf_mod.f90:38
+
+static double ferm(double a) { return 1.0 / (1.0 + std::exp(a)); }  // rgf_mod.f90:475-479
+
+// ------------------------------------------------------------ device arena
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    void reserve(size_t bytes) {
+        if (bytes <= cap) return;
+        release();
+        QT_HIP(hipMalloc((void**)&base, bytes));
+        cap = bytes;
+    }
+    void reset() { used = 0; }
+    template <class T>
+    T* take(size_t count) {
+        const size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+        if (used + bytes > cap) throw hip_error("workspace arena exhausted", 4);
+        T* p = (T*)(base + used);
+        used += bytes;
+        return p;
+    }
+    void release() {
+        if (base) (void)hipFree(base);
+        base = nullptr;
+        cap = used = 0;
+    }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Timer timer;
+    Arena arena;
+    qt_timing last{};
+    std::mutex mu;
+    int* host_count = nullptr;  // pinned
+    hipEvent_t ev[4];
+};
+
+static Ctx* g_ctx = nullptr;
+static std::mutex g_init_mu;
+static thread_local std::string g_last_error;
+
+// ------------------------------------------------------------ operand helpers
+static ZOp op_member(const double2* p, int64_t ld, int64_t sb) {
+    ZOp o;
+    o.p = p;
+    o.ld = ld;
+    o.sb = sb;
+    return o;
+}
+static ZOp op_k(const double2* p, int64_t ld, int64_t sk) {
+    ZOp o;
+    o.p = p;
+    o.ld = ld;
+    o.sk = sk;
+    return o;
+}
+
+struct Gemm {
+    hipStream_t s;
+    Timer* t;
+    int batch;
+    const int* kidx;
+    const int* active = nullptr;
+    int flop_batch = -1;
+    // C = alpha op(A) op(B) + beta D
+    void operator()(int opa, int opb, int M, int N, int K, ZOp A, ZOp B, double2* C, int64_t ldc,
+                    int64_t sbc, double alpha = 1.0, ZOp D = ZOp(), double beta = 0.0) const {
+        ZGemmArgs g;
+        g.M = M;
+        g.N = N;
+        g.K = K;
+        g.batch = batch;
+        g.A = A;
+        g.B = B;
+        g.C = C;
+        g.ldc = ldc;
+        g.sbc = sbc;
+        g.D = D;
+        g.alpha = alpha;
+        g.beta = D.p ? beta : 0.0;
+        g.kidx = kidx;
+        g.active = active;
+        g.flop_batch = flop_batch;
+        zgemm(s, opa, opb, g, t);
+    }
+};
+
+// ------------------------------------------------------------ workspace
+constexpr int NTEMP = 14;
+
+struct Work {
+    int B = 0;               // points in the batch
+    int nmax = 0;
+    int64_t nn = 0;          // nmax^2
+    int64_t ndiag = 0;       // sum(nm)
+    double2* Gl = nullptr;   // slab-major: slab s at hoff[s]*B, member stride nm[s]^2
+    double2* Gln = nullptr;
+    double2* H00 = nullptr;  // Hii + sigma_r S per point (only with sigma), same layout
+    double2* SLS = nullptr;  // sigma_lesser S per point (only with sigma), same layout
+    double2* T[NTEMP];       // temps, 2B * nn each
+    double2* sigF = nullptr; // first-contact self-energy [B][nn]
+    double2* sigL = nullptr; // last-contact self-energy  [B][nn]
+    int* piv = nullptr;      // 2B * nmax
+    int* info = nullptr;     // 2B
+    double2* R = nullptr;    // 2B * 16 * nmax
+    double* zre = nullptr;   // 2B energies (the batch twice, for the lead members)
+    int* kidx = nullptr;     // 2B
+    double* fF = nullptr;    // B  occupation of the first contact
+    double* fL = nullptr;    // B  occupation of the last contact
+    double* nfF = nullptr;   // B  -fF
+    double* nfL = nullptr;   // B  -fL
+    int* active = nullptr;   // 2B
+    int* iters = nullptr;    // 2B
+    int* lead_it = nullptr;  // 2B  (first contact, then last contact)
+    int* count = nullptr;    // 1
+    double* partial = nullptr;  // 2B * max(64, tiles^2)
+    int* infoacc = nullptr;     // B accumulated QT_INFO bits
+    double* tr = nullptr;       // B
+    double* tre = nullptr;      // B
+    double2* diag = nullptr;    // 3 * B * ndiag staging
+};
+
+static size_t work_bytes(const qt_ham& h, int B, bool sigma) {
+    const int64_t nn = (int64_t)h.nmax * h.nmax;
+    const int64_t tiles = (h.nmax + 31) / 32;
+    size_t b = 0;
+    auto add = [&](size_t bytes) { b += (bytes + 255) & ~size_t(255); };
+    add(sizeof(double2) * (size_t)h.hset * B * (sigma ? 4 : 2));
+    for (int i = 0; i < NTEMP; ++i) add(sizeof(double2) * (size_t)nn * 2 * B);
+    add(sizeof(double2) * (size_t)nn * B * 2);
+    add(sizeof(int) * (size_t)2 * B * h.nmax);
+    add(sizeof(double2) * (size_t)2 * B * 16 * h.nmax);
+    int64_t ndiag = 0;
+    for (int s = 0; s < h.nx; ++s) ndiag += h.nm[s];
+    add(sizeof(double2) * (size_t)3 * B * ndiag);
+    add(sizeof(double) * (size_t)2 * B * std::max<int64_t>(64, tiles * tiles));
+    add(4096 + sizeof(double) * 16 * (size_t)B + sizeof(int) * 16 * (size_t)B);
+    return b + (1 << 20);
+}
+
+static void work_carve(Arena& a, Work& w, const qt_ham& h, int B, bool sigma) {
+    a.reset();
+    w.B = B;
+    w.nmax = h.nmax;
+    w.nn = (int64_t)h.nmax * h.nmax;
+    w.ndiag = 0;
+    for (int s = 0; s < h.nx; ++s) w.ndiag += h.nm[s];
+    const int64_t tiles = (h.nmax + 31) / 32;
+    w.Gl = a.take<double2>((size_t)h.hset * B);
+    w.Gln = a.take<double2>((size_t)h.hset * B);
+    w.H00 = sigma ? a.take<double2>((size_t)h.hset * B) : nullptr;
+    w.SLS = sigma ? a.take<double2>((size_t)h.hset * B) : nullptr;
+    for (int i = 0; i < NTEMP; ++i) w.T[i] = a.take<double2>((size_t)w.nn * 2 * B);
+    w.sigF = a.take<double2>((size_t)w.nn * B);
+    w.sigL = a.take<double2>((size_t)w.nn * B);
+    w.piv = a.take<int>((size_t)2 * B * h.nmax);
+    w.info = a.take<int>((size_t)2 * B);
+    w.R = a.take<double2>((size_t)2 * B * 16 * h.nmax);
+    w.zre = a.take<double>((size_t)2 * B);
+    w.kidx = a.take<int>((size_t)2 * B);
+    w.fF = a.take<double>(B);
+    w.fL = a.take<double>(B);
+    w.nfF = a.take<double>(B);
+    w.nfL = a.take<double>(B);
+    w.active = a.take<int>((size_t)2 * B);
+    w.iters = a.take<int>((size_t)2 * B);
+    w.lead_it = a.take<int>((size_t)2 * B);
+    w.count = a.take<int>(1);
+    w.partial = a.take<double>((size_t)2 * B * std::max<int64_t>(64, tiles * tiles));
+    w.infoacc = a.take<int>(B);
+    w.tr = a.take<double>(B);
+    w.tre = a.take<double>(B);
+    w.diag = a.take<double2>((size_t)3 * B * w.ndiag);
+}
+
+static void inv_batch(Ctx& c, Work& w, int n, int batch, double2* A, int64_t sa, double2* X,
+                      int64_t sx, const int* active, int flop_batch = -1) {
+    InvArgs a;
+    a.n = n;
+    a.batch = batch;
+    a.A = A;
+    a.sa = sa;
+    a.X = X;
+    a.sx = sx;
+    a.piv = w.piv;
+    a.info = w.info;
+    a.R = w.R;
+    a.active = active;
+    a.flop_batch = flop_batch;
+    zinv(c.stream, a, &c.timer);
+}
+
+// accumulate inversion failures of members into per-point info bits
+__global__ void info_accum_kernel(const int* info, int batch, int* acc, int period) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b < batch && info[b] != 0) atomicOr(&acc[b % period], (int)QT_INFO_SINGULAR);
+}
+
+static void note_inv_info(Ctx& c, Work& w, int batch, int period) {
+    hipLaunchKernelGGL(info_accum_kernel, dim3((batch + 255) / 256), dim3(256), 0, c.stream, w.info,
+                       batch, w.infoacc, period);
+    QT_HIP(hipGetLastError());
+}
+
+// lead iteration counts -> info bit when nmax was reached without convergence
+__global__ void lead_info_kernel(const int* active, int members, int* acc, int period) {
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    if (m < members && active[m]) atomicOr(&acc[m % period], (int)QT_INFO_LEAD_NMAX);
+}
+
+// ------------------------------------------------------------ Sancho-Rubio
+// Members m with z_m = zre[m] + i eta (rgf_mod.f90:508).  H00 addressed
+// per member or per k; H10 per member (overwritten).  Outputs G00 (and
+// GBB) per member with stride nn.  rgf_mod.f90:482-568.
+struct SanchoIO {
+    int n = 0, members = 0;
+    const double* zre = nullptr;
+    const int* kidx = nullptr;
+    double eta = 1e-5;
+    int nmax = 50;
+    double tol = 1e-100;
+    bool reset = false;
+    ZOp S;           // null = identity
+    ZOp H00;
+    double2* H10 = nullptr;
+    double2* G00 = nullptr;
+    double2* GBB = nullptr;
+    int64_t nn = 0;
+    int* iters_out = nullptr;     // members (device)
+    int* info_bits = nullptr;     // per point accumulator (device), may be null
+    int period = 1;               // member m belongs to point m % period
+    int64_t total_iters = 0;      // host: sum of iterations over members
+};
+
+static void sancho_run(Ctx& c, Work& w, SanchoIO& io) {
+    const int n = io.n, M = io.members;
+    const int64_t nn = io.nn;
+    hipStream_t s = c.stream;
+    double2* const* T = w.T;
+    double2 *HSS = T[0], *HBB = T[1], *H10 = io.H10, *H01 = T[2], *A = T[3], *Ai = T[4];
+    double2 *B1 = T[5], *B2 = T[6], *C1 = T[7], *H01n = T[8], *Cp = T[9], *Ap = T[10];
+    // H_BB = H00, H_10 = H10, H_01 = H10^H, H_SS = H00  (rgf_mod.f90:515-518)
+    copy_blocks(s, n, M, HSS, nn, io.H00, io.kidx, 0, false);
+    copy_blocks(s, n, M, HBB, nn, io.H00, io.kidx, 0, false);
+    copy_blocks(s, n, M, H01, nn, op_member(H10, n, nn), nullptr, 0, true);
+    fill_int(s, w.active, M, 1);
+    fill_int(s, w.iters, M, 0);
+    int nact = M;
+    Gemm g{s, &c.timer, M, nullptr};
+    g.active = w.active;
+    io.total_iters = 0;
+    for (int it = 1; it <= io.nmax && nact > 0; ++it) {
+        io.total_iters += nact;
+        g.flop_batch = nact;
+        ShiftArgs sh;
+        sh.n = n;
+        sh.batch = M;
+        sh.out = A;
+        sh.sout = nn;
+        sh.zre = io.zre;
+        sh.zim = io.eta;
+        sh.S = io.S;
+        sh.M1 = op_member(HBB, n, nn);
+        sh.c1 = -1.0;
+        sh.kidx = io.kidx;
+        sh.active = w.active;
+        shifted(s, sh);                                   // A = z S00 - H_BB
+        inv_batch(c, w, n, M, A, nn, Ai, nn, w.active, nact);  // invert(A)
+        const ZOp a = op_member(Ai, n, nn), h10 = op_member(H10, n, nn), h01 = op_member(H01, n, nn);
+        g(OP_N, OP_N, n, n, n, a, h10, B1, n, nn);                       // B = A H_10
+        g(OP_N, OP_N, n, n, n, h01, op_member(B1, n, nn), C1, n, nn);    // C = H_01 B
+        g(OP_N, OP_N, n, n, n, h10, op_member(B1, n, nn), Cp, n, nn);    // C = H_10 B
+        g(OP_N, OP_N, n, n, n, a, h01, B2, n, nn);                       // B = A H_01
+        g(OP_N, OP_N, n, n, n, h10, op_member(B2, n, nn), Ap, n, nn);    // A = H_10 B
+        g(OP_N, OP_N, n, n, n, h01, op_member(B2, n, nn), H01n, n, nn);  // C = H_01 B
+        SanchoUpdArgs u;
+        u.n = n;
+        u.batch = M;
+        u.HSS = HSS;
+        u.HBB = HBB;
+        u.Cm = C1;
+        u.Ap = Ap;
+        u.H01n = H01n;
+        u.s = nn;
+        u.partial = w.partial;
+        u.active = w.active;
+        u.iters = w.iters;
+        u.iter = it;
+        u.tol = io.tol;
+        u.reset = (io.reset && it == io.nmax) ? 1 : 0;
+        u.H00 = io.H00.p;
+        u.sh00 = io.H00.sb;
+        u.skh00 = io.H00.sk;
+        u.kidx = io.kidx;
+        sancho_update(s, u);  // H_SS += C, H_BB += C + A, error = sum |H_01|, stop test
+        std::swap(H10, Cp);   // H_10 = C (new coupling)
+        std::swap(H01, H01n); // H_01 = C
+        count_active(s, w.active, M, w.count);
+        QT_HIP(hipMemcpyAsync(c.host_count, w.count, sizeof(int), hipMemcpyDeviceToHost, s));
+        QT_HIP(hipStreamSynchronize(s));
+        nact = *c.host_count;
+    }
+    if (io.info_bits && nact > 0) {
+        hipLaunchKernelGGL(lead_info_kernel, dim3((M + 255) / 256), dim3(256), 0, s, w.active, M,
+                           io.info_bits, io.period);
+        QT_HIP(hipGetLastError());
+    }
+    if (io.iters_out)
+        QT_HIP(hipMemcpyAsync(io.iters_out, w.iters, sizeof(int) * M, hipMemcpyDeviceToDevice, s));
+    // G00 = inv(z S00 - H_SS), GBB = inv(z S00 - H_BB)  (rgf_mod.f90:553-557)
+    for (int pass = 0; pass < (io.GBB ? 2 : 1); ++pass) {
+        ShiftArgs sh;
+        sh.n = n;
+        sh.batch = M;
+        sh.out = A;
+        sh.sout = nn;
+        sh.zre = io.zre;
+        sh.zim = io.eta;
+        sh.S = io.S;
+        sh.M1 = op_member(pass == 0 ? HSS : HBB, n, nn);
+        sh.c1 = -1.0;
+        sh.kidx = io.kidx;
+        shifted(s, sh);
+        inv_batch(c, w, n, M, A, nn, pass == 0 ? io.G00 : io.GBB, nn, nullptr);
+        if (io.info_bits) note_inv_info(c, w, M, io.period);
+    }
+}
+
+// ------------------------------------------------------------ full-block host outputs
+struct FullOut {
+    bool any = false;
+    void* const* G_r = nullptr;
+    void* const* G_lesser = nullptr;
+    void* const* G_greater = nullptr;
+    void* const* Jdens = nullptr;
+    void* const* Gl = nullptr;
+    void* const* Gln = nullptr;
+    int nx = 0;
+    int p0 = 0;  // first point of the batch
+};
+
+static void d2h_blocks(Ctx& c, void* const* dst, int nx, int p0, int B, int s, int n,
+                       const double2* src, int64_t sb) {
+    if (!dst) return;
+    QT_HIP(hipStreamSynchronize(c.stream));
+    for (int b = 0; b < B; ++b) {
+        void* d = dst[(int64_t)(p0 + b) * nx + s];
+        if (!d) continue;
+        QT_HIP(hipMemcpy(d, src + b * sb, sizeof(double2) * (size_t)n * n, hipMemcpyDeviceToHost));
+    }
+}
+
+// ------------------------------------------------------------ one batch
+struct Solve {
+    Ctx& c;
+    qt_ham& h;
+    const qt_rgf_params& p;
+    Work& w;
+    int B;
+    bool fwd;
+    bool fix;
+    bool sigma;
+    FullOut full;
+    double2* gr = nullptr;  // diag destinations (device), already offset to the batch
+    double2* glo = nullptr;
+    double2* gg = nullptr;
+    int64_t lead_iterations = 0;
+
+    std::vector<int64_t> doff;  // diag offset of slab s
+
+    ZOp opH00(int s) const {
+        if (sigma) return op_member(w.H00 + h.hoff[s] * B, h.nm[s], sq(s));
+        return op_k(h.H + h.hoff[s], h.nm[s], h.hset);
+    }
+    ZOp opS(int s) const {
+        if (!h.S) return ZOp();
+        return op_k(h.S + h.hoff[s], h.nm[s], h.hset);
+    }
+    ZOp opC(int s) const {  // H1i[s]: nm[s] x nm[s-1]; s = 0 and s = nx are square
+        const int rows = s < h.nx ? h.nm[s] : h.nm[h.nx - 1];
+        return op_k(h.C + h.coff[s], rows, h.cset);
+    }
+    int64_t sq(int s) const { return (int64_t)h.nm[s] * h.nm[s]; }
+    double2* gl(int s) const { return w.Gl + h.hoff[s] * B; }
+    double2* gln(int s) const { return w.Gln + h.hoff[s] * B; }
+    ZOp opGl(int s) const { return op_member(gl(s), h.nm[s], sq(s)); }
+    ZOp opGln(int s) const { return op_member(gln(s), h.nm[s], sq(s)); }
+    ZOp opSLS(int s) const {
+        if (!w.SLS) return ZOp();
+        return op_member(w.SLS + h.hoff[s] * B, h.nm[s], sq(s));
+    }
+
+    void diag_out(int s, const double2* G, const double2* L) {
+        DiagArgs d;
+        d.n = h.nm[s];
+        d.batch = B;
+        d.G = op_member(G, h.nm[s], w.nn);
+        d.Gl = op_member(L, h.nm[s], w.nn);
+        d.gr = gr;
+        d.gl = glo;
+        d.gg = gg;
+        d.ld = w.ndiag;
+        d.off = doff[s];
+        diag_extract(c.stream, d);
+    }
+
+    // G_r, G_lesser, G_greater (and Jdens from `jd`) of slab s to the host
+    void full_slab(int s, const double2* G, const double2* L, const double2* jd) {
+        if (!full.any) return;
+        const int n = h.nm[s];
+        d2h_blocks(c, full.G_r, full.nx, full.p0, B, s, n, G, w.nn);
+        d2h_blocks(c, full.G_lesser, full.nx, full.p0, B, s, n, L, w.nn);
+        if (full.G_greater) {
+            AntiArgs an;
+            an.n = n;
+            an.batch = B;
+            an.out = w.T[12];
+            an.sout = w.nn;
+            an.X = op_member(G, n, w.nn);
+            an.aconst = 1.0;
+            an.Y = op_member(L, n, w.nn);
+            an.c1 = 1.0;
+            antiherm(c.stream, an);  // G^> = G^< + (G - G^H)
+            d2h_blocks(c, full.G_greater, full.nx, full.p0, B, s, n, w.T[12], w.nn);
+        }
+        if (jd) d2h_blocks(c, full.Jdens, full.nx, full.p0, B, s, n, jd, w.nn);
+    }
+
+    void run(qt_timing& tm) {
+        hipStream_t s = c.stream;
+        const int nx = h.nx;
+        const int64_t nn = w.nn;
+        const int sf = fwd ? 0 : nx - 1;  // first slab of pass 1
+        const int sl = fwd ? nx - 1 : 0;  // last slab of pass 1
+        const int step = fwd ? 1 : -1;
+        const int nF = h.nm[sf], nL = h.nm[sl];
+        const bool want_j = full.any && full.Jdens;
+        double2* const* T = w.T;
+        Gemm g{s, &c.timer, B, w.kidx};
+        QT_HIP(hipEventRecord(c.ev[0], s));
+
+        // ---------------- leads
+        // first contact: forward = left lead: sancho(H10 = H1i(1)^H),
+        //   Sigma = H1i(1) G00 H1i(1)^H            (rgf_mod.f90:80-93)
+        //   backward = right lead: sancho(H10 = H1i(nx+1)),
+        //   Sigma = H1i(nx+1) G00 H1i(nx+1)^H      (rgf_mod.f90:302-314)
+        // last contact: forward = right lead: sancho(H10 = H1i(nx+1)),
+        //   Sigma = H1i(nx+1)^H G00 H1i(nx+1)      (rgf_mod.f90:157-161)
+        //   backward = left lead: sancho(H10 = H1i(1)^H),
+        //   Sigma = H1i(1)^H G00 H1i(1)            (rgf_mod.f90:370-374)
+        const int cF = fwd ? 0 : nx, cL = fwd ? nx : 0;
+        {
+            double2* H10buf = T[11];
+            double2* G00buf = T[12];
+            double2* H00buf = T[13];
+            copy_blocks(s, nF, B, H10buf, nn, opC(cF), w.kidx, 0, fwd);
+            copy_blocks(s, nL, B, H10buf + B * nn, nn, opC(cL), w.kidx, 0, !fwd);
+            copy_blocks(s, nF, B, H00buf, nn, opH00(sf), w.kidx, 0, false);
+            copy_blocks(s, nL, B, H00buf + B * nn, nn, opH00(sl), w.kidx, 0, false);
+            // both contacts in one decimation batch of 2B members when they
+            // share the block size and the overlap is the identity
+            const int parts = (nF == nL && !h.S) ? 1 : 2;
+            for (int part = 0; part < parts; ++part) {
+                SanchoIO io;
+                io.n = parts == 1 ? nF : (part == 0 ? nF : nL);
+                io.members = parts == 1 ? 2 * B : B;
+                io.nn = nn;
+                io.zre = w.zre + part * B;
+                io.kidx = w.kidx + part * B;
+                io.eta = p.lead_eta;
+                io.nmax = p.lead_nmax;
+                io.tol = p.lead_tol;
+                io.reset = (p.flags & QT_F_LEAD_RESET) != 0;
+                io.S = opS(part == 0 ? sf : sl);
+                io.H00 = op_member(H00buf + part * B * nn, io.n, nn);
+                io.H10 = H10buf + part * B * nn;
+                io.G00 = G00buf + part * B * nn;
+                io.iters_out = w.lead_it + part * B;
+                io.info_bits = w.infoacc;
+                io.period = B;
+                sancho_run(c, w, io);
+                lead_iterations += io.total_iters;
+            }
+            // Sigma_first = U G00 U^H, U = H1i(cF)
+            g(OP_N, OP_N, nF, nF, nF, opC(cF), op_member(G00buf, nF, nn), T[0], nF, nn);
+            g(OP_N, OP_C, nF, nF, nF, op_member(T[0], nF, nn), opC(cF), w.sigF, nF, nn);
+            // Sigma_last = V^H G00 V, V = H1i(cL)
+            g(OP_C, OP_N, nL, nL, nL, opC(cL), op_member(G00buf + B * nn, nL, nn), T[0], nL, nn);
+            g(OP_N, OP_N, nL, nL, nL, op_member(T[0], nL, nn), opC(cL), w.sigL, nL, nn);
+        }
+        QT_HIP(hipEventRecord(c.ev[1], s));
+
+        // ---------------- pass 1, first slab (rgf_mod.f90:91-104 / 312-325)
+        {
+            const int n = nF;
+            ShiftArgs sh;
+            sh.n = n;
+            sh.batch = B;
+            sh.out = T[0];
+            sh.sout = nn;
+            sh.zre = w.zre;
+            sh.S = opS(sf);
+            sh.M1 = opH00(sf);
+            sh.c1 = -1.0;
+            sh.M2 = op_member(w.sigF, n, nn);
+            sh.c2 = -1.0;
+            sh.kidx = w.kidx;
+            shifted(s, sh);                                   // z S - H00 - Sigma
+            inv_batch(c, w, n, B, T[0], nn, gl(sf), sq(sf), nullptr);
+            note_inv_info(c, w, B, B);
+            AntiArgs an;                                      // -(Sigma - Sigma^H) f + sl S
+            an.n = n;
+            an.batch = B;
+            an.out = T[1];
+            an.sout = nn;
+            an.X = op_member(w.sigF, n, nn);
+            an.afac = w.nfF;
+            an.Y = opSLS(sf);
+            an.c1 = 1.0;
+            antiherm(s, an);
+            g(OP_N, OP_N, n, n, n, opGl(sf), op_member(T[1], n, nn), T[2], n, nn);
+            g(OP_N, OP_C, n, n, n, op_member(T[2], n, nn), opGl(sf), gln(sf), n, sq(sf));
+        }
+        // ---------------- pass 1, interior slabs (rgf_mod.f90:113-136 / 330-353)
+        int sp = sf;
+        for (int sc = sf + step; sc != sl; sc += step) {
+            const int n = h.nm[sc], np = h.nm[sp];
+            const ZOp U = opC(fwd ? sc : sc + 1);  // H1i(ii) forward, H1i(ii+1) backward
+            g(OP_N, OP_N, n, np, np, U, opGl(sp), T[0], n, nn);          // U Gl
+            ShiftArgs sh;
+            sh.n = n;
+            sh.batch = B;
+            sh.out = T[1];
+            sh.sout = nn;
+            sh.zre = w.zre;
+            sh.S = opS(sc);
+            sh.M1 = opH00(sc);
+            sh.c1 = -1.0;
+            sh.kidx = w.kidx;
+            shifted(s, sh);                                              // z S - H00
+            g(OP_N, OP_C, n, n, np, op_member(T[0], n, nn), U, T[1], n, nn, -1.0,
+              op_member(T[1], n, nn), 1.0);                              // - U Gl U^H
+            inv_batch(c, w, n, B, T[1], nn, gl(sc), sq(sc), nullptr);
+            note_inv_info(c, w, B, B);
+            g(OP_N, OP_N, n, np, np, U, opGln(sp), T[2], n, nn);         // U Gln
+            g(OP_N, OP_C, n, n, np, op_member(T[2], n, nn), U, T[3], n, nn, 1.0, opSLS(sc),
+              1.0);                                                      // U Gln U^H + sl S
+            g(OP_N, OP_N, n, n, n, opGl(sc), op_member(T[3], n, nn), T[4], n, nn);
+            g(OP_N, OP_C, n, n, n, op_member(T[4], n, nn), opGl(sc), gln(sc), n, sq(sc));
+            sp = sc;
+        }
+        // ---------------- last slab of pass 1 (rgf_mod.f90:172-202 / 385-415)
+        double2* Gc = T[6];   // G of the slab just solved (n x n per point, stride nn)
+        double2* Lc = T[7];   // its G^<
+        double2* Gn = T[8];
+        double2* Ln = T[9];
+        {
+            const int n = nL, np = h.nm[sp];
+            const ZOp U = opC(fwd ? sl : sl + 1);
+            g(OP_N, OP_N, n, np, np, U, opGl(sp), T[0], n, nn);
+            ShiftArgs sh;
+            sh.n = n;
+            sh.batch = B;
+            sh.out = T[1];
+            sh.sout = nn;
+            sh.zre = w.zre;
+            sh.S = opS(sl);
+            sh.M1 = opH00(sl);
+            sh.c1 = -1.0;
+            sh.M2 = op_member(w.sigL, n, nn);
+            sh.c2 = -1.0;
+            sh.kidx = w.kidx;
+            shifted(s, sh);                                              // z S - H00 - Sigma
+            g(OP_N, OP_C, n, n, np, op_member(T[0], n, nn), U, T[1], n, nn, -1.0,
+              op_member(T[1], n, nn), 1.0);                              // - U Gl U^H
+            inv_batch(c, w, n, B, T[1], nn, Gc, nn, nullptr);
+            note_inv_info(c, w, B, B);
+            copy_blocks(s, n, B, gl(sl), sq(sl), op_member(Gc, n, nn), nullptr, 0, false);
+            g(OP_N, OP_N, n, np, np, U, opGln(sp), T[2], n, nn);
+            g(OP_N, OP_C, n, n, np, op_member(T[2], n, nn), U, T[3], n, nn, 1.0, opSLS(sl), 1.0);
+            AntiArgs an;  // sig = -(Sigma - Sigma^H) f + U Gln U^H + sl S
+            an.n = n;
+            an.batch = B;
+            an.out = T[4];
+            an.sout = nn;
+            an.X = op_member(w.sigL, n, nn);
+            an.afac = w.nfL;
+            an.Y = op_member(T[3], n, nn);
+            an.c1 = 1.0;
+            antiherm(s, an);
+            g(OP_N, OP_N, n, n, n, op_member(Gc, n, nn), op_member(T[4], n, nn), T[5], n, nn);
+            g(OP_N, OP_C, n, n, n, op_member(T[5], n, nn), op_member(Gc, n, nn), Lc, n, nn);
+            CurrentArgs ca;  // spectral current at the last contact
+            ca.n = n;
+            ca.batch = B;
+            ca.Sig = op_member(w.sigL, n, nn);
+            ca.G = op_member(Gc, n, nn);
+            ca.Gl = op_member(Lc, n, nn);
+            ca.f = w.fL;
+            ca.partial = w.partial;
+            ca.out = fwd ? w.tr : w.tre;
+            contact_current(s, ca);
+            diag_out(sl, Gc, Lc);
+            const double2* jd = nullptr;
+            if (want_j) {
+                // Jdens = Om (G^< + f (G - G^H)),  Om = -(Sigma - Sigma^H)
+                AntiArgs om;
+                om.n = n;
+                om.batch = B;
+                om.out = T[0];
+                om.sout = nn;
+                om.X = op_member(w.sigL, n, nn);
+                om.aconst = -1.0;
+                antiherm(s, om);
+                AntiArgs wv;
+                wv.n = n;
+                wv.batch = B;
+                wv.out = T[1];
+                wv.sout = nn;
+                wv.X = op_member(Gc, n, nn);
+                wv.afac = w.fL;
+                wv.Y = op_member(Lc, n, nn);
+                wv.c1 = 1.0;
+                antiherm(s, wv);
+                g(OP_N, OP_N, n, n, n, op_member(T[0], n, nn), op_member(T[1], n, nn), T[10], n, nn);
+                jd = T[10];
+            }
+            full_slab(sl, Gc, Lc, jd);
+        }
+        QT_HIP(hipEventRecord(c.ev[2], s));
+
+        // ---------------- pass 2 (rgf_mod.f90:212-250 / 420-458)
+        int sn = sl;
+        for (int sc = sl - step; sc != sf - step; sc -= step) {
+            const int n = h.nm[sc], nq = h.nm[sn];
+            // coupling between slab sc and its solved neighbour: the reference
+            // uses H1i(ii) in both variants; QT_F_COUPLING_FIX selects H1i(ii+1)
+            // in the forward variant
+            const ZOp W = opC(fwd && fix ? sc + 1 : sc);  // nq x n
+            double2 *X = T[0], *GN0 = T[1], *P = T[2], *Y = T[3], *Q = T[4], *U = T[5];
+            g(OP_N, OP_C, n, nq, n, opGl(sc), W, X, n, nn);                                   // X = Gl W^H
+            g(OP_N, OP_N, n, nq, nq, op_member(X, n, nn), op_member(Gc, nq, nn), GN0, n, nn);  // GN0 = X G
+            g(OP_N, OP_N, nq, n, n, W, opGl(sc), P, nq, nn);                                   // P = W Gl
+            g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(P, nq, nn), Gn, n, nn, 1.0,
+              opGl(sc), 1.0);                                                                  // G = Gl + GN0 P
+            g(OP_N, OP_N, n, nq, nq, op_member(X, n, nn), op_member(Lc, nq, nn), Y, n, nn);    // Y = X G<
+            g(OP_N, OP_C, n, n, nq, op_member(Y, n, nn), op_member(X, n, nn), U, n, nn, 1.0,
+              opGln(sc), 1.0);                                                                 // U = Gln + Y X^H
+            g(OP_N, OP_N, nq, n, n, W, opGln(sc), Q, nq, nn);                                  // Q = W Gln
+            g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(Q, nq, nn), Y, n, nn);    // Tm = GN0 Q
+            // G< = U + Tm - Tm^H: the reference's last term Gln W^H GN0^H equals
+            // -(GN0 W Gln)^H because Gln is anti-Hermitian
+            AntiArgs an;
+            an.n = n;
+            an.batch = B;
+            an.out = Ln;
+            an.sout = nn;
+            an.X = op_member(Y, n, nn);
+            an.aconst = 1.0;
+            an.Y = op_member(U, n, nn);
+            an.c1 = 1.0;
+            antiherm(s, an);
+            const double2* jd = nullptr;
+            if (want_j) {
+                // A = G<(nbr) X^H + G(nbr) Q ; Jdens = -2 W^H A  (rgf_mod.f90:215-221)
+                g(OP_N, OP_C, nq, n, nq, op_member(Lc, nq, nn), op_member(X, n, nn), U, nq, nn);
+                g(OP_N, OP_N, nq, n, nq, op_member(Gc, nq, nn), op_member(Q, nq, nn), U, nq, nn, 1.0,
+                  op_member(U, nq, nn), 1.0);
+                g(OP_C, OP_N, n, n, nq, W, op_member(U, nq, nn), T[10], n, nn, -2.0);
+                jd = T[10];
+            }
+            std::swap(Gc, Gn);
+            std::swap(Lc, Ln);
+            diag_out(sc, Gc, Lc);
+            full_slab(sc, Gc, Lc, jd);
+            sn = sc;
+        }
+        // ---------------- current at the first contact (rgf_mod.f90:256-266 / 459-469)
+        {
+            CurrentArgs ca;
+            ca.n = nF;
+            ca.batch = B;
+            ca.Sig = op_member(w.sigF, nF, nn);
+            ca.G = op_member(Gc, nF, nn);
+            ca.Gl = op_member(Lc, nF, nn);
+            ca.f = w.fF;
+            ca.partial = w.partial;
+            ca.out = fwd ? w.tre : w.tr;
+            contact_current(s, ca);
+        }
+        // Gl / Gln workspace lists of the reference interface
+        if (full.any) {
+            for (int sc = 0; sc < nx; ++sc) {
+                d2h_blocks(c, full.Gl, full.nx, full.p0, B, sc, h.nm[sc], gl(sc), sq(sc));
+                if (sc != sl) d2h_blocks(c, full.Gln, full.nx, full.p0, B, sc, h.nm[sc], gln(sc), sq(sc));
+            }
+        }
+        QT_HIP(hipEventRecord(c.ev[3], s));
+        QT_HIP(hipEventSynchronize(c.ev[3]));
+        float t01 = 0, t12 = 0, t23 = 0;
+        QT_HIP(hipEventElapsedTime(&t01, c.ev[0], c.ev[1]));
+        QT_HIP(hipEventElapsedTime(&t12, c.ev[1], c.ev[2]));
+        QT_HIP(hipEventElapsedTime(&t23, c.ev[2], c.ev[3]));
+        tm.lead_ms += t01;
+        tm.forward_ms += t12;
+        tm.backward_ms += t23;
+    }
+};
+
+static int status_of(const hip_error& e) {
+    g_last_error = e.what();
+    return e.code;
+}
+
+}  // namespace qt
+
+using namespace qt;
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* qt_status_string(int st) {
+    switch (st) {
+        case QT_OK: return "ok";
+        case QT_ERR_INVALID: return "invalid argument or shape";
+        case QT_ERR_NOT_INIT: return "qt_init() has not been called";
+        case QT_ERR_HIP: return g_last_error.empty() ? "HIP runtime error" : g_last_error.c_str();
+        case QT_ERR_NOMEM: return "device memory exhausted";
+        case QT_ERR_SINGULAR: return "singular block inversion (block zeroed)";
+        case QT_ERR_UNSUPPORTED: return "unsupported shape or option";
+        case QT_ERR_NO_DEVICE: return "no HIP device";
+        default: return "unknown status";
+    }
+}
+
+int qt_init(int device) {
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    try {
+        if (g_ctx) {
+            if (g_ctx->device == device) return QT_OK;
+            return QT_ERR_INVALID;
+        }
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return QT_ERR_NO_DEVICE;
+        if (device < 0 || device >= n) return QT_ERR_INVALID;
+        QT_HIP(hipSetDevice(device));
+        Ctx* c = new Ctx();
+        c->device = device;
+        QT_HIP(hipStreamCreate(&c->stream));
+        QT_HIP(hipHostMalloc((void**)&c->host_count, sizeof(int), hipHostMallocDefault));
+        for (int i = 0; i < 4; ++i) QT_HIP(hipEventCreate(&c->ev[i]));
+        g_ctx = c;
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+int qt_finish(void) {
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (!g_ctx) return QT_OK;
+    Ctx* c = g_ctx;
+    (void)hipStreamSynchronize(c->stream);
+    c->arena.release();
+    if (c->timer.created)
+        for (int k = 0; k < 2; ++k)
+            for (int i = 0; i < 4096; ++i) (void)hipEventDestroy(c->timer.ev[k][i]);
+    for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c->ev[i]);
+    (void)hipHostFree(c->host_count);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    g_ctx = nullptr;
+    return QT_OK;
+}
+
+int qt_ham_create(int32_t nx, const int32_t* nm, int32_t nk, const void* const* Hii,
+                  const void* const* H1i, const void* const* Sii, qt_ham** out) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (nx < 2 || nk < 1 || !nm || !Hii || !H1i || !out) return QT_ERR_INVALID;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    qt_ham* h = new qt_ham();
+    try {
+        h->nx = nx;
+        h->nk = nk;
+        h->nm.assign(nm, nm + nx);
+        h->hoff.resize(nx);
+        h->coff.resize(nx + 1);
+        int64_t o = 0;
+        for (int s = 0; s < nx; ++s) {
+            if (nm[s] <= 0 || nm[s] > 1024) throw hip_error("block size out of range", QT_ERR_INVALID);
+            h->hoff[s] = o;
+            o += (int64_t)nm[s] * nm[s];
+            h->nmax = std::max(h->nmax, nm[s]);
+            if (nm[s] != nm[0]) h->uniform = false;
+        }
+        h->hset = o;
+        o = 0;
+        for (int s = 0; s <= nx; ++s) {
+            h->coff[s] = o;
+            const int r = s < nx ? nm[s] : nm[nx - 1];
+            const int cc = s == 0 ? nm[0] : nm[s - 1];
+            o += (int64_t)r * cc;
+        }
+        h->cset = o;
+        QT_HIP(hipMalloc((void**)&h->H, sizeof(double2) * h->hset * nk));
+        QT_HIP(hipMalloc((void**)&h->C, sizeof(double2) * h->cset * nk));
+        if (Sii) QT_HIP(hipMalloc((void**)&h->S, sizeof(double2) * h->hset * nk));
+        for (int k = 0; k < nk; ++k) {
+            for (int s = 0; s < nx; ++s) {
+                const size_t bytes = sizeof(double2) * (size_t)nm[s] * nm[s];
+                if (!Hii[(int64_t)k * nx + s]) throw hip_error("null Hii block", QT_ERR_INVALID);
+                QT_HIP(hipMemcpy(h->H + k * h->hset + h->hoff[s], Hii[(int64_t)k * nx + s], bytes,
+                                 hipMemcpyHostToDevice));
+                if (Sii) {
+                    if (!Sii[(int64_t)k * nx + s]) throw hip_error("null Sii block", QT_ERR_INVALID);
+                    QT_HIP(hipMemcpy(h->S + k * h->hset + h->hoff[s], Sii[(int64_t)k * nx + s],
+                                     bytes, hipMemcpyHostToDevice));
+                }
+            }
+            for (int s = 0; s <= nx; ++s) {
+                const int r = s < nx ? nm[s] : nm[nx - 1];
+                const int cc = s == 0 ? nm[0] : nm[s - 1];
+                if (!H1i[(int64_t)k * (nx + 1) + s]) throw hip_error("null H1i block", QT_ERR_INVALID);
+                QT_HIP(hipMemcpy(h->C + k * h->cset + h->coff[s], H1i[(int64_t)k * (nx + 1) + s],
+                                 sizeof(double2) * (size_t)r * cc, hipMemcpyHostToDevice));
+            }
+        }
+        *out = h;
+        return QT_OK;
+    } catch (const hip_error& e) {
+        if (h->H) (void)hipFree(h->H);
+        if (h->C) (void)hipFree(h->C);
+        if (h->S) (void)hipFree(h->S);
+        delete h;
+        return status_of(e);
+    }
+}
+
+int qt_ham_destroy(qt_ham* h) {
+    if (!h) return QT_OK;
+    if (g_ctx) {
+        std::lock_guard<std::mutex> lk(g_ctx->mu);
+        (void)hipStreamSynchronize(g_ctx->stream);
+    }
+    if (h->H) (void)hipFree(h->H);
+    if (h->C) (void)hipFree(h->C);
+    if (h->S) (void)hipFree(h->S);
+    delete h;
+    return QT_OK;
+}
+
+static int choose_batch(qt_ham& h, int npts, bool sigma) {
+    const char* env = getenv("QT_BATCH");
+    if (env && atoi(env) > 0) return std::min(npts, atoi(env));
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) freeb = (size_t)8 << 30;
+    const size_t budget = (size_t)(0.80 * (double)(freeb + g_ctx->arena.cap));
+    int B = std::min(npts, 128);
+    while (B > 1 && work_bytes(h, B, sigma) > budget) B = B * 3 / 4;
+    return std::max(B, 1);
+}
+
+int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* En,
+                 const int32_t* kidx, const void* const* sigma_lesser_ph,
+                 const void* const* sigma_r_ph, qt_rgf_outputs* out) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (!h || !p || npts < 0 || (npts > 0 && !En) || !out) return QT_ERR_INVALID;
+    if (npts == 0) return QT_OK;
+    if (p->precision != QT_PREC_FP64) return QT_ERR_UNSUPPORTED;
+    const bool fwd = p->variant == QT_RGF_FORWARD;
+    if (!fwd && p->variant != QT_RGF_BACKWARD) return QT_ERR_INVALID;
+    const bool fix = (p->flags & QT_F_COUPLING_FIX) != 0;
+    if (!h->uniform && (!fwd || !fix)) return QT_ERR_UNSUPPORTED;
+    if ((sigma_lesser_ph == nullptr) != (sigma_r_ph == nullptr)) return QT_ERR_INVALID;
+    for (int i = 0; i < npts; ++i)
+        if (kidx && (kidx[i] < 0 || kidx[i] >= h->nk)) return QT_ERR_INVALID;
+    const bool dev_out = (p->flags & QT_F_DEVICE_OUTPUTS) != 0;
+    const bool sigma = sigma_r_ph != nullptr;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        const int B = choose_batch(*h, npts, sigma);
+        c.arena.reserve(work_bytes(*h, B, sigma));
+        Work w;
+        work_carve(c.arena, w, *h, B, sigma);
+        qt_timing tm{};
+        tm.batch = B;
+        c.timer.reset();
+        hipEvent_t e0, e1;
+        QT_HIP(hipEventCreate(&e0));
+        QT_HIP(hipEventCreate(&e1));
+        QT_HIP(hipEventRecord(e0, c.stream));
+        std::vector<int64_t> doff(h->nx);
+        int64_t od = 0;
+        for (int s = 0; s < h->nx; ++s) { doff[s] = od; od += h->nm[s]; }
+        const int64_t ndiag = od;
+        std::vector<double> hz(2 * B), hf(4 * B);
+        std::vector<int> hk(2 * B);
+        std::vector<int> hinfo(B), hit(2 * B);
+        std::vector<double> htr(B), htre(B);
+        int status = QT_OK;
+        for (int p0 = 0; p0 < npts; p0 += B) {
+            const int nb = std::min(B, npts - p0);
+            // pad the batch by repeating its last point (results discarded)
+            for (int b = 0; b < B; ++b) {
+                const int ip = p0 + std::min(b, nb - 1);
+                hz[b] = hz[B + b] = En[ip];
+                hk[b] = hk[B + b] = kidx ? kidx[ip] : 0;
+                // occupations: rgf_mod.f90:100,182 (forward) / :321,395 (backward):
+                // the first contact uses (mur, TEMPr), the last one (mul, TEMPl)
+                const double fF = ferm((En[ip] - p->mur) / (BOLTZ * p->tempr));
+                const double fL = ferm((En[ip] - p->mul) / (BOLTZ * p->templ));
+                hf[b] = fF;
+                hf[B + b] = fL;
+                hf[2 * B + b] = -fF;
+                hf[3 * B + b] = -fL;
+            }
+            QT_HIP(hipMemcpyAsync(w.zre, hz.data(), sizeof(double) * 2 * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.kidx, hk.data(), sizeof(int) * 2 * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.fF, hf.data(), sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.fL, hf.data() + B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.nfF, hf.data() + 2 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.nfL, hf.data() + 3 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+            fill_int(c.stream, w.infoacc, B, 0);
+            if (sigma) {
+                // H00 = Hii + sigma_r_ph * S,  SLS = sigma_lesser_ph * S  (rgf_mod.f90:120-121, 132)
+                for (int s = 0; s < h->nx; ++s) {
+                    const int n = h->nm[s];
+                    const size_t bytes = sizeof(double2) * (size_t)n * n;
+                    double2* hr = w.H00 + h->hoff[s] * B;
+                    double2* hl = w.SLS + h->hoff[s] * B;
+                    for (int b = 0; b < B; ++b) {
+                        const int ip = p0 + std::min(b, nb - 1);
+                        const void* sr = sigma_r_ph[(int64_t)ip * h->nx + s];
+                        const void* sl = sigma_lesser_ph[(int64_t)ip * h->nx + s];
+                        QT_HIP(hipMemcpy(w.T[0] + b * w.nn, sr, bytes, hipMemcpyHostToDevice));
+                        QT_HIP(hipMemcpy(w.T[1] + b * w.nn, sl, bytes, hipMemcpyHostToDevice));
+                    }
+                    Gemm g{c.stream, &c.timer, B, w.kidx};
+                    ZOp S = h->S ? op_k(h->S + h->hoff[s], n, h->hset) : ZOp();
+                    if (S.p) {
+                        g(OP_N, OP_N, n, n, n, op_member(w.T[0], n, w.nn), S, w.T[2], n, w.nn);
+                        g(OP_N, OP_N, n, n, n, op_member(w.T[1], n, w.nn), S, hl, n, (int64_t)n * n);
+                    } else {
+                        copy_blocks(c.stream, n, B, hl, (int64_t)n * n, op_member(w.T[1], n, w.nn),
+                                    nullptr, 0, false);
+                    }
+                    ShiftArgs sh;
+                    sh.n = n;
+                    sh.batch = B;
+                    sh.out = hr;
+                    sh.sout = (int64_t)n * n;
+                    sh.M1 = op_k(h->H + h->hoff[s], n, h->hset);
+                    sh.c1 = 1.0;
+                    sh.M2 = op_member(S.p ? w.T[2] : w.T[0], n, w.nn);
+                    sh.c2 = 1.0;
+                    sh.kidx = w.kidx;
+                    shifted(c.stream, sh);
+                }
+            }
+            Solve sv{c, *h, *p, w, B, fwd, fix, sigma};
+            sv.doff = doff;
+            const bool want_diag = out->gr_diag || out->gl_diag || out->gg_diag;
+            if (want_diag) {
+                if (dev_out) {
+                    sv.gr = out->gr_diag ? (double2*)out->gr_diag + (int64_t)p0 * ndiag : w.diag;
+                    sv.glo = out->gl_diag ? (double2*)out->gl_diag + (int64_t)p0 * ndiag : w.diag + B * ndiag;
+                    sv.gg = out->gg_diag ? (double2*)out->gg_diag + (int64_t)p0 * ndiag : w.diag + 2 * B * ndiag;
+                    if (nb < B) {  // padded members must not write past the caller's arrays
+                        sv.gr = w.diag;
+                        sv.glo = w.diag + B * ndiag;
+                        sv.gg = w.diag + 2 * B * ndiag;
+                    }
+                } else {
+                    sv.gr = w.diag;
+                    sv.glo = w.diag + B * ndiag;
+                    sv.gg = w.diag + 2 * B * ndiag;
+                }
+            }
+            sv.full.any = out->G_r || out->G_lesser || out->G_greater || out->Jdens || out->Gl || out->Gln;
+            if (sv.full.any) {
+                if (nb < B) {
+                    // full-block outputs are written per real point only
+                }
+                sv.full.G_r = out->G_r;
+                sv.full.G_lesser = out->G_lesser;
+                sv.full.G_greater = out->G_greater;
+                sv.full.Jdens = out->Jdens;
+                sv.full.Gl = out->Gl;
+                sv.full.Gln = out->Gln;
+                sv.full.nx = h->nx;
+                sv.full.p0 = p0;
+            }
+            if (sv.full.any && nb < B) {
+                // restrict host block copies to the real points of this batch
+                static thread_local std::vector<void*> bufs[6];
+                void* const* src[6] = {out->G_r, out->G_lesser, out->G_greater, out->Jdens, out->Gl, out->Gln};
+                void* const** dst[6] = {&sv.full.G_r, &sv.full.G_lesser, &sv.full.G_greater,
+                                        &sv.full.Jdens, &sv.full.Gl, &sv.full.Gln};
+                for (int q = 0; q < 6; ++q) {
+                    if (!src[q]) continue;
+                    bufs[q].assign((size_t)(p0 + B) * h->nx, nullptr);
+                    for (int ip = p0; ip < p0 + nb; ++ip)
+                        for (int s = 0; s < h->nx; ++s)
+                            bufs[q][(size_t)ip * h->nx + s] = src[q][(size_t)ip * h->nx + s];
+                    *dst[q] = bufs[q].data();
+                }
+            }
+            sv.run(tm);
+            tm.lead_iterations += sv.lead_iterations;
+            // per-point scalars
+            if (dev_out) {
+                if (out->tr) QT_HIP(hipMemcpyAsync(out->tr + p0, w.tr, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
+                if (out->tre) QT_HIP(hipMemcpyAsync(out->tre + p0, w.tre, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
+                if (out->info) QT_HIP(hipMemcpyAsync(out->info + p0, w.infoacc, sizeof(int) * nb, hipMemcpyDeviceToDevice, c.stream));
+                if (want_diag && nb < B) {
+                    if (out->gr_diag) QT_HIP(hipMemcpyAsync((double2*)out->gr_diag + (int64_t)p0 * ndiag, w.diag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToDevice, c.stream));
+                    if (out->gl_diag) QT_HIP(hipMemcpyAsync((double2*)out->gl_diag + (int64_t)p0 * ndiag, w.diag + B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToDevice, c.stream));
+                    if (out->gg_diag) QT_HIP(hipMemcpyAsync((double2*)out->gg_diag + (int64_t)p0 * ndiag, w.diag + 2 * B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToDevice, c.stream));
+                }
+                QT_HIP(hipMemcpyAsync(hinfo.data(), w.infoacc, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+            } else {
+                QT_HIP(hipMemcpyAsync(htr.data(), w.tr, sizeof(double) * B, hipMemcpyDeviceToHost, c.stream));
+                QT_HIP(hipMemcpyAsync(htre.data(), w.tre, sizeof(double) * B, hipMemcpyDeviceToHost, c.stream));
+                QT_HIP(hipMemcpyAsync(hinfo.data(), w.infoacc, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+                if (out->gr_diag) QT_HIP(hipMemcpyAsync((double2*)out->gr_diag + (int64_t)p0 * ndiag, w.diag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToHost, c.stream));
+                if (out->gl_diag) QT_HIP(hipMemcpyAsync((double2*)out->gl_diag + (int64_t)p0 * ndiag, w.diag + B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToHost, c.stream));
+                if (out->gg_diag) QT_HIP(hipMemcpyAsync((double2*)out->gg_diag + (int64_t)p0 * ndiag, w.diag + 2 * B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToHost, c.stream));
+            }
+            QT_HIP(hipMemcpyAsync(hit.data(), w.lead_it, sizeof(int) * 2 * B, hipMemcpyDeviceToHost, c.stream));
+            QT_HIP(hipStreamSynchronize(c.stream));
+            for (int b = 0; b < nb; ++b) {
+                if (!dev_out) {
+                    if (out->tr) out->tr[p0 + b] = htr[b];
+                    if (out->tre) out->tre[p0 + b] = htre[b];
+                    if (out->info) out->info[p0 + b] = hinfo[b];
+                }
+                if (out->lead_iters && !dev_out) {
+                    out->lead_iters[2 * (p0 + b)] = hit[b];
+                    out->lead_iters[2 * (p0 + b) + 1] = hit[B + b];
+                }
+                if (hinfo[b] & QT_INFO_SINGULAR) status = QT_ERR_SINGULAR;
+            }
+        }
+        QT_HIP(hipEventRecord(e1, c.stream));
+        QT_HIP(hipEventSynchronize(e1));
+        float tot = 0;
+        QT_HIP(hipEventElapsedTime(&tot, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c.timer.flush();
+        tm.total_ms = tot;
+        tm.gemm_ms = c.timer.ms[0];
+        tm.gemm_flops = c.timer.flops[0];
+        tm.gemm_launches = c.timer.launches[0];
+        tm.inv_ms = c.timer.ms[1];
+        tm.inv_flops = c.timer.flops[1];
+        tm.inv_launches = c.timer.launches[1];
+        c.last = tm;
+        return status;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+int qt_sancho(int32_t nm, int32_t npts, const double* En, double eta, int32_t nmax, double tol,
+              int32_t reset_on_fail, const void* S00, const void* const* H00, const void* const* H10,
+              void* const* G00, void* const* GBB, int32_t* iters, int32_t* converged) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (nm <= 0 || nm > 1024 || npts < 0 || !En || !H00 || !H10 || !G00) return QT_ERR_INVALID;
+    if (npts == 0) return QT_OK;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        // a one-slab pseudo Hamiltonian sizes the workspace
+        qt_ham h;
+        h.nx = 1;
+        h.nk = 1;
+        h.nm = {nm};
+        h.hoff = {0};
+        h.hset = (int64_t)nm * nm;
+        h.nmax = nm;
+        int B = std::min(npts, 256);
+        size_t freeb = 0, total = 0;
+        (void)hipMemGetInfo(&freeb, &total);
+        while (B > 1 && work_bytes(h, (B + 1) / 2, false) > 0.8 * (freeb + c.arena.cap)) B = B * 3 / 4;
+        const int half = (B + 1) / 2;  // the workspace holds 2*half members
+        c.arena.reserve(work_bytes(h, half, false) + sizeof(double2) * (size_t)nm * nm * (2 * half + 2));
+        Work w;
+        work_carve(c.arena, w, h, half, false);
+        double2* Sdev = S00 ? c.arena.take<double2>((size_t)nm * nm) : nullptr;
+        double2* GBBdev = GBB ? c.arena.take<double2>((size_t)nm * nm * 2 * half) : nullptr;
+        if (Sdev) QT_HIP(hipMemcpy(Sdev, S00, sizeof(double2) * nm * nm, hipMemcpyHostToDevice));
+        const int64_t nn = (int64_t)nm * nm;
+        std::vector<double> hz(2 * half);
+        std::vector<int> hit(2 * half), hact(2 * half);
+        for (int p0 = 0; p0 < npts; p0 += B) {
+            const int nb = std::min(B, npts - p0);
+            for (int m = 0; m < nb; ++m) {
+                hz[m] = En[p0 + m];
+                QT_HIP(hipMemcpy(w.T[13] + m * nn, H00[p0 + m], sizeof(double2) * nn, hipMemcpyHostToDevice));
+                QT_HIP(hipMemcpy(w.T[11] + m * nn, H10[p0 + m], sizeof(double2) * nn, hipMemcpyHostToDevice));
+            }
+            QT_HIP(hipMemcpy(w.zre, hz.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
+            SanchoIO io;
+            io.n = nm;
+            io.members = nb;
+            io.nn = nn;
+            io.zre = w.zre;
+            io.kidx = nullptr;
+            io.eta = eta;
+            io.nmax = nmax;
+            io.tol = tol;
+            io.reset = reset_on_fail != 0;
+            io.S = Sdev ? op_k(Sdev, nm, 0) : ZOp();
+            io.H00 = op_member(w.T[13], nm, nn);
+            io.H10 = w.T[11];
+            io.G00 = w.T[12];
+            io.GBB = GBBdev;
+            io.iters_out = w.lead_it;
+            sancho_run(c, w, io);
+            QT_HIP(hipStreamSynchronize(c.stream));
+            QT_HIP(hipMemcpy(hit.data(), w.lead_it, sizeof(int) * nb, hipMemcpyDeviceToHost));
+            QT_HIP(hipMemcpy(hact.data(), w.active, sizeof(int) * nb, hipMemcpyDeviceToHost));
+            for (int m = 0; m < nb; ++m) {
+                QT_HIP(hipMemcpy(G00[p0 + m], w.T[12] + m * nn, sizeof(double2) * nn, hipMemcpyDeviceToHost));
+                if (GBB && GBB[p0 + m])
+                    QT_HIP(hipMemcpy(GBB[p0 + m], GBBdev + m * nn, sizeof(double2) * nn, hipMemcpyDeviceToHost));
+                if (iters) iters[p0 + m] = hit[m];
+                if (converged) converged[p0 + m] = hact[m] ? 0 : 1;
+            }
+        }
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+int qt_zgemm_batched(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_t k, int32_t batch,
+                     double alpha, const void* A, int64_t lda, int64_t strideA, const void* B,
+                     int64_t ldb, int64_t strideB, double beta, void* R, int64_t ldr,
+                     int64_t strideR) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (m <= 0 || n <= 0 || k < 0 || batch <= 0 || !A || !B || !R) return QT_ERR_INVALID;
+    if (opa < 0 || opa > 2 || opb < 0 || opb > 2) return QT_ERR_INVALID;
+    const int64_t ar = opa == OP_N ? m : k, ac = opa == OP_N ? k : m;
+    const int64_t br = opb == OP_N ? k : n, bc = opb == OP_N ? n : k;
+    if (lda < ar || ldb < br || ldr < m) return QT_ERR_INVALID;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        // extents of the three operands (elements)
+        const int64_t ea = (batch - 1) * strideA + lda * (ac - 1) + ar;
+        const int64_t eb = (batch - 1) * strideB + ldb * (bc - 1) + br;
+        const int64_t er = (batch - 1) * strideR + ldr * (n - 1) + m;
+        c.arena.reserve(sizeof(double2) * (size_t)(ea + eb + er) + 4096);
+        c.arena.reset();
+        double2* dA = c.arena.take<double2>(ea);
+        double2* dB = c.arena.take<double2>(eb);
+        double2* dR = c.arena.take<double2>(er);
+        QT_HIP(hipMemcpy(dA, A, sizeof(double2) * ea, hipMemcpyHostToDevice));
+        QT_HIP(hipMemcpy(dB, B, sizeof(double2) * eb, hipMemcpyHostToDevice));
+        if (beta != 0.0) QT_HIP(hipMemcpy(dR, R, sizeof(double2) * er, hipMemcpyHostToDevice));
+        ZGemmArgs g;
+        g.M = m;
+        g.N = n;
+        g.K = k;
+        g.batch = batch;
+        g.A = op_member(dA, lda, strideA);
+        g.B = op_member(dB, ldb, strideB);
+        g.C = dR;
+        g.ldc = ldr;
+        g.sbc = strideR;
+        if (beta != 0.0) g.D = op_member(dR, ldr, strideR);
+        g.alpha = alpha;
+        g.beta = beta;
+        zgemm(c.stream, opa, opb, g, &c.timer);
+        QT_HIP(hipStreamSynchronize(c.stream));
+        QT_HIP(hipMemcpy(R, dR, sizeof(double2) * er, hipMemcpyDeviceToHost));
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+int qt_invert_batched(int32_t n, int32_t batch, void* A, int32_t* info) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (n <= 0 || n > 1024 || batch <= 0 || !A) return QT_ERR_INVALID;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        const int64_t nn = (int64_t)n * n;
+        c.arena.reserve(sizeof(double2) * (size_t)(2 * nn * batch + 16 * (int64_t)n * batch) +
+                        sizeof(int) * (size_t)(n + 1) * batch + 8192);
+        c.arena.reset();
+        Work w;
+        double2* dA = c.arena.take<double2>(nn * batch);
+        double2* dX = c.arena.take<double2>(nn * batch);
+        w.R = c.arena.take<double2>(16 * (int64_t)n * batch);
+        w.piv = c.arena.take<int>((int64_t)n * batch);
+        w.info = c.arena.take<int>(batch);
+        QT_HIP(hipMemcpy(dA, A, sizeof(double2) * nn * batch, hipMemcpyHostToDevice));
+        inv_batch(c, w, n, batch, dA, nn, dX, nn, nullptr);
+        QT_HIP(hipStreamSynchronize(c.stream));
+        QT_HIP(hipMemcpy(A, dX, sizeof(double2) * nn * batch, hipMemcpyDeviceToHost));
+        std::vector<int> hinfo(batch);
+        QT_HIP(hipMemcpy(hinfo.data(), w.info, sizeof(int) * batch, hipMemcpyDeviceToHost));
+        int st = QT_OK;
+        for (int b = 0; b < batch; ++b) {
+            if (info) info[b] = hinfo[b];
+            if (hinfo[b]) st = QT_ERR_SINGULAR;
+        }
+        return st;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+int qt_device_alloc(size_t bytes, void** ptr) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    try {
+        QT_HIP(hipMalloc(ptr, bytes));
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+int qt_device_free(void* ptr) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    return hipFree(ptr) == hipSuccess ? QT_OK : QT_ERR_HIP;
+}
+int qt_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? QT_OK : QT_ERR_HIP;
+}
+int qt_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? QT_OK : QT_ERR_HIP;
+}
+int qt_synchronize(void) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    return hipStreamSynchronize(g_ctx->stream) == hipSuccess ? QT_OK : QT_ERR_HIP;
+}
+int qt_set_kernel_timing(int enable) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    g_ctx->timer.enabled = enable != 0;
+    return QT_OK;
+}
+int qt_last_timing(qt_timing* t) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (!t) return QT_ERR_INVALID;
+    *t = g_ctx->last;
+    return QT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+! output_mod.f90 -- spectrum writers of the NEGF driver (module Output).
+!
+! Record layout of the reference writer (output_mod.f90:14-61 there): one
+! line per cell and energy, '(4E18.4)' = x, E, coeff(1)*Re tr, coeff(2)*Im tr
+! with tr the k-averaged trace of the cell's diagonal block, and an empty
+! record after each energy.  File name: dataset // '%4.4i' // '.dat'.
+module Output
+    use matrix_c, only: type_matrix_complex
+    implicit none
+    private
+    integer, parameter :: dp = 8
+    public :: write_spectrum_summed_over_k, write_spectrum_from_diagonals
+
+contains
+
+    subroutine open_spectrum(dataset, i, append)
+        character(len=*), intent(in) :: dataset
+        integer, intent(in) :: i
+        logical, intent(in) :: append
+        character(len=4) :: tag
+        write (tag, '(I4.4)') i
+        if (append) then
+            open (unit=11, file=trim(dataset)//tag//'.dat', status='unknown', position='append')
+        else
+            open (unit=11, file=trim(dataset)//tag//'.dat', status='unknown')
+        end if
+    end subroutine open_spectrum
+
+    ! G(length, nen, nk): diagonal blocks per slab, energy and k-point
+    subroutine write_spectrum_summed_over_k(dataset, i, G, nen, en, nk, length, NB, NS, Lx, coeff, append)
+        character(len=*), intent(in) :: dataset
+        type(type_matrix_complex), intent(in), dimension(length, nen, nk) :: G
+        integer, intent(in) :: i, nen, length, NB, NS, nk
+        real(dp), intent(in) :: Lx, en(nen), coeff(2)
+        logical, intent(in), optional :: append
+        logical :: app
+        integer :: ie, j, c, ik, b, cell
+        complex(dp) :: t
+        app = .false.
+        if (present(append)) app = append
+        call open_spectrum(dataset, i, app)
+        do ie = 1, nen
+            cell = 0
+            do j = 1, length
+                do c = 1, NS
+                    cell = cell + 1
+                    t = (0d0, 0d0)
+                    do ik = 1, nk
+                        do b = 1, NB
+                            t = t + G(j, ie, ik)%m((c - 1)*NB + b, (c - 1)*NB + b)
+                        end do
+                    end do
+                    t = t/dble(nk)
+                    write (11, '(4E18.4)') dble(cell)*Lx, en(ie), dble(t)*coeff(1), aimag(t)*coeff(2)
+                end do
+            end do
+            write (11, '(A)') ''
+        end do
+        close (11)
+    end subroutine write_spectrum_summed_over_k
+
+    ! D(nd, nk, nen): per energy and k-point the concatenated diagonals of
+    ! the length slabs (nd = length*NS*NB)
+    subroutine write_spectrum_from_diagonals(dataset, i, D, nen, en, nk, length, NB, NS, Lx, coeff, append)
+        character(len=*), intent(in) :: dataset
+        integer, intent(in) :: i, nen, length, NB, NS, nk
+        complex(dp), intent(in) :: D(length*NS*NB, nk, nen)
+        real(dp), intent(in) :: Lx, en(nen), coeff(2)
+        logical, intent(in), optional :: append
+        logical :: app
+        integer :: ie, cell, ik, o
+        complex(dp) :: t
+        app = .false.
+        if (present(append)) app = append
+        call open_spectrum(dataset, i, app)
+        do ie = 1, nen
+            do cell = 1, length*NS
+                o = (cell - 1)*NB
+                t = (0d0, 0d0)
+                do ik = 1, nk
+                    t = t + sum(D(o + 1:o + NB, ik, ie))
+                end do
+                t = t/dble(nk)
+                write (11, '(4E18.4)') dble(cell)*Lx, en(ie), dble(t)*coeff(1), aimag(t)*coeff(2)
+            end do
+            write (11, '(A)') ''
+        end do
+        close (11)
+    end subroutine write_spectrum_from_diagonals
+
+end module Output

@@ -1,0 +1,373 @@
+"""Python host layer over the C ABI of libqtrgf.so (include/qt_rgf.h).
+
+Mirrors the reference's RGF entry points for Python callers and tests:
+
+* :func:`rgf_variableblock_forward` / :func:`rgf_variableblock_backward`
+  -- rgf_mod.f90:46-269 / 272-472, one energy point, full block lists;
+* :func:`solve_batch` -- the E x k loop of negf_mod.f90:115-127 as one
+  batched device call returning the diagonal blocks and contact currents;
+* :func:`sancho` -- the lead decimation of rgf_mod.f90:482-568.
+
+There is no CPU fallback: every call goes to the HIP library and raises
+:class:`QTError` when it is missing or no device is visible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libqtrgf.so")
+
+QT_RGF_FORWARD = 0
+QT_RGF_BACKWARD = 1
+QT_PREC_FP64 = 0
+QT_F_COUPLING_FIX = 0x1
+QT_F_DEVICE_OUTPUTS = 0x2
+QT_F_LEAD_RESET = 0x4
+QT_INFO_SINGULAR = 0x1
+QT_INFO_LEAD_NMAX = 0x2
+QT_ERR_SINGULAR = 5
+
+EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham_destroy",
+            "qt_rgf_solve", "qt_sancho", "qt_device_alloc", "qt_device_free", "qt_memcpy_h2d",
+            "qt_memcpy_d2h", "qt_synchronize", "qt_set_kernel_timing", "qt_last_timing",
+            "qt_zgemm_batched", "qt_invert_batched"]
+
+
+class QTError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"qt status {status}: {msg}")
+        self.status = status
+
+
+class Params(C.Structure):
+    _fields_ = [("variant", C.c_int32), ("precision", C.c_int32), ("flags", C.c_int32),
+                ("lead_nmax", C.c_int32), ("mul", C.c_double), ("mur", C.c_double),
+                ("templ", C.c_double), ("tempr", C.c_double), ("lead_eta", C.c_double),
+                ("lead_tol", C.c_double)]
+
+
+class Outputs(C.Structure):
+    _fields_ = [("tr", C.POINTER(C.c_double)), ("tre", C.POINTER(C.c_double)),
+                ("lead_iters", C.POINTER(C.c_int32)), ("info", C.POINTER(C.c_int32)),
+                ("gr_diag", C.c_void_p), ("gl_diag", C.c_void_p), ("gg_diag", C.c_void_p),
+                ("G_r", C.POINTER(C.c_void_p)), ("G_lesser", C.POINTER(C.c_void_p)),
+                ("G_greater", C.POINTER(C.c_void_p)), ("Jdens", C.POINTER(C.c_void_p)),
+                ("Gl", C.POINTER(C.c_void_p)), ("Gln", C.POINTER(C.c_void_p))]
+
+
+class Timing(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("lead_ms", C.c_double), ("forward_ms", C.c_double),
+                ("backward_ms", C.c_double), ("gemm_ms", C.c_double), ("gemm_flops", C.c_double),
+                ("inv_ms", C.c_double), ("inv_flops", C.c_double), ("gemm_launches", C.c_int64),
+                ("inv_launches", C.c_int64), ("lead_iterations", C.c_int64), ("batch", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib: Optional[C.CDLL] = None
+_inited = False
+
+
+def lib() -> C.CDLL:
+    """Loads libqtrgf.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise QTError(-1, f"{LIBPATH} not built (python -m quantum_transport_amd.build)")
+        L = C.CDLL(LIBPATH)
+        L.qt_status_string.restype = C.c_char_p
+        L.qt_status_string.argtypes = [C.c_int]
+        for name in EXPORTED:
+            if name != "qt_status_string":
+                getattr(L, name).restype = C.c_int
+        L.qt_device_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+        L.qt_device_free.argtypes = [C.c_void_p]
+        L.qt_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.qt_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+def check(st: int) -> int:
+    if st != 0:
+        raise QTError(st, lib().qt_status_string(st).decode())
+    return st
+
+
+def init(device: int = 0) -> None:
+    global _inited
+    if not _inited:
+        check(lib().qt_init(C.c_int(device)))
+        _inited = True
+
+
+def finish() -> None:
+    global _inited
+    if _inited:
+        check(lib().qt_finish())
+        _inited = False
+
+
+def _f(a) -> np.ndarray:
+    return np.asfortranarray(a, dtype=np.complex128)
+
+
+def _ptrs(arrs: Sequence[Optional[np.ndarray]]):
+    return (C.c_void_p * len(arrs))(*[None if a is None else a.ctypes.data for a in arrs])
+
+
+class Hamiltonian:
+    """Device-resident block-tridiagonal Hamiltonian (qt_ham_create).
+
+    Hii[ik][s] (nx blocks per k-point), H1i[ik][s] (nx+1 blocks,
+    H1i[s] = H_{s,s-1}), optional Sii[ik][s] (None = identity)."""
+
+    def __init__(self, Hii: List[List[np.ndarray]], H1i: List[List[np.ndarray]],
+                 Sii: Optional[List[List[np.ndarray]]] = None):
+        init()
+        self.nk = len(Hii)
+        self.nx = len(Hii[0])
+        self.nm = [int(h.shape[0]) for h in Hii[0]]
+        self._keep = [_f(h) for ik in range(self.nk) for h in Hii[ik]]
+        self._keep1 = [_f(h) for ik in range(self.nk) for h in H1i[ik]]
+        self._keepS = None if Sii is None else [_f(h) for ik in range(self.nk) for h in Sii[ik]]
+        nm = (C.c_int32 * self.nx)(*self.nm)
+        h = C.c_void_p()
+        check(lib().qt_ham_create(C.c_int32(self.nx), nm, C.c_int32(self.nk), _ptrs(self._keep),
+                                  _ptrs(self._keep1),
+                                  None if self._keepS is None else _ptrs(self._keepS), C.byref(h)))
+        self.handle = h
+        self._keep = self._keep1 = self._keepS = None  # uploaded
+
+    @classmethod
+    def from_device(cls, dev) -> "Hamiltonian":
+        return cls(dev.Hii, dev.H1i, dev.Sii)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().qt_ham_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_params(variant: str = "forward", mul: float = 0.0, mur: float = 0.0,
+                templ: float = 300.0, tempr: float = 300.0, coupling_fix: bool = False,
+                lead_eta: float = 1e-5, lead_nmax: int = 50, lead_tol: float = 1e-100,
+                lead_reset: bool = False, device_outputs: bool = False) -> Params:
+    flags = (QT_F_COUPLING_FIX if coupling_fix else 0) | (QT_F_DEVICE_OUTPUTS if device_outputs else 0) \
+        | (QT_F_LEAD_RESET if lead_reset else 0)
+    return Params(QT_RGF_FORWARD if variant == "forward" else QT_RGF_BACKWARD, QT_PREC_FP64, flags,
+                  lead_nmax, mul, mur, templ, tempr, lead_eta, lead_tol)
+
+
+def solve_batch(H: Hamiltonian, En, kidx=None, *, variant: str = "forward", mul: float = 0.0,
+                mur: float = 0.0, templ: float = 300.0, tempr: float = 300.0,
+                coupling_fix: bool = False, sigma_lesser_ph=None, sigma_r_ph=None,
+                diag: bool = True, full: bool = False, lead_eta: float = 1e-5,
+                lead_nmax: int = 50, raise_singular: bool = False) -> dict:
+    """Batched RGF over points (En[ip], kidx[ip]).  Returns tr, tre,
+    lead_iters, info and (diag=True) the diagonals of G^R, G^<, G^> per
+    slab, [npts, sum(nm)]; with full=True also the full block lists
+    G_r/G_lesser/G_greater/Jdens/Gl/Gln as [npts][nx] arrays."""
+    En = np.ascontiguousarray(En, dtype=np.float64)
+    npts = len(En)
+    kidx = np.zeros(npts, np.int32) if kidx is None else np.ascontiguousarray(kidx, dtype=np.int32)
+    nd = sum(H.nm)
+    res = dict(tr=np.zeros(npts), tre=np.zeros(npts), lead_iters=np.zeros((npts, 2), np.int32),
+               info=np.zeros(npts, np.int32))
+    o = Outputs()
+    o.tr = res["tr"].ctypes.data_as(C.POINTER(C.c_double))
+    o.tre = res["tre"].ctypes.data_as(C.POINTER(C.c_double))
+    o.lead_iters = res["lead_iters"].ctypes.data_as(C.POINTER(C.c_int32))
+    o.info = res["info"].ctypes.data_as(C.POINTER(C.c_int32))
+    if diag:
+        for k in ("gr_diag", "gl_diag", "gg_diag"):
+            res[k] = np.zeros((npts, nd), np.complex128)
+            setattr(o, k, res[k].ctypes.data)
+    keep = []
+    if full:
+        for k in ("G_r", "G_lesser", "G_greater", "Jdens", "Gl", "Gln"):
+            blocks = [[np.zeros((n, n), np.complex128, order="F") for n in H.nm] for _ in range(npts)]
+            res[k] = blocks
+            arr = _ptrs([b for pt in blocks for b in pt])
+            keep.append(arr)
+            setattr(o, k, C.cast(arr, C.POINTER(C.c_void_p)))
+    sl = sr = None
+    if sigma_r_ph is not None:
+        sls = [_f(b) for pt in sigma_lesser_ph for b in pt]
+        srs = [_f(b) for pt in sigma_r_ph for b in pt]
+        keep += [sls, srs]
+        sl, sr = _ptrs(sls), _ptrs(srs)
+    p = make_params(variant, mul, mur, templ, tempr, coupling_fix, lead_eta, lead_nmax)
+    st = lib().qt_rgf_solve(H.handle, C.byref(p), C.c_int32(npts), En.ctypes.data_as(C.c_void_p),
+                            kidx.ctypes.data_as(C.c_void_p), sl, sr, C.byref(o))
+    if st == QT_ERR_SINGULAR and not raise_singular:
+        res["status"] = st
+    else:
+        check(st)
+        res["status"] = 0
+    return res
+
+
+def _single(variant, Hii, H1i, Sii, En, mul, mur, TEMPl, TEMPr, sigma_lesser_ph, sigma_r_ph,
+            coupling_fix):
+    H = Hamiltonian([Hii], [H1i], None if Sii is None else [Sii])
+    try:
+        sl = None if sigma_lesser_ph is None else [sigma_lesser_ph]
+        sr = None if sigma_r_ph is None else [sigma_r_ph]
+        r = solve_batch(H, [En], None, variant=variant, mul=mul, mur=mur, templ=TEMPl,
+                        tempr=TEMPr, coupling_fix=coupling_fix, sigma_lesser_ph=sl,
+                        sigma_r_ph=sr, diag=False, full=True)
+    finally:
+        H.close()
+    out = {k: r[k][0] for k in ("G_r", "G_lesser", "G_greater", "Jdens", "Gl", "Gln")}
+    out.update(tr=float(r["tr"][0]), tre=float(r["tre"][0]), lead_iters=tuple(r["lead_iters"][0]),
+               info=int(r["info"][0]))
+    return out
+
+
+def rgf_variableblock_forward(Hii, H1i, En, mul, mur, TEMPl=300.0, TEMPr=300.0, Sii=None,
+                              sigma_lesser_ph=None, sigma_r_ph=None, coupling_fix=False) -> dict:
+    """rgf_mod.f90:46-269 on the device (one point, full block lists)."""
+    return _single("forward", Hii, H1i, Sii, En, mul, mur, TEMPl, TEMPr, sigma_lesser_ph,
+                   sigma_r_ph, coupling_fix)
+
+
+def rgf_variableblock_backward(Hii, H1i, En, mul, mur, TEMPl=300.0, TEMPr=300.0, Sii=None,
+                               sigma_lesser_ph=None, sigma_r_ph=None) -> dict:
+    """rgf_mod.f90:272-472 on the device (one point, full block lists)."""
+    return _single("backward", Hii, H1i, Sii, En, mul, mur, TEMPl, TEMPr, sigma_lesser_ph,
+                   sigma_r_ph, False)
+
+
+def sancho(H00s, H10s, En, S00=None, eta: float = 1e-5, nmax: int = 50, tol: float = 1e-100,
+           reset_on_fail: bool = False, want_gbb: bool = True):
+    """Batched Sancho-Rubio (rgf_mod.f90:482-568): lists of H00/H10 per
+    point; returns G00, GBB lists, iterations and convergence flags."""
+    init()
+    En = np.ascontiguousarray(En, dtype=np.float64)
+    npts = len(En)
+    nm = H00s[0].shape[0]
+    H00s = [_f(h) for h in H00s]
+    H10s = [_f(h) for h in H10s]
+    G00 = [np.zeros((nm, nm), np.complex128, order="F") for _ in range(npts)]
+    GBB = [np.zeros((nm, nm), np.complex128, order="F") for _ in range(npts)] if want_gbb else None
+    it = np.zeros(npts, np.int32)
+    cv = np.zeros(npts, np.int32)
+    S = None if S00 is None else _f(S00)
+    check(lib().qt_sancho(C.c_int32(nm), C.c_int32(npts), En.ctypes.data_as(C.c_void_p),
+                          C.c_double(eta), C.c_int32(nmax), C.c_double(tol),
+                          C.c_int32(1 if reset_on_fail else 0),
+                          None if S is None else S.ctypes.data_as(C.c_void_p), _ptrs(H00s),
+                          _ptrs(H10s), _ptrs(G00), None if GBB is None else _ptrs(GBB),
+                          it.ctypes.data_as(C.c_void_p), cv.ctypes.data_as(C.c_void_p)))
+    return G00, GBB, it, cv.astype(bool)
+
+
+def set_kernel_timing(enable: bool) -> None:
+    init()
+    check(lib().qt_set_kernel_timing(C.c_int(1 if enable else 0)))
+
+
+def last_timing() -> dict:
+    t = Timing()
+    check(lib().qt_last_timing(C.byref(t)))
+    return t.as_dict()
+
+
+class DeviceBuffer:
+    """Raw device allocation through the library (outputs kept resident)."""
+
+    def __init__(self, nbytes: int):
+        init()
+        self.nbytes = nbytes
+        p = C.c_void_p()
+        check(lib().qt_device_alloc(C.c_size_t(nbytes), C.byref(p)))
+        self.ptr = p
+
+    def to_host(self, dtype, shape) -> np.ndarray:
+        a = np.empty(shape, dtype=dtype)
+        check(lib().qt_memcpy_d2h(a.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(a.nbytes)))
+        return a
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().qt_device_free(self.ptr)
+            self.ptr = None
+
+
+def solve_batch_device(H: Hamiltonian, En, kidx, bufs: dict, *, variant: str = "forward",
+                       mul: float = 0.0, mur: float = 0.0, templ: float = 300.0,
+                       tempr: float = 300.0, coupling_fix: bool = False) -> None:
+    """Batched solve with outputs left in device buffers (bufs: tr, tre,
+    gr_diag, gl_diag, gg_diag -> DeviceBuffer)."""
+    En = np.ascontiguousarray(En, dtype=np.float64)
+    npts = len(En)
+    kidx = np.zeros(npts, np.int32) if kidx is None else np.ascontiguousarray(kidx, dtype=np.int32)
+    o = Outputs()
+    if "tr" in bufs:
+        o.tr = C.cast(bufs["tr"].ptr, C.POINTER(C.c_double))
+    if "tre" in bufs:
+        o.tre = C.cast(bufs["tre"].ptr, C.POINTER(C.c_double))
+    for k in ("gr_diag", "gl_diag", "gg_diag"):
+        if k in bufs:
+            setattr(o, k, bufs[k].ptr)
+    p = make_params(variant, mul, mur, templ, tempr, coupling_fix, device_outputs=True)
+    st = lib().qt_rgf_solve(H.handle, C.byref(p), C.c_int32(npts), En.ctypes.data_as(C.c_void_p),
+                            kidx.ctypes.data_as(C.c_void_p), None, None, C.byref(o))
+    if st != QT_ERR_SINGULAR:
+        check(st)
+
+
+def zgemm_batched(A: np.ndarray, B: np.ndarray, opa: int = 0, opb: int = 0, alpha: float = 1.0,
+                  beta: float = 0.0, R: Optional[np.ndarray] = None) -> np.ndarray:
+    """R[b] = alpha op(A[b]) op(B[b]) + beta R[b] on the device (MUL_C,
+    matrix_c.f90:615-660).  A, B: (batch, rows, cols) stacks of column-major
+    blocks (each block stored Fortran-ordered)."""
+    init()
+    A = np.asarray(A, dtype=np.complex128)
+    B = np.asarray(B, dtype=np.complex128)
+    batch = A.shape[0]
+    ar, ac = A.shape[1:]
+    br, bc = B.shape[1:]
+    m, k = (ar, ac) if opa == 0 else (ac, ar)
+    n = bc if opb == 0 else br
+    Af = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))  # column-major blocks
+    Bf = np.ascontiguousarray(np.transpose(B, (0, 2, 1)))
+    if R is None:
+        Rf = np.zeros((batch, n, m), np.complex128)
+    else:
+        Rf = np.ascontiguousarray(np.transpose(np.asarray(R, np.complex128), (0, 2, 1)))
+    check(lib().qt_zgemm_batched(C.c_int32(opa), C.c_int32(opb), C.c_int32(m), C.c_int32(n),
+                                 C.c_int32(k), C.c_int32(batch), C.c_double(alpha),
+                                 Af.ctypes.data_as(C.c_void_p), C.c_int64(ar), C.c_int64(ar * ac),
+                                 Bf.ctypes.data_as(C.c_void_p), C.c_int64(br), C.c_int64(br * bc),
+                                 C.c_double(beta), Rf.ctypes.data_as(C.c_void_p), C.c_int64(m),
+                                 C.c_int64(m * n)))
+    return np.transpose(Rf, (0, 2, 1))
+
+
+def invert_batched(A: np.ndarray):
+    """Inverse of a (batch, n, n) stack on the device (invert,
+    linalg.f90:42-62); returns (inverses, info)."""
+    init()
+    A = np.asarray(A, dtype=np.complex128)
+    batch, n = A.shape[0], A.shape[1]
+    Af = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))
+    info = np.zeros(batch, np.int32)
+    st = lib().qt_invert_batched(C.c_int32(n), C.c_int32(batch), Af.ctypes.data_as(C.c_void_p),
+                                 info.ctypes.data_as(C.c_void_p))
+    if st not in (0, QT_ERR_SINGULAR):
+        check(st)
+    return np.transpose(Af, (0, 2, 1)), info

@@ -1,0 +1,65 @@
+"""The Fortran host modules end to end: the qt_negf driver (namelist
+'input' + Wannier 'ham_dat', negf_solve on the device) reproduces the
+reference's test_cnt outputs ldos/ndos/pdos0000.dat record by record."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "quantum_transport_amd", "fortran", "_build", "qt_negf")
+
+INPUT = """&input
+ns=3
+nx=5
+nen=200
+emin=-6.0
+emax=-1.0
+mu(:) = -3.7,-3.7
+temp(:) = 300.0, 300.0
+/
+"""
+
+
+def _records(path):
+    with open(path) as fh:
+        return fh.read().split("\n")
+
+
+@pytest.mark.gpu
+def test_qt_negf_reproduces_test_cnt(tmp_path, golden_dir):
+    if not os.path.exists(DRIVER):
+        pytest.fail("qt_negf not built (make -C quantum_transport_amd/fortran)")
+    shutil.copy(os.path.join(golden_dir, "ham_dat"), tmp_path / "ham_dat")
+    (tmp_path / "input").write_text(INPUT)
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    for name in ("ldos", "ndos", "pdos"):
+        ours = _records(tmp_path / f"{name}0000.dat")
+        ref = _records(os.path.join(golden_dir, f"{name}0000.dat"))
+        assert len(ours) == len(ref), name
+        ref_vals = np.array([[float(t) for t in ln.split()] for ln in ref if ln.strip()])
+        scale = np.abs(ref_vals[:, 3]).max()
+        same = differ = 0
+        for lo, lr in zip(ours, ref):
+            if not lr.strip():
+                assert lo == lr  # empty record after each energy
+                continue
+            vo, vr = lo.split(), lr.split()
+            # x and E columns are written identically
+            assert vo[:2] == vr[:2], (lo, lr)
+            # the printed spectral value matches wherever it is not rounding noise
+            if abs(float(vr[3])) > 1e-6 * scale:
+                if vo[3] == vr[3]:
+                    same += 1
+                else:
+                    differ += 1
+            else:
+                assert abs(float(vo[3])) < 1e-6 * scale
+            # record layout (4E18.4)
+            assert len(lo) == len(lr) == 72
+        print(f"{name}: {same} significant records identical, {differ} differ")
+        assert differ == 0 and same > 1000

@@ -1,0 +1,322 @@
+"""GPU parity of the HIP RGF path against the CPU oracle, through the C ABI.
+
+Tolerances: the plain-C and vendor LAPACK restatements of the oracle already
+differ by up to ~1e-10 (relative to the block scale) on these devices,
+because the Schur-complement inversions amplify rounding; the FP64 device
+path is held to the same floor: 1e-9 elementwise relative to each point's
+largest element, 1e-10 median normwise."""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from quantum_transport_amd import hamiltonian as hm
+
+pytestmark = pytest.mark.gpu
+
+BOLTZ = 8.61734e-05
+
+
+@pytest.fixture(scope="module")
+def qt():
+    from quantum_transport_amd import rgf
+
+    rgf.init(0)
+    return rgf
+
+
+def _rel_err(a, b):
+    """elementwise error relative to each point's largest |b|, and the
+    per-point normwise error."""
+    a = np.atleast_2d(a)
+    b = np.atleast_2d(b)
+    scale = np.abs(b).max(axis=1, keepdims=True)
+    scale[scale == 0] = 1.0
+    ew = (np.abs(a - b) / scale).max()
+    nw = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+    return ew, nw
+
+
+def _check_diags(gpu, ref, what, ew_tol=1e-9, nw_tol=1e-10):
+    for key in ("gr_diag", "gl_diag", "gg_diag"):
+        if key == "gl_diag" or key == "gg_diag":
+            # G^< (G^>) cancels to rounding noise where f ~ 0 (1): measure them
+            # on the scale of the spectral function G - G^H
+            s = np.maximum(np.abs(ref["gr_diag"]).max(axis=1), np.abs(ref[key]).max(axis=1))[:, None]
+            d = np.abs(gpu[key] - ref[key]) / s
+            ew = d.max()
+            nw = np.sqrt((d ** 2).sum(axis=1) / max(d.shape[1], 1))
+        else:
+            ew, nw = _rel_err(gpu[key], ref[key])
+        print(f"{what} {key}: max elementwise {ew:.2e}, median normwise {np.median(nw):.2e}")
+        assert ew < ew_tol, (what, key, ew)
+        assert np.median(nw) < nw_tol, (what, key, np.median(nw))
+
+
+def _check_currents(gpu, ref, what, tol=1e-8):
+    for key in ("tr", "tre"):
+        scale = max(np.abs(ref[key]).max(), 1e-300)
+        err = np.abs(gpu[key] - ref[key]).max() / scale
+        print(f"{what} {key}: max err / max |{key}| = {err:.2e}")
+        assert err < tol, (what, key, err)
+
+
+# ----------------------------------------------------------------- Sancho
+
+
+def test_sancho_golden_cnt(qt, oracle_mod, flavour, golden_dir):
+    """test_cnt/sancho_g00.dat, sancho_gbb.dat (rgf_mod.f90:82-89,163-170)."""
+    w = hm.read_ham_dat(f"{golden_dir}/ham_dat")
+    H00, H10 = w.mat_def(0, 0, 0, 3)
+    g00 = np.loadtxt(f"{golden_dir}/sancho_g00.dat")
+    gbb = np.loadtxt(f"{golden_dir}/sancho_gbb.dat")
+    Es = g00[:, 0]
+    H10s = [H10.conj().T if int(t) == 2 else H10 for t in g00[:, 1]]
+    G, GB, it, cv = qt.sancho([H00] * len(Es), H10s, Es)
+    assert cv.all()
+    e0 = np.array([abs(-np.trace(g).imag - v) / abs(v) for g, v in zip(G, g00[:, 2])])
+    eb = np.array([abs(-np.trace(g).imag - v) / abs(v) for g, v in zip(GB, gbb[:, 2])])
+    errs = np.concatenate([e0, eb])
+    print(f"sancho golden: max {errs.max():.2e} median {np.median(errs):.2e}, iters {np.unique(it)}")
+    assert errs.max() < 1e-8 and np.median(errs) < 1e-12
+    # iteration counts equal the oracle's (stop test sum |H01| < 1e-100)
+    for i in range(0, len(Es), 37):
+        _, _, ito, _ = oracle_mod.sancho(None, H00, H10s[i], Es[i], flavour=flavour)
+        assert ito == it[i]
+
+
+@pytest.mark.parametrize("E", [-1.3, 0.0, 0.7, 2.5])
+def test_sancho_chain_closed_form(qt, E):
+    from tests import _dense
+
+    eps, t = 0.2, -1.0
+    G, _, it, cv = qt.sancho([np.array([[eps]])], [np.array([[t]])], [E])
+    g = _dense.chain_surface_g(E + 1e-5j, eps, t)
+    assert cv[0] and abs(G[0][0, 0] - g) <= 1e-7 * abs(g)
+
+
+# ----------------------------------------------------------------- forward RGF, golden
+
+
+def test_forward_cnt_golden(qt, golden_dir):
+    """test_cnt/{ldos,ndos,pdos}0000.dat: all 200 energies, nx=5, ns=3."""
+    w = hm.read_ham_dat(f"{golden_dir}/ham_dat")
+    dev = hm.device_from_wannier(w, 5, 3, [(0.0, 0.0)])
+    E = hm.energy_grid(-6.0, -1.0, 200)
+    H = qt.Hamiltonian.from_device(dev)
+    r = qt.solve_batch(H, E, None, mul=-3.7, mur=-3.7)
+    H.close()
+    for name, key, coef in (("ldos", "gr_diag", -2.0), ("ndos", "gl_diag", 1.0), ("pdos", "gg_diag", -1.0)):
+        ref = np.loadtxt(f"{golden_dir}/{name}0000.dat").reshape(200, 15, 4)
+        ours = r[key].reshape(200, 5, 3, 32).sum(-1).reshape(200, 15).imag * coef
+        scale = np.abs(ref[:, :, 3]).max()
+        big = np.abs(ref[:, :, 3]) > 1e-6 * scale
+        sig = np.array([float("%.3E" % v) for v in ours[big]])
+        mism = np.sum(sig != ref[:, :, 3][big])
+        print(f"{name}: {big.sum()} significant values, {mism} differ at 4 digits")
+        assert mism == 0
+        assert np.all(np.abs(ours[~big]) < 1e-6 * scale)
+
+
+# ----------------------------------------------------------------- parity vs oracle
+
+
+def test_forward_parity_small_configs(qt, oracle_mod, flavour):
+    """nb=64 x 32 identical slabs, 256 energies in [-3,3] (eta=1e-5 leads,
+    z = E in the device): diagonals of G^R, G^<, G^> and tr/tre."""
+    _, dev = hm.synthetic_device(64, 32, seed=1001)
+    E = hm.energy_grid(-3.0, 3.0, 256)
+    if flavour != "mkl":
+        E = E[::16]
+    t0 = time.time()
+    ref = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, flavour=flavour)
+    t1 = time.time()
+    H = qt.Hamiltonian.from_device(dev)
+    gpu = qt.solve_batch(H, E, None, mul=0.0, mur=-0.1)
+    t2 = time.time()
+    H.close()
+    print(f"oracle {t1 - t0:.1f}s, gpu {t2 - t1:.2f}s")
+    assert (gpu["info"] == 0).all()
+    assert np.array_equal(gpu["lead_iters"], ref["sancho_iters"])
+    _check_diags(gpu, ref, "C1")
+    _check_currents(gpu, ref, "C1")
+
+
+def test_forward_parity_kz_batch(qt, oracle_mod, flavour):
+    """several kz sets in one batch: H_ii(kz) = H_ii + T_z e^{ikz} + h.c."""
+    kz = [0.0, np.pi / 4, np.pi / 2, 3 * np.pi / 4]
+    _, dev = hm.synthetic_device(48, 10, seed=1002, kz=kz, tz_density=0.05)
+    E = np.repeat(hm.energy_grid(-2.0, 2.0, 12), 4)
+    k = np.tile(np.arange(4), 12)
+    ref = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour=flavour)
+    H = qt.Hamiltonian.from_device(dev)
+    gpu = qt.solve_batch(H, E, k, mul=0.0, mur=-0.1)
+    H.close()
+    _check_diags(gpu, ref, "kz")
+    _check_currents(gpu, ref, "kz")
+
+
+def test_backward_parity(qt, oracle_mod, flavour):
+    _, dev = hm.synthetic_device(32, 8, seed=7)
+    E = hm.energy_grid(-2.5, 2.5, 24)
+    ref = oracle_mod.rgf_batch(dev, E, None, 0.05, -0.15, variant="backward", flavour=flavour)
+    H = qt.Hamiltonian.from_device(dev)
+    gpu = qt.solve_batch(H, E, None, variant="backward", mul=0.05, mur=-0.15)
+    H.close()
+    _check_diags(gpu, ref, "backward")
+    _check_currents(gpu, ref, "backward")
+
+
+@pytest.mark.parametrize("variant", ["forward", "backward"])
+def test_full_block_lists(qt, oracle_mod, variant):
+    """Every block the reference interface returns: G_r, G_lesser,
+    G_greater, Jdens, Gl, Gln (rgf_mod.f90:53), for one energy point."""
+    _, dev = hm.synthetic_device(12, 5, seed=3, coupling_density=0.3)
+    Hii, H1i = dev.Hii[0], dev.H1i[0]
+    for E in (-0.12, 0.8):
+        ref = oracle_mod.rgf(Hii, H1i, E, 0.0, -0.2, 300.0, 300.0, variant=variant)
+        fn = qt.rgf_variableblock_forward if variant == "forward" else qt.rgf_variableblock_backward
+        got = fn(Hii, H1i, E, 0.0, -0.2, 300.0, 300.0)
+        for key in ("G_r", "G_lesser", "G_greater", "Jdens", "Gl", "Gln"):
+            for s in range(5):
+                if key == "Gln" and s == (4 if variant == "forward" else 0):
+                    continue  # not set by the reference for the last slab of pass 1
+                a, b = got[key][s], ref[key][s]
+                scale = max(np.abs(b).max(), np.abs(ref["G_r"][s]).max())
+                assert np.abs(a - b).max() <= 1e-10 * scale, (variant, key, s, E)
+        assert abs(got["tr"] - ref["tr"]) <= 1e-9 * max(abs(ref["tr"]), 1e-12)
+        assert abs(got["tre"] - ref["tre"]) <= 1e-9 * max(abs(ref["tre"]), 1e-12)
+
+
+def test_uneven_slabs_coupling_fix(qt, oracle_mod):
+    sizes = [24, 40, 16, 32, 24]
+    dev = hm.uneven_device(sizes, seed=5, coupling_density=0.2)
+    E = hm.energy_grid(-2.0, 2.0, 8)
+    H = qt.Hamiltonian.from_device(dev)
+    gpu = qt.solve_batch(H, E, None, mul=0.0, mur=-0.1, coupling_fix=True)
+    H.close()
+    ref = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, coupling_fix=True)
+    _check_diags(gpu, ref, "uneven")
+    _check_currents(gpu, ref, "uneven")
+
+
+def test_uneven_slabs_need_fix(qt):
+    dev = hm.uneven_device([16, 24, 16], seed=5)
+    H = qt.Hamiltonian.from_device(dev)
+    with pytest.raises(qt.QTError):
+        qt.solve_batch(H, [0.1], None)
+    H.close()
+
+
+def test_sigma_ph_and_overlap(qt, oracle_mod):
+    """Non-zero sigma_r_ph / sigma_lesser_ph per point and a non-identity
+    overlap S (rgf_mod.f90:77-79, 99-102, 120-133)."""
+    rng = np.random.default_rng(17)
+    n, nx = 10, 4
+    _, dev = hm.synthetic_device(n, nx, seed=9, coupling_density=0.3)
+    Hii, H1i = dev.Hii[0], dev.H1i[0]
+    X = 0.05 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    S = np.eye(n) + X @ X.conj().T
+    Sii = [S] * nx
+    E = [-0.3, 0.45]
+    sr, sl = [], []
+    for _ in E:
+        a = [0.02 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) for _ in range(nx)]
+        sr.append([x - 0.05j * np.eye(n) for x in a])
+        b = [0.03 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) for _ in range(nx)]
+        sl.append([1j * (y @ y.conj().T) for y in b])
+    H = qt.Hamiltonian(dev.Hii, dev.H1i, [Sii])
+    gpu = qt.solve_batch(H, E, None, mul=0.1, mur=-0.1, sigma_lesser_ph=sl, sigma_r_ph=sr, full=True)
+    H.close()
+    for ip, e in enumerate(E):
+        ref = oracle_mod.rgf(Hii, H1i, e, 0.1, -0.1, 300.0, 300.0, Sii=Sii, sigma_lesser_ph=sl[ip],
+                             sigma_r_ph=sr[ip])
+        for key in ("G_r", "G_lesser", "G_greater", "Jdens"):
+            for s in range(nx):
+                a, b = gpu[key][ip][s], ref[key][s]
+                scale = max(np.abs(b).max(), np.abs(ref["G_r"][s]).max())
+                assert np.abs(a - b).max() <= 1e-10 * scale, (key, s, ip)
+
+
+def test_singular_block_is_zeroed(qt, oracle_mod):
+    """An isolated orbital at exactly the energy E makes z - H00 - Sigma
+    singular in the device (z = E + 0i, rgf_mod.f90:62): the block is zeroed
+    as linalg.f90:50-58 does, the point is flagged and the rest of the sweep
+    follows the oracle."""
+    _, dev = hm.synthetic_device(8, 4, seed=13)
+    E = 0.375
+    Hii = [h.copy() for h in dev.Hii[0]]
+    H1i = [h.copy() for h in dev.H1i[0]]
+    for s in range(4):
+        Hii[s][3, :] = 0
+        Hii[s][:, 3] = 0
+        Hii[s][3, 3] = E
+    for s in range(5):
+        H1i[s][3, :] = 0
+        H1i[s][:, 3] = 0
+    ref = oracle_mod.rgf(Hii, H1i, E, 0.0, 0.0)
+    assert ref["info"] != 0
+    H = qt.Hamiltonian([Hii], [H1i])
+    r = qt.solve_batch(H, [E, E + 0.01], None, full=True)
+    H.close()
+    assert r["status"] == qt.QT_ERR_SINGULAR
+    assert r["info"][0] & qt.QT_INFO_SINGULAR and not (r["info"][1] & qt.QT_INFO_SINGULAR)
+    assert np.all(r["Gl"][0][0] == 0)
+    for s in range(4):
+        b = ref["G_r"][s]
+        assert np.abs(r["G_r"][0][s] - b).max() <= 1e-10 * max(np.abs(b).max(), 1e-300)
+
+
+# ----------------------------------------------------------------- production shape
+
+
+def test_production_block_size_parity(qt, oracle_mod, flavour):
+    """nb=512 blocks (the production slab size) on a short device, 2 kz:
+    full oracle comparison."""
+    if flavour != "mkl":
+        pytest.skip("needs the vendor-LAPACK oracle for n=512")
+    kz = [0.0, np.pi / 2]
+    _, dev = hm.synthetic_device(512, 4, seed=1002, kz=kz, tz_density=0.02)
+    E = np.array([-1.5, -0.05, 0.6, 1.9])
+    k = np.array([0, 1, 1, 0])
+    t0 = time.time()
+    ref = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour=flavour)
+    print(f"oracle n=512: {time.time() - t0:.1f}s")
+    H = qt.Hamiltonian.from_device(dev)
+    gpu = qt.solve_batch(H, E, k, mul=0.0, mur=-0.1)
+    H.close()
+    _check_diags(gpu, ref, "n512")
+    _check_currents(gpu, ref, "n512")
+
+
+def test_production_shape_properties(qt):
+    """nb=512 x 100 slabs, 4 kz (the production shape): properties that hold
+    at any size -- occupations within the contacts' Fermi window, positive
+    spectral weight, current conservation tr = -tre, and bit-identical
+    results whatever the batch composition."""
+    kz = [0.0, np.pi / 4, np.pi / 2, 3 * np.pi / 4]
+    _, dev = hm.synthetic_device(512, 100, seed=1002, kz=kz, tz_density=0.02)
+    H = qt.Hamiltonian.from_device(dev)
+    E = np.array([-0.2, -0.05, 0.0, 0.3, -1.1, 1.7])
+    k = np.array([0, 1, 2, 3, 1, 2])
+    mul, mur, T = 0.0, -0.1, 300.0
+    r = qt.solve_batch(H, E, k, mul=mul, mur=mur)
+    r2 = qt.solve_batch(H, E[:3], k[:3], mul=mul, mur=mur)
+    H.close()
+    for key in ("gr_diag", "gl_diag", "gg_diag", "tr", "tre"):
+        assert np.array_equal(r[key][:3], r2[key]), key
+    A = -2 * r["gr_diag"].imag
+    assert (A > -1e-9 * A.max()).all()
+    nocc = r["gl_diag"].imag
+    nemp = -r["gg_diag"].imag
+    # forward variant: left contact occupied with f(E-mur), right with f(E-mul)
+    fa = 1 / (1 + np.exp((E - mur) / (BOLTZ * T)))
+    fb = 1 / (1 + np.exp((E - mul) / (BOLTZ * T)))
+    lo, hi = np.minimum(fa, fb)[:, None], np.maximum(fa, fb)[:, None]
+    occ = nocc / np.maximum(nocc + nemp, 1e-300)
+    big = (nocc + nemp) > 1e-8 * (nocc + nemp).max()
+    assert (occ[big] >= lo.repeat(occ.shape[1], 1)[big] - 1e-8).all()
+    assert (occ[big] <= hi.repeat(occ.shape[1], 1)[big] + 1e-8).all()
+    scale = np.abs(r["tr"]).max()
+    assert np.abs(r["tr"] + r["tre"]).max() <= 1e-8 * max(scale, 1e-300)
