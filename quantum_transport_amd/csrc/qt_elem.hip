@@ -182,35 +182,59 @@ __global__ void diag_kernel(DiagArgs a) {
 }
 
 // ---------------------------------------------------------------- Sancho step
-__global__ __launch_bounds__(256) void sancho_update_kernel(SanchoUpdArgs a) {
-    const int b = blockIdx.y;
-    if (!a.active[b]) return;
-    const int64_t nn = (int64_t)a.n * a.n;
-    const int64_t base = b * a.s;
-    const int kz = a.kidx ? a.kidx[b] : 0;
-    const double2* H00 = a.H00 ? a.H00 + b * a.sh00 + kz * a.skh00 : nullptr;
-    __shared__ double red[256];
-    double acc = 0.0;
-    for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < nn; e += (int64_t)gridDim.x * 256) {
-        const double2 c = a.Cm[base + e];
-        const double2 ap = a.Ap[base + e];
-        double2 ss = a.HSS[base + e], bb = a.HBB[base + e];
-        ss.x += c.x; ss.y += c.y;
-        bb.x += c.x; bb.y += c.y;
-        bb.x += ap.x; bb.y += ap.y;
-        a.HSS[base + e] = ss;
-        a.HBB[base + e] = bb;
-        const double2 h = a.H01n[base + e];
-        acc += sqrt(h.y * h.y + h.x * h.x);
-    }
-    red[threadIdx.x] = acc;
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    red[threadIdx.x] = v;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) a.partial[(int64_t)b * gridDim.x + blockIdx.x] = red[0];
-    (void)H00;
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void sancho_update_kernel(SanchoUpdArgs a) {
+    const int b = blockIdx.y;
+    if (!a.active[b]) return;
+    const bool on1 = a.active1 ? a.active1[b] != 0 : true;
+    const bool on2 = a.HSS2 && a.active2[b] != 0;
+    const int64_t nn = (int64_t)a.n * a.n;
+    const int64_t base = b * a.s;
+    __shared__ double red[256];
+    double acc1 = 0.0, acc2 = 0.0;
+    for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < nn; e += (int64_t)gridDim.x * 256) {
+        const double2 c = a.Cm[base + e];
+        const double2 ap = a.Ap[base + e];
+        double2 bb = a.HBB[base + e];
+        bb.x += c.x + ap.x;
+        bb.y += c.y + ap.y;
+        a.HBB[base + e] = bb;
+        if (on1) {
+            double2 ss = a.HSS[base + e];
+            ss.x += c.x;
+            ss.y += c.y;
+            a.HSS[base + e] = ss;
+        }
+        if (on2) {
+            double2 s2 = a.HSS2[base + e];
+            s2.x += ap.x;
+            s2.y += ap.y;
+            a.HSS2[base + e] = s2;
+        }
+        const double2 h = a.H01n[base + e];
+        acc1 += sqrt(h.y * h.y + h.x * h.x);
+        if (a.HSS2) {
+            const double2 g = a.H10n[base + e];
+            acc2 += sqrt(g.y * g.y + g.x * g.x);
+        }
+    }
+    const double s1 = block_sum(acc1, red);
+    const double s2 = a.HSS2 ? block_sum(acc2, red) : 0.0;
+    if (threadIdx.x == 0) {
+        a.partial[((int64_t)b * 2 + 0) * gridDim.x + blockIdx.x] = s1;
+        a.partial[((int64_t)b * 2 + 1) * gridDim.x + blockIdx.x] = s2;
+    }
 }
 
 // convergence decision after the partial sums: rgf_mod.f90:535-551
@@ -218,30 +242,45 @@ __global__ void sancho_decide_kernel(SanchoUpdArgs a, int ncol) {
     const int b = blockIdx.x;
     if (!a.active[b]) return;
     __shared__ double red[256];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < ncol; i += 256) acc += a.partial[(int64_t)b * ncol + i];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+    double acc1 = 0.0, acc2 = 0.0;
+    for (int i = threadIdx.x; i < ncol; i += 256) {
+        acc1 += a.partial[((int64_t)b * 2 + 0) * ncol + i];
+        acc2 += a.partial[((int64_t)b * 2 + 1) * ncol + i];
     }
-    const double err = red[0];
-    const bool conv = fabs(err) < a.tol;
+    const double err1 = block_sum(acc1, red);
+    const double err2 = block_sum(acc2, red);
+    const bool on1 = a.active1 ? a.active1[b] != 0 : true;
+    const bool on2 = a.HSS2 && a.active2[b] != 0;
+    const bool conv1 = fabs(err1) < a.tol;
+    const bool conv2 = fabs(err2) < a.tol;
     // reset on the last iteration (gw_mod.f90:1440-1445)
-    if (!conv && a.reset) {
+    if (a.reset && ((on1 && !conv1) || (on2 && !conv2))) {
         const int64_t nn = (int64_t)a.n * a.n;
         const int kz = a.kidx ? a.kidx[b] : 0;
         const double2* H00 = a.H00 + b * a.sh00 + kz * a.skh00;
         for (int64_t e = threadIdx.x; e < nn; e += 256) {
-            a.HSS[b * a.s + e] = H00[e];
+            if (on1 && !conv1) a.HSS[b * a.s + e] = H00[e];
+            if (on2 && !conv2) a.HSS2[b * a.s + e] = H00[e];
             a.HBB[b * a.s + e] = H00[e];
         }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        a.iters[b] = a.iter;
-        if (conv) a.active[b] = 0;
+        bool still = false;
+        if (on1) {
+            a.iters[b] = a.iter;
+            if (conv1) {
+                if (a.active1) a.active1[b] = 0;
+            } else {
+                still = true;
+            }
+        }
+        if (on2) {
+            a.iters2[b] = a.iter;
+            if (conv2) a.active2[b] = 0;
+            else still = true;
+        }
+        a.active[b] = still ? 1 : 0;
     }
 }
 
@@ -336,7 +375,7 @@ void diag_extract(hipStream_t s, const DiagArgs& a) {
 void sancho_update(hipStream_t s, const SanchoUpdArgs& a) {
     if (a.batch <= 0) return;
     const int64_t nn = (int64_t)a.n * a.n;
-    const int nblk = (int)std::min<int64_t>((nn + 255) / 256, 64);
+    const int nblk = (int)std::min<int64_t>((nn + 255) / 256, 32);
     hipLaunchKernelGGL(sancho_update_kernel, dim3(nblk, a.batch), dim3(256), 0, s, a);
     QT_HIP(hipGetLastError());
     hipLaunchKernelGGL(sancho_decide_kernel, dim3(a.batch), dim3(256), 0, s, a, nblk);

@@ -71,8 +71,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int til
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int gk = k0 + kk + q;
-                double2 v = (gi < M && gk < K) ? Ap[gk + gi * lda] : make_double2(0.0, 0.0);
-                ra[q] = OPA == OP_C ? cconj(v) : v;
+                ra[q] = (gi < M && gk < K) ? Ap[gk + gi * lda] : make_double2(0.0, 0.0);
             }
         }
     };
@@ -84,7 +83,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int til
         } else {
             const int ii = tid >> 2, kk = (tid & 3) * 4;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) As[buf][kk + q][ii] = ra[q];
+            for (int q = 0; q < 4; ++q) As[buf][kk + q][ii] = OPA == OP_C ? cconj(ra[q]) : ra[q];
         }
     };
     auto loadB = [&](int k0) {
@@ -102,8 +101,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int til
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int gj = n0 + jj + q;
-                double2 v = (gj < N && gk < K) ? Bp[gj + gk * ldb] : make_double2(0.0, 0.0);
-                rb[q] = OPB == OP_C ? cconj(v) : v;
+                rb[q] = (gj < N && gk < K) ? Bp[gj + gk * ldb] : make_double2(0.0, 0.0);
             }
         }
     };
@@ -115,7 +113,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int til
         } else {
             const int kk = tid >> 4, jj = (tid & 15) * 4;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) Bs[buf][kk][jj + q] = rb[q];
+            for (int q = 0; q < 4; ++q) Bs[buf][kk][jj + q] = OPB == OP_C ? cconj(rb[q]) : rb[q];
         }
     };
 

@@ -135,14 +135,20 @@ void diag_extract(hipStream_t s, const DiagArgs& a);
 
 // Sancho-Rubio step update (rgf_mod.f90:526-541):
 //   H_SS += C; H_BB += C + Ap; err_b = sum |H01n|; converged when err < tol
+// With HSS2 set, the same decimation also carries the opposite surface of
+// the lead (the run with H_01 and H_10 exchanged): H_SS2 += Ap, stopping on
+// sum |H10n|.  `active` is the union used to skip converged members.
 struct SanchoUpdArgs {
     int n = 0, batch = 0;
-    double2 *HSS = nullptr, *HBB = nullptr;
-    const double2 *Cm = nullptr, *Ap = nullptr, *H01n = nullptr;
-    int64_t s = 0;                 // common batch stride of all five
-    double* partial = nullptr;     // [batch][blocks]
-    int* active = nullptr;         // cleared on convergence
-    int* iters = nullptr;          // iteration count per member
+    double2 *HSS = nullptr, *HBB = nullptr, *HSS2 = nullptr;
+    const double2 *Cm = nullptr, *Ap = nullptr, *H01n = nullptr, *H10n = nullptr;
+    int64_t s = 0;                 // common batch stride of all six
+    double* partial = nullptr;     // [batch][2][blocks]
+    int* active = nullptr;         // union of the two flags below
+    int* active1 = nullptr;        // first surface still iterating
+    int* active2 = nullptr;        // second surface still iterating (HSS2)
+    int* iters = nullptr;          // iteration count per member (first surface)
+    int* iters2 = nullptr;         // second surface
     int iter = 0;
     double tol = 0.0;
     int reset = 0;                 // last iteration with reset_on_fail

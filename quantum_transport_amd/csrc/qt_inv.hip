@@ -1,22 +1,29 @@
 // qt_inv.hip -- batched complex128 matrix inversion, blocked Gauss-Jordan
-// with partial (row) pivoting, for the Schur complements of the RGF sweeps
-// and the lead decimation (the role of ZGETRF+ZGETRI in linalg.f90:42-62).
+// with partial pivoting, for the Schur complements of the RGF sweeps and the
+// lead decimation (the role of ZGETRF+ZGETRI in linalg.f90:42-62).
+//
+// Rows are never moved.  Pivoting is kept as the list pivp[j] = physical
+// row chosen at step j ("lazy" pivoting); all row operations of
+// Gauss-Jordan act on physical rows, so the working array ends up holding
+// inv(Q A) with row j stored at physical row pivp[j], where Q moves row
+// pivp[j] of A to position j.  The output gather undoes both permutations:
+//     inv(A)[r][c] = W[pivp[r]][pinv[c]],   pinv = inverse of pivp.
 //
 // For each panel of NB columns [kb, kb+NB):
-//   1. panel kernel (one workgroup per matrix, panel resident in LDS):
-//      unblocked Gauss-Jordan on the n x NB panel; pivot = first row of
-//      maximal |re|+|im| among the rows not yet pivoted (the IZAMAX choice
-//      of ZGETRF); rows are swapped inside the panel and recorded in piv.
-//   2. swap kernel: applies the panel's row swaps to every other column,
-//      copies the pivot block row A[K, O] into R (NB x n, zero in the panel
-//      columns) and zeroes A[K, O].
-//   3. update: A += P * R on MFMA (P = the new panel columns), panel columns
-//      masked from the write.  This is the blocked form of the GJ step
+//   1. panel kernel (one workgroup per matrix, one row per thread, the
+//      panel in registers): unblocked Gauss-Jordan on the n x NB panel; the
+//      pivot of column j is the first-indexed row of maximal |re|+|im|
+//      (the IZAMAX choice of ZGETRF) among the rows not yet pivoted; the
+//      pivot row is broadcast through LDS, two barriers per column.
+//   2. update kernel (one workgroup per 64-column tile outside the panel):
+//      copies the NB pivot rows of its columns to LDS (R), then for every
+//      64-row block adds P R on the FP64 matrix cores, with P the new panel
+//      columns and the pivot rows' old values dropped:
 //        A_KO <- inv(A_KK) A_KO,  A_OO <- A_OO - A_OK inv(A_KK) A_KO.
-// After the last panel the working matrix holds inv(P A); the inverse is
-// gathered with the column interchanges applied in reverse order.  A zero
-// pivot sets info and the output block is zeroed (linalg.f90:50-58).
+// A zero pivot sets info (first column, 1-based) and the output block is
+// zeroed (linalg.f90:50-58).
 #include <algorithm>
+#include <climits>
 
 #include "qt_internal.h"
 
@@ -24,12 +31,14 @@ namespace qt {
 
 namespace {
 
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ double cabs1(double2 v) { return fabs(v.x) + fabs(v.y); }
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ double2 crecip(double2 a) {
-    // Smith's algorithm for 1/(a.x + i a.y)
+    // Smith's algorithm for 1 / (a.x + i a.y)
     if (fabs(a.x) >= fabs(a.y)) {
         const double r = a.y / a.x, d = a.x + a.y * r;
         return make_double2(1.0 / d, -r / d);
@@ -39,37 +48,44 @@ __device__ __forceinline__ double2 crecip(double2 a) {
     }
 }
 
-template <int NB>
-__global__ __launch_bounds__(256) void gj_panel_kernel(double2* A, int64_t sa, int n, int kb,
-                                                       int* piv, int* info, const int* active) {
+// ------------------------------------------------------------ panel
+template <int NB, int NT>
+__global__ __launch_bounds__(NT) void gj_panel_kernel(double2* A, int64_t sa, int n, int kb,
+                                                      int* pivp, int* info, const int* active) {
+    constexpr int NW = NT / 64;
     const int b = blockIdx.x;
     if (active && !active[b]) return;
-    extern __shared__ double2 P[];  // [NB][n], column c at P + c*n
-    __shared__ double red_v[4];
-    __shared__ int red_i[4];
-    __shared__ int s_p;
-    __shared__ double2 s_d;
-
+    __shared__ unsigned used[32];          // rows already pivoted (n <= 1024)
+    __shared__ double red_v[2][NW];
+    __shared__ int red_i[2][NW];
+    __shared__ double2 prow[2][NB];
     double2* Ab = A + b * sa;
-    int* pv = piv + (int64_t)b * n;
+    int* pv = pivp + (int64_t)b * n;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ncols = min(NB, n - kb);
+    const int r = tid;                     // this thread's physical row (n <= NT)
 
-    for (int idx = tid; idx < n * ncols; idx += 256) {
-        const int c = idx / n, r = idx - c * n;
-        P[idx] = Ab[r + (int64_t)(kb + c) * n];
-    }
+    if (tid < 32) used[tid] = 0u;
     __syncthreads();
+    for (int j = tid; j < kb; j += NT) atomicOr(&used[pv[j] >> 5], 1u << (pv[j] & 31));
+    __syncthreads();
+    bool mine = r < n && !((used[r >> 5] >> (r & 31)) & 1u);
 
-    for (int c = 0; c < ncols; ++c) {
-        const int j = kb + c;
-        double2* Pc = P + c * n;
-        // --- pivot search over rows r >= j (first maximum wins)
+    double2 v[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        v[c] = (r < n && c < ncols) ? Ab[r + (int64_t)(kb + c) * n] : make_double2(0.0, 0.0);
+
+#pragma clang loop unroll(full)
+    for (int c = 0; c < NB; ++c) {
+        if (c >= ncols) continue;  // uniform over the workgroup
+        const int buf = c & 1;
+        // pivot search among rows not yet pivoted
         double best = -1.0;
-        int bi = n;
-        for (int r = j + tid; r < n; r += 256) {
-            const double v = cabs1(Pc[r]);
-            if (v > best) { best = v; bi = r; }
+        int bi = INT_MAX;
+        if (mine) {
+            best = cabs1(v[c]);
+            bi = r;
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -77,131 +93,185 @@ __global__ __launch_bounds__(256) void gj_panel_kernel(double2* A, int64_t sa, i
             const int oi = __shfl_xor(bi, off, 64);
             if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
         }
-        if (lane == 0) { red_v[wave] = best; red_i[wave] = bi; }
+        if (lane == 0) { red_v[buf][wave] = best; red_i[buf][wave] = bi; }
         __syncthreads();
+        double bv = red_v[buf][0];
+        int p = red_i[buf][0];
+#pragma unroll
+        for (int w = 1; w < NW; ++w)
+            if (red_v[buf][w] > bv || (red_v[buf][w] == bv && red_i[buf][w] < p)) {
+                bv = red_v[buf][w];
+                p = red_i[buf][w];
+            }
+        const int j = kb + c;
+        if (p < 0 || p >= n) p = 0;  // unreachable: n - j rows are always unpivoted
+        if (r == p) {
+#pragma unroll
+            for (int cc = 0; cc < NB; ++cc) prow[buf][cc] = v[cc];
+            mine = false;
+        }
         if (tid == 0) {
-            double bv = red_v[0];
-            int bidx = red_i[0];
-            for (int w = 1; w < 4; ++w)
-                if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bidx)) { bv = red_v[w]; bidx = red_i[w]; }
-            if (bidx >= n) bidx = j;
-            s_p = bidx;
-            pv[j] = bidx;
+            pv[j] = p;
             if (!(bv > 0.0) && info[b] == 0) info[b] = j + 1;
         }
         __syncthreads();
-        const int p = s_p;
-        if (p != j && tid < ncols) {
-            double2 t = P[tid * n + j];
-            P[tid * n + j] = P[tid * n + p];
-            P[tid * n + p] = t;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            const double2 pivv = Pc[j];
-            s_d = (pivv.x == 0.0 && pivv.y == 0.0) ? make_double2(0.0, 0.0) : crecip(pivv);
-        }
-        __syncthreads();
-        const double2 d = s_d;
-        // scale the pivot row
-        if (tid < ncols) {
-            if (tid == c) P[c * n + j] = d;
-            else P[tid * n + j] = cmul(P[tid * n + j], d);
-        }
-        __syncthreads();
-        // eliminate column c from every other row
-        for (int r = tid; r < n; r += 256) {
-            if (r == j) continue;
-            const double2 f = Pc[r];
+        const double2 pivv = prow[buf][c];
+        const double2 d = (pivv.x == 0.0 && pivv.y == 0.0) ? make_double2(0.0, 0.0) : crecip(pivv);
+        if (r == p) {
+#pragma unroll
+            for (int cc = 0; cc < NB; ++cc) v[cc] = (cc == c) ? d : cmul(v[cc], d);
+        } else if (r < n) {
+            const double2 g = cmul(v[c], d);  // f / pivot
 #pragma unroll
             for (int cc = 0; cc < NB; ++cc) {
-                if (cc >= ncols) break;
                 if (cc == c) {
-                    const double2 m = cmul(f, d);
-                    Pc[r] = make_double2(-m.x, -m.y);
+                    v[cc] = make_double2(-g.x, -g.y);
                 } else {
-                    const double2 u = P[cc * n + j];
-                    double2 v = P[cc * n + r];
-                    v.x -= f.x * u.x - f.y * u.y;
-                    v.y -= f.x * u.y + f.y * u.x;
-                    P[cc * n + r] = v;
+                    const double2 u = prow[buf][cc];
+                    v[cc].x -= g.x * u.x - g.y * u.y;
+                    v[cc].y -= g.x * u.y + g.y * u.x;
                 }
             }
         }
-        __syncthreads();
     }
-    for (int idx = tid; idx < n * ncols; idx += 256) {
-        const int c = idx / n, r = idx - c * n;
-        Ab[r + (int64_t)(kb + c) * n] = P[idx];
+    if (r < n) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+            if (c < ncols) Ab[r + (int64_t)(kb + c) * n] = v[c];
     }
 }
 
-// One thread per (member, column outside the panel): apply the panel's row
-// swaps, copy the pivot block row to R and zero it in A.
-__global__ __launch_bounds__(256) void gj_swap_kernel(double2* A, int64_t sa, int n, int kb, int nb,
-                                                      const int* piv, double2* R, int64_t sr,
-                                                      const int* active) {
+// ------------------------------------------------------------ update
+// Workgroup = (64-column tile, member); 4 waves, each a 32x32 sub-tile of
+// every 64-row block; K = panel width, complex product as four
+// v_mfma_f64_16x16x4 (operand order as in the GEMM kernel).
+template <int NB>
+__global__ __launch_bounds__(256) void gj_update_kernel(double2* A, int64_t sa, int n, int kb, int w,
+                                                        const int* pivp, const int* active) {
     const int b = blockIdx.y;
     if (active && !active[b]) return;
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= n) return;
-    double2* Rb = R + b * sr;
-    if (c >= kb && c < kb + nb) {
-        for (int t = 0; t < nb; ++t) Rb[t + (int64_t)c * nb] = make_double2(0.0, 0.0);
-        return;
+    const int c0 = blockIdx.x * 64;
+    if (c0 >= kb && c0 + 64 <= kb + w) return;  // tile entirely inside the panel
+    __shared__ double2 Rs[NB][64];
+    __shared__ double2 Ps[NB][64];
+    __shared__ unsigned pmask[32];
+    double2* Ab = A + b * sa;
+    const int* pv = pivp + (int64_t)b * n;
+    const int tid = threadIdx.x;
+    if (tid < 32) pmask[tid] = 0u;
+    __syncthreads();
+    if (tid < w) atomicOr(&pmask[pv[kb + tid] >> 5], 1u << (pv[kb + tid] & 31));
+    // R = pivot rows of this tile's columns (old values)
+    for (int e = tid; e < NB * 64; e += 256) {
+        const int t = e >> 6, c = e & 63, col = c0 + c;
+        Rs[t][c] = (t < w && col < n && !(col >= kb && col < kb + w))
+                       ? Ab[pv[kb + t] + (int64_t)col * n]
+                       : make_double2(0.0, 0.0);
     }
-    double2* col = A + b * sa + (int64_t)c * n;
-    const int* pv = piv + (int64_t)b * n;
-    for (int t = 0; t < nb; ++t) {
-        const int j = kb + t, p = pv[j];
-        if (p != j) {
-            const double2 x = col[j];
-            col[j] = col[p];
-            col[p] = x;
+    __syncthreads();
+
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int rb = 0; rb < n; rb += 64) {
+        // P block: rows rb..rb+63 of the panel columns, stored [k][i]
+        for (int e = tid; e < NB * 64; e += 256) {
+            const int k = e >> 6, i = e & 63, row = rb + i;
+            Ps[k][i] = (k < w && row < n) ? Ab[row + (int64_t)(kb + k) * n] : make_double2(0.0, 0.0);
         }
-    }
-    for (int t = 0; t < nb; ++t) {
-        Rb[t + (int64_t)c * nb] = col[kb + t];
-        col[kb + t] = make_double2(0.0, 0.0);
+        __syncthreads();
+        double4_t accR[2][2], accI[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                accR[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+                accI[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+            }
+#pragma unroll
+        for (int kq = 0; kq < NB / 4; ++kq) {
+            const int k = kq * 4 + lk;
+            double2 av[2], bv[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) av[i] = Ps[k][wm * 32 + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bv[j] = Rs[k][wn * 32 + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
+                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[j].y, av[i].y, accR[i][j], 0, 0, 0);
+                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].x, accI[i][j], 0, 0, 0);
+                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].y, accI[i][j], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int row = rb + wm * 32 + i * 16 + li;
+            if (row >= n) continue;
+            const bool prow_ = (pmask[row >> 5] >> (row & 31)) & 1u;
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int col = c0 + wn * 32 + j * 16 + 4 * q + lk;
+                    if (col >= n || (col >= kb && col < kb + w)) continue;
+                    double2* dst = Ab + row + (int64_t)col * n;
+                    double2 base = prow_ ? make_double2(0.0, 0.0) : *dst;
+                    *dst = make_double2(base.x + accR[i][j][q], base.y + accI[i][j][q]);
+                }
+        }
+        __syncthreads();
     }
 }
 
-// Output gather: X[:, c] = W[:, perm[c]] with the column interchanges
-// applied in reverse order; zero block when the factorisation failed.
+// ------------------------------------------------------------ output gather
+// X[r][c] = W[pivp[r]][pinv[c]]; zero block when the factorisation failed.
 __global__ __launch_bounds__(256) void gj_finish_kernel(const double2* W, int64_t sw, double2* X,
-                                                        int64_t sx, int n, const int* piv,
+                                                        int64_t sx, int n, const int* pivp,
                                                         const int* info, const int* active) {
     const int b = blockIdx.y;
     if (active && !active[b]) return;
-    extern __shared__ int perm[];
-    const int* pv = piv + (int64_t)b * n;
-    if (threadIdx.x == 0) {
-        for (int c = 0; c < n; ++c) perm[c] = c;
-        for (int j = n - 1; j >= 0; --j) {
-            const int p = pv[j];
-            const int t = perm[j];
-            perm[j] = perm[p];
-            perm[p] = t;
-        }
+    __shared__ int rows[1024];
+    __shared__ int pinv[1024];
+    const int* pv = pivp + (int64_t)b * n;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const int p = pv[i];
+        rows[i] = p;
+        pinv[p] = i;
     }
     __syncthreads();
     const bool bad = info[b] != 0;
     const double2* Wb = W + b * sw;
     double2* Xb = X + b * sx;
-    // columns of this block: blockIdx.x covers 8 columns
     for (int cc = 0; cc < 8; ++cc) {
         const int c = blockIdx.x * 8 + cc;
         if (c >= n) break;
-        const double2* src = Wb + (int64_t)perm[c] * n;
+        const double2* src = Wb + (int64_t)pinv[c] * n;
         double2* dst = Xb + (int64_t)c * n;
-        for (int r = threadIdx.x; r < n; r += 256)
-            dst[r] = bad ? make_double2(0.0, 0.0) : src[r];
+        for (int r = threadIdx.x; r < n; r += 256) dst[r] = bad ? make_double2(0.0, 0.0) : src[rows[r]];
     }
 }
 
 __global__ void zero_int_kernel(int* p, int n) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+template <int NB, int NT>
+void run_panels(hipStream_t s, const InvArgs& a) {
+    const int n = a.n;
+    for (int kb = 0; kb < n; kb += NB) {
+        const int w = std::min(NB, n - kb);
+        hipLaunchKernelGGL((gj_panel_kernel<NB, NT>), dim3(a.batch), dim3(NT), 0, s, a.A, a.sa, n, kb,
+                           a.piv, a.info, a.active);
+        QT_HIP(hipGetLastError());
+        if (w == n) continue;
+        hipLaunchKernelGGL((gj_update_kernel<NB>), dim3((n + 63) / 64, a.batch), dim3(256), 0, s, a.A,
+                           a.sa, n, kb, w, a.piv, a.active);
+        QT_HIP(hipGetLastError());
+    }
 }
 
 }  // namespace
@@ -211,58 +281,15 @@ void zinv(hipStream_t s, const InvArgs& a, Timer* t) {
     if (n <= 0 || a.batch <= 0) return;
     if (n > 1024) throw hip_error("zinv: block size above 1024", 6);
     if (t && t->enabled) t->begin(s, 1);
-    const int nb = (n <= 512) ? 16 : 8;
-    const int64_t sr = (int64_t)16 * n;  // R member stride (sized for nb = 16)
     hipLaunchKernelGGL(zero_int_kernel, dim3((a.batch + 255) / 256), dim3(256), 0, s, a.info, a.batch);
-    const size_t lds = (size_t)n * nb * sizeof(double2);
-    static bool attr_set = false;
-    if (!attr_set) {
-        QT_HIP(hipFuncSetAttribute((const void*)gj_panel_kernel<16>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 512 * 16 * 16));
-        QT_HIP(hipFuncSetAttribute((const void*)gj_panel_kernel<8>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 1024 * 8 * 16));
-        attr_set = true;
-    }
-    for (int kb = 0; kb < n; kb += nb) {
-        const int w = std::min(nb, n - kb);
-        if (nb == 16)
-            hipLaunchKernelGGL(gj_panel_kernel<16>, dim3(a.batch), dim3(256), lds, s, a.A, a.sa, n,
-                               kb, a.piv, a.info, a.active);
-        else
-            hipLaunchKernelGGL(gj_panel_kernel<8>, dim3(a.batch), dim3(256), lds, s, a.A, a.sa, n,
-                               kb, a.piv, a.info, a.active);
-        QT_HIP(hipGetLastError());
-        if (w == n) continue;  // a single panel holds the whole matrix
-        hipLaunchKernelGGL(gj_swap_kernel, dim3((n + 255) / 256, a.batch), dim3(256), 0, s, a.A,
-                           a.sa, n, kb, w, a.piv, a.R, sr, a.active);
-        QT_HIP(hipGetLastError());
-        // A[:, :] += A[:, kb:kb+w] * R   (panel columns masked)
-        ZGemmArgs g;
-        g.M = n;
-        g.N = n;
-        g.K = w;
-        g.batch = a.batch;
-        g.A.p = a.A + (int64_t)kb * n;
-        g.A.ld = n;
-        g.A.sb = a.sa;
-        g.B.p = a.R;
-        g.B.ld = w;
-        g.B.sb = sr;
-        g.C = a.A;
-        g.ldc = n;
-        g.sbc = a.sa;
-        g.D.p = a.A;
-        g.D.ld = n;
-        g.D.sb = a.sa;
-        g.alpha = 1.0;
-        g.beta = 1.0;
-        g.active = a.active;
-        g.skip_lo = kb;
-        g.skip_hi = kb + w;
-        zgemm(s, OP_N, OP_N, g, nullptr);
-    }
-    hipLaunchKernelGGL(gj_finish_kernel, dim3((n + 7) / 8, a.batch), dim3(256), n * sizeof(int), s,
-                       a.A, a.sa, a.X, a.sx, n, a.piv, a.info, a.active);
+    if (n <= 256)
+        run_panels<32, 256>(s, a);
+    else if (n <= 512)
+        run_panels<32, 512>(s, a);
+    else
+        run_panels<16, 1024>(s, a);
+    hipLaunchKernelGGL(gj_finish_kernel, dim3((n + 7) / 8, a.batch), dim3(256), 0, s, a.A, a.sa, a.X,
+                       a.sx, n, a.piv, a.info, a.active);
     QT_HIP(hipGetLastError());
     if (t && t->enabled)
         t->end(s, 1, 8.0 * (double)n * n * n * (a.flop_batch >= 0 ? a.flop_batch : a.batch));

@@ -33,6 +33,7 @@ struct qt_ham {
     double2* C = nullptr;       // [nk][cset]
     double2* S = nullptr;       // [nk][hset] or null (identity)
     bool uniform = true;
+    bool leads_same = false;    // both contacts are the same semi-infinite lead
     int nmax = 0;
 };
 
@@ -130,7 +131,7 @@ struct Gemm {
 };
 
 // ------------------------------------------------------------ workspace
-constexpr int NTEMP = 14;
+constexpr int NTEMP = 15;
 
 struct Work {
     int B = 0;               // points in the batch
@@ -156,6 +157,9 @@ struct Work {
     int* active = nullptr;   // 2B
     int* iters = nullptr;    // 2B
     int* lead_it = nullptr;  // 2B  (first contact, then last contact)
+    int* act1 = nullptr;     // 2B  decimation flags of the two lead surfaces
+    int* act2 = nullptr;     // 2B
+    int* iters2 = nullptr;   // 2B
     int* count = nullptr;    // 1
     double* partial = nullptr;  // 2B * max(64, tiles^2)
     int* infoacc = nullptr;     // B accumulated QT_INFO bits
@@ -209,6 +213,9 @@ static void work_carve(Arena& a, Work& w, const qt_ham& h, int B, bool sigma) {
     w.active = a.take<int>((size_t)2 * B);
     w.iters = a.take<int>((size_t)2 * B);
     w.lead_it = a.take<int>((size_t)2 * B);
+    w.act1 = a.take<int>((size_t)2 * B);
+    w.act2 = a.take<int>((size_t)2 * B);
+    w.iters2 = a.take<int>((size_t)2 * B);
     w.count = a.take<int>(1);
     w.partial = a.take<double>((size_t)2 * B * std::max<int64_t>(64, tiles * tiles));
     w.infoacc = a.take<int>(B);
@@ -269,8 +276,11 @@ struct SanchoIO {
     double2* H10 = nullptr;
     double2* G00 = nullptr;
     double2* GBB = nullptr;
+    double2* G00b = nullptr;      // opposite surface of the same lead (run with
+                                  // H_01 and H_10 exchanged); null = not wanted
     int64_t nn = 0;
     int* iters_out = nullptr;     // members (device)
+    int* iters2_out = nullptr;    // opposite surface
     int* info_bits = nullptr;     // per point accumulator (device), may be null
     int period = 1;               // member m belongs to point m % period
     int64_t total_iters = 0;      // host: sum of iterations over members
@@ -287,6 +297,14 @@ static void sancho_run(Ctx& c, Work& w, SanchoIO& io) {
     copy_blocks(s, n, M, HSS, nn, io.H00, io.kidx, 0, false);
     copy_blocks(s, n, M, HBB, nn, io.H00, io.kidx, 0, false);
     copy_blocks(s, n, M, H01, nn, op_member(H10, n, nn), nullptr, 0, true);
+    const bool dual = io.G00b != nullptr;
+    double2* HSS2 = T[14];
+    if (dual) {
+        copy_blocks(s, n, M, HSS2, nn, io.H00, io.kidx, 0, false);
+        fill_int(s, w.act1, M, 1);
+        fill_int(s, w.act2, M, 1);
+        fill_int(s, w.iters2, M, 0);
+    }
     fill_int(s, w.active, M, 1);
     fill_int(s, w.iters, M, 0);
     int nact = M;
@@ -325,6 +343,13 @@ static void sancho_run(Ctx& c, Work& w, SanchoIO& io) {
         u.Cm = C1;
         u.Ap = Ap;
         u.H01n = H01n;
+        if (dual) {
+            u.HSS2 = HSS2;
+            u.H10n = Cp;
+            u.active1 = w.act1;
+            u.active2 = w.act2;
+            u.iters2 = w.iters2;
+        }
         u.s = nn;
         u.partial = w.partial;
         u.active = w.active;
@@ -351,7 +376,25 @@ static void sancho_run(Ctx& c, Work& w, SanchoIO& io) {
     }
     if (io.iters_out)
         QT_HIP(hipMemcpyAsync(io.iters_out, w.iters, sizeof(int) * M, hipMemcpyDeviceToDevice, s));
+    if (dual && io.iters2_out)
+        QT_HIP(hipMemcpyAsync(io.iters2_out, w.iters2, sizeof(int) * M, hipMemcpyDeviceToDevice, s));
     // G00 = inv(z S00 - H_SS), GBB = inv(z S00 - H_BB)  (rgf_mod.f90:553-557)
+    if (dual) {
+        ShiftArgs sh;
+        sh.n = n;
+        sh.batch = M;
+        sh.out = A;
+        sh.sout = nn;
+        sh.zre = io.zre;
+        sh.zim = io.eta;
+        sh.S = io.S;
+        sh.M1 = op_member(HSS2, n, nn);
+        sh.c1 = -1.0;
+        sh.kidx = io.kidx;
+        shifted(s, sh);
+        inv_batch(c, w, n, M, A, nn, io.G00b, nn, nullptr);
+        if (io.info_bits) note_inv_info(c, w, M, io.period);
+    }
     for (int pass = 0; pass < (io.GBB ? 2 : 1); ++pass) {
         ShiftArgs sh;
         sh.n = n;
@@ -501,9 +544,36 @@ struct Solve {
             copy_blocks(s, nL, B, H10buf + B * nn, nn, opC(cL), w.kidx, 0, !fwd);
             copy_blocks(s, nF, B, H00buf, nn, opH00(sf), w.kidx, 0, false);
             copy_blocks(s, nL, B, H00buf + B * nn, nn, opH00(sl), w.kidx, 0, false);
-            // both contacts in one decimation batch of 2B members when they
-            // share the block size and the overlap is the identity
-            const int parts = (nF == nL && !h.S) ? 1 : 2;
+            // Identical leads: the last contact's decimation is the first
+            // one with H_01 and H_10 exchanged, so one run of B members
+            // carries both surfaces.  Otherwise both contacts go in one batch
+            // of 2B members when they share the block size and the overlap is
+            // the identity.
+            const bool dual = h.leads_same && !sigma;
+            const int parts = dual ? 0 : ((nF == nL && !h.S) ? 1 : 2);
+            if (dual) {
+                SanchoIO io;
+                io.n = nF;
+                io.members = B;
+                io.nn = nn;
+                io.zre = w.zre;
+                io.kidx = w.kidx;
+                io.eta = p.lead_eta;
+                io.nmax = p.lead_nmax;
+                io.tol = p.lead_tol;
+                io.reset = (p.flags & QT_F_LEAD_RESET) != 0;
+                io.S = opS(sf);
+                io.H00 = op_member(H00buf, nF, nn);
+                io.H10 = H10buf;
+                io.G00 = G00buf;
+                io.G00b = G00buf + B * nn;
+                io.iters_out = w.lead_it;
+                io.iters2_out = w.lead_it + B;
+                io.info_bits = w.infoacc;
+                io.period = B;
+                sancho_run(c, w, io);
+                lead_iterations += io.total_iters;
+            }
             for (int part = 0; part < parts; ++part) {
                 SanchoIO io;
                 io.n = parts == 1 ? nF : (part == 0 ? nF : nL);
@@ -870,6 +940,16 @@ int qt_ham_create(int32_t nx, const int32_t* nm, int32_t nk, const void* const* 
                                  sizeof(double2) * (size_t)r * cc, hipMemcpyHostToDevice));
             }
         }
+        // both contacts are the same lead when the outer slabs, the outer
+        // couplings and the outer overlaps coincide for every k-point
+        bool same = nm[0] == nm[nx - 1];
+        for (int k = 0; same && k < nk; ++k) {
+            const size_t bytes = sizeof(double2) * (size_t)nm[0] * nm[0];
+            same = same && !memcmp(Hii[(int64_t)k * nx], Hii[(int64_t)k * nx + nx - 1], bytes);
+            same = same && !memcmp(H1i[(int64_t)k * (nx + 1)], H1i[(int64_t)k * (nx + 1) + nx], bytes);
+            if (Sii) same = same && !memcmp(Sii[(int64_t)k * nx], Sii[(int64_t)k * nx + nx - 1], bytes);
+        }
+        h->leads_same = same;
         *out = h;
         return QT_OK;
     } catch (const hip_error& e) {
@@ -1053,7 +1133,6 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
                 }
             }
             sv.run(tm);
-            tm.lead_iterations += sv.lead_iterations;
             // per-point scalars
             if (dev_out) {
                 if (out->tr) QT_HIP(hipMemcpyAsync(out->tr + p0, w.tr, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
@@ -1086,6 +1165,7 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
                     out->lead_iters[2 * (p0 + b) + 1] = hit[B + b];
                 }
                 if (hinfo[b] & QT_INFO_SINGULAR) status = QT_ERR_SINGULAR;
+                tm.lead_iterations += hit[b] + hit[B + b];  // per lead, as the reference runs them
             }
         }
         QT_HIP(hipEventRecord(e1, c.stream));
