@@ -49,6 +49,12 @@ enum {
 /* RGF variants */
 #define QT_RGF_FORWARD 0   /* rgf_variableblock_forward,  rgf_mod.f90:46-269 */
 #define QT_RGF_BACKWARD 1  /* rgf_variableblock_backward, rgf_mod.f90:272-472 */
+#define QT_RGF_GW 2        /* green_RGF_RS, gw_mod.f90:1131-1345: forward sweep with
+                              the left contact filled with f(E-mul, TEMPl) and its
+                              current taken with f(E-mul, TEMPr), the right contact
+                              with f(E-mur, TEMPl) and tr of opposite sign; the
+                              right lead couples through H1i[nx] (callers replicate
+                              the last coupling) */
 
 /* precision modes */
 #define QT_PREC_FP64 0     /* complex128 everywhere */
@@ -116,6 +122,15 @@ typedef struct {
     void* const* Jdens;
     void* const* Gl;
     void* const* Gln;
+    /* QT_RGF_GW only, [npts*nx] host blocks each, may be NULL:
+     * cur: real nm x nm bond-current blocks, Re(H_{l,l+1} G<_{l+1,l}) -
+     *      Re(H_{l+1,l} G<_{l,l+1}) (gw_mod.f90:1252-1273);
+     * GRi1 / GLi1: G_{l,l+1} and G<_{l,l+1};  GGi1: the reference's
+     * Gl^H H^H G^> term (gw_mod.f90:1275-1284) */
+    void* const* cur;
+    void* const* GRi1;
+    void* const* GLi1;
+    void* const* GGi1;
 } qt_rgf_outputs;
 
 /* Replaces the E x k loop of negf_mod.f90:115-127 around

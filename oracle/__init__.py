@@ -227,3 +227,30 @@ def conv1d2(X: np.ndarray, Y: np.ndarray, flavour: str = "plain") -> np.ndarray:
     lib(flavour).qto_conv1d2(C.c_int(len(X)), X.ctypes.data_as(C.c_void_p),
                              Y.ctypes.data_as(C.c_void_p), Z.ctypes.data_as(C.c_void_p))
     return Z
+
+
+def green_rgf_rs(Hii, H1i, E: float, mu, TEMP, sigma_lesser_ph=None, sigma_r_ph=None,
+                 offdiag: bool = True, flavour: str = "plain") -> dict:
+    """gw_mod.f90:1131-1345 on (nm, nm, nx) stacks (Fortran order)."""
+    Hii = np.asfortranarray(Hii, dtype=np.complex128)
+    H1i = np.asfortranarray(H1i, dtype=np.complex128)
+    nm, _, nx = Hii.shape
+    sl = None if sigma_lesser_ph is None else np.asfortranarray(sigma_lesser_ph, dtype=np.complex128)
+    sr = None if sigma_r_ph is None else np.asfortranarray(sigma_r_ph, dtype=np.complex128)
+    out = {k: np.zeros((nm, nm, nx), np.complex128, order="F") for k in ("ndens", "pdens", "ldos")}
+    out["cur"] = np.zeros((nm, nm, nx), np.float64, order="F")
+    for k in ("GRi1", "GLi1", "GGi1"):
+        out[k] = np.zeros((nm, nm, nx), np.complex128, order="F") if offdiag else None
+    tr = C.c_double(0.0)
+    tre = C.c_double(0.0)
+    T = np.ascontiguousarray(TEMP, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+    st = lib(flavour).qto_green_rgf_rs(p(T), C.c_int(nm), C.c_int(nx), C.c_double(E), p(mu), p(Hii),
+                                       p(H1i), p(sl), p(sr), p(out["ndens"]), p(out["pdens"]),
+                                       p(out["ldos"]), C.byref(tr), C.byref(tre), p(out["cur"]),
+                                       p(out["GRi1"]), p(out["GLi1"]), p(out["GGi1"]))
+    if st != 0:
+        raise ValueError(f"oracle green_rgf_rs returned {st}")
+    out.update(tr=tr.value, tre=tre.value)
+    return out

@@ -797,3 +797,199 @@ void qto_conv1d2(int n, const qto_c* X, const qto_c* Y, qto_c* Z) {
         Z[e] = s;
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* green_RGF_RS, gw_mod.f90:1131-1345                                  */
+/* ------------------------------------------------------------------ */
+
+/* C = op(A) op(B), all nm x nm */
+static void g2(int opa, int opb, int nm, const qto_c* A, const qto_c* B, qto_c* C) {
+    qto_zgemm(opa, opb, nm, nm, nm, 1.0, A, nm, B, nm, 0.0, C, nm);
+}
+
+int qto_green_rgf_rs(const double* TEMP, int nm, int nx, double E, const double* mu,
+                     const qto_c* Hii, const qto_c* H1i, const qto_c* sigma_lesser_ph,
+                     const qto_c* sigma_r_ph, qto_c* ndens, qto_c* pdens, qto_c* ldos,
+                     double* tr, double* tre, double* cur, qto_c* GRi1, qto_c* GLi1,
+                     qto_c* GGi1) {
+    if (nx < 2) return -1;
+    const size_t nn = (size_t)nm * nm;
+    const double mul = mu[0], mur = mu[1], templ = TEMP[0], tempr = TEMP[1];
+    const qto_c z = E + 0.0 * I; /* gw_mod.f90:1155 */
+    qto_c *H00 = zalloc(nn), *H10 = zalloc(nn), *A = zalloc(nn), *B = zalloc(nn), *Cm = zalloc(nn);
+    qto_c *D = zalloc(nn), *S00 = zalloc(nn), *G00 = zalloc(nn), *GBB = zalloc(nn), *GN0 = zalloc(nn);
+    qto_c *Gn = zalloc(nn), *Gp = zalloc(nn), *sig = zalloc(nn), *sigmal = zalloc(nn);
+    qto_c *sigmar = zalloc(nn), *Gl = zalloc(nn * nx), *Gln = zalloc(nn * nx);
+    qto_c* Glp = zalloc(nn * nx); /* never filled by the reference (stays zero) */
+    for (int i = 0; i < nm; ++i) S00[i + (size_t)i * nm] = 1.0;
+#define BLK(p, s) ((p) + (size_t)(s) * nn)
+#define SR(s) (sigma_r_ph ? BLK(sigma_r_ph, s)[e] : 0.0)
+#define SL(s) (sigma_lesser_ph ? BLK(sigma_lesser_ph, s)[e] : 0.0)
+    for (size_t e = 0; e < nn * nx; ++e) { ldos[e] = ndens[e] = pdens[e] = 0.0; cur[e] = 0.0; }
+    /* left contact :1173-1193 */
+    for (size_t e = 0; e < nn; ++e) { H00[e] = BLK(Hii, 0)[e] + SR(0); H10[e] = BLK(H1i, 0)[e]; }
+    dagger(H10, nm, nm, D);
+    qto_sancho(nm, E, 1e-3, 100, 1e-100, 1, S00, H00, D, G00, GBB, NULL);
+    g2(QTO_OP_N, QTO_OP_N, nm, H10, G00, A);
+    g2(QTO_OP_N, QTO_OP_C, nm, A, H10, sigmal);
+    const double fL_inj = qto_ferm((E - mul) / (BOLTZ * templ));
+    for (int j = 0; j < nm; ++j)
+        for (int i = 0; i < nm; ++i) {
+            size_t e = i + (size_t)j * nm;
+            sig[e] = -(sigmal[e] - conj(sigmal[j + (size_t)i * nm])) * fL_inj + SL(0);
+        }
+    for (size_t e = 0; e < nn; ++e) A[e] = z * S00[e] - H00[e] - sigmal[e];
+    qto_invert(A, nm);
+    memcpy(BLK(Gl, 0), A, nn * sizeof(qto_c));
+    g2(QTO_OP_N, QTO_OP_N, nm, A, sig, B);
+    g2(QTO_OP_N, QTO_OP_C, nm, B, A, Cm);
+    memcpy(BLK(Gln, 0), Cm, nn * sizeof(qto_c));
+    /* first pass :1194-1211 */
+    for (int l = 1; l <= nx - 2; ++l) {
+        for (size_t e = 0; e < nn; ++e) { H00[e] = BLK(Hii, l)[e] + SR(l); H10[e] = BLK(H1i, l)[e]; }
+        g2(QTO_OP_N, QTO_OP_N, nm, H10, BLK(Gl, l - 1), B);
+        g2(QTO_OP_N, QTO_OP_C, nm, B, H10, Cm);
+        for (size_t e = 0; e < nn; ++e) A[e] = z * S00[e] - H00[e] - Cm[e];
+        qto_invert(A, nm);
+        memcpy(BLK(Gl, l), A, nn * sizeof(qto_c));
+        memcpy(sig, BLK(Gln, l - 1), nn * sizeof(qto_c));
+        g2(QTO_OP_N, QTO_OP_N, nm, H10, sig, B);
+        g2(QTO_OP_N, QTO_OP_C, nm, B, H10, Cm);
+        for (size_t e = 0; e < nn; ++e) Cm[e] += SL(l);
+        g2(QTO_OP_N, QTO_OP_N, nm, A, Cm, B);
+        g2(QTO_OP_N, QTO_OP_C, nm, B, A, Gn);
+        memcpy(BLK(Gln, l), Gn, nn * sizeof(qto_c));
+    }
+    /* right contact :1212-1247 */
+    const int L = nx - 1;
+    for (size_t e = 0; e < nn; ++e) { H00[e] = BLK(Hii, L)[e] + SR(L); H10[e] = BLK(H1i, L)[e]; }
+    qto_sancho(nm, E, 1e-3, 100, 1e-100, 1, S00, H00, H10, G00, GBB, NULL);
+    g2(QTO_OP_C, QTO_OP_N, nm, H10, G00, A);
+    g2(QTO_OP_N, QTO_OP_N, nm, A, H10, sigmar);
+    g2(QTO_OP_N, QTO_OP_N, nm, H10, BLK(Gl, L - 1), B);
+    g2(QTO_OP_N, QTO_OP_C, nm, B, H10, Cm);
+    for (size_t e = 0; e < nn; ++e) G00[e] = z * S00[e] - H00[e] - sigmar[e] - Cm[e];
+    qto_invert(G00, nm);
+    memcpy(BLK(ldos, L), G00, nn * sizeof(qto_c));
+    memcpy(sig, BLK(Gln, L - 1), nn * sizeof(qto_c));
+    g2(QTO_OP_N, QTO_OP_N, nm, H10, sig, B);
+    g2(QTO_OP_N, QTO_OP_C, nm, B, H10, Cm);
+    const double fR = qto_ferm((E - mur) / (BOLTZ * templ)); /* TEMPl, :1232 */
+    for (int j = 0; j < nm; ++j)
+        for (int i = 0; i < nm; ++i) {
+            size_t e = i + (size_t)j * nm;
+            sig[e] = -(sigmar[e] - conj(sigmar[j + (size_t)i * nm])) * fR + Cm[e] + SL(L);
+        }
+    g2(QTO_OP_N, QTO_OP_N, nm, G00, sig, B);
+    g2(QTO_OP_N, QTO_OP_C, nm, B, G00, Gn);
+    memcpy(BLK(ndens, L), Gn, nn * sizeof(qto_c));
+    for (int j = 0; j < nm; ++j)
+        for (int i = 0; i < nm; ++i) {
+            size_t e = i + (size_t)j * nm;
+            Gp[e] = Gn[e] + (G00[e] - conj(G00[j + (size_t)i * nm]));
+        }
+    memcpy(BLK(pdens, L), Gp, nn * sizeof(qto_c));
+    for (int j = 0; j < nm; ++j)
+        for (int i = 0; i < nm; ++i) {
+            size_t e = i + (size_t)j * nm;
+            A[e] = -(sigmar[e] - conj(sigmar[j + (size_t)i * nm])) * fR;
+        }
+    g2(QTO_OP_N, QTO_OP_N, nm, A, Gp, B);
+    for (int j = 0; j < nm; ++j)
+        for (int i = 0; i < nm; ++i) {
+            size_t e = i + (size_t)j * nm;
+            A[e] = -(sigmar[e] - conj(sigmar[j + (size_t)i * nm])) * (fR - 1.0);
+        }
+    g2(QTO_OP_N, QTO_OP_N, nm, A, Gn, Cm);
+    double tim = 0.0;
+    for (int i = 0; i < nm; ++i) tim -= creal(B[i + (size_t)i * nm] - Cm[i + (size_t)i * nm]);
+    *tr = tim;
+    /* second pass :1250-1330 */
+    for (int l = nx - 2; l >= 0; --l) {
+        memcpy(H10, BLK(H1i, l), nn * sizeof(qto_c));
+        memcpy(A, Gn, nn * sizeof(qto_c));
+        g2(QTO_OP_N, QTO_OP_C, nm, H10, BLK(Gl, l), B);
+        g2(QTO_OP_N, QTO_OP_N, nm, A, B, Cm);
+        memcpy(A, BLK(Gln, l), nn * sizeof(qto_c));
+        g2(QTO_OP_N, QTO_OP_N, nm, H10, A, B);
+        g2(QTO_OP_N, QTO_OP_N, nm, G00, B, A);
+        for (size_t e = 0; e < nn; ++e) B[e] = Cm[e] + A[e];
+        g2(QTO_OP_C, QTO_OP_N, nm, H10, B, A);
+        for (size_t e = 0; e < nn; ++e) BLK(cur, l)[e] = creal(A[e]);
+        memcpy(A, Gn, nn * sizeof(qto_c));
+        g2(QTO_OP_N, QTO_OP_C, nm, BLK(Gl, l), H10, B);
+        g2(QTO_OP_N, QTO_OP_N, nm, B, A, Cm);
+        memcpy(A, BLK(Gln, l), nn * sizeof(qto_c));
+        g2(QTO_OP_N, QTO_OP_C, nm, A, H10, B);
+        g2(QTO_OP_N, QTO_OP_C, nm, B, G00, A);
+        for (size_t e = 0; e < nn; ++e) B[e] = Cm[e] + A[e];
+        if (GLi1) memcpy(BLK(GLi1, l), B, nn * sizeof(qto_c));
+        g2(QTO_OP_N, QTO_OP_N, nm, H10, B, A);
+        for (size_t e = 0; e < nn; ++e) BLK(cur, l)[e] -= creal(A[e]);
+        if (GGi1) {
+            memcpy(A, Gp, nn * sizeof(qto_c));
+            g2(QTO_OP_C, QTO_OP_C, nm, BLK(Gl, l), H10, B);
+            g2(QTO_OP_N, QTO_OP_N, nm, B, A, Cm);
+            memcpy(A, BLK(Glp, l), nn * sizeof(qto_c));
+            g2(QTO_OP_N, QTO_OP_C, nm, A, H10, B);
+            g2(QTO_OP_N, QTO_OP_N, nm, B, G00, A);
+            for (size_t e = 0; e < nn; ++e) BLK(GGi1, l)[e] = Cm[e] + A[e];
+        }
+        memcpy(D, BLK(Gl, l), nn * sizeof(qto_c));
+        g2(QTO_OP_N, QTO_OP_C, nm, D, H10, B);
+        g2(QTO_OP_N, QTO_OP_N, nm, B, G00, GN0);
+        if (GRi1) memcpy(BLK(GRi1, l), GN0, nn * sizeof(qto_c));
+        g2(QTO_OP_N, QTO_OP_N, nm, GN0, H10, A);
+        g2(QTO_OP_N, QTO_OP_N, nm, A, D, Cm);
+        for (size_t e = 0; e < nn; ++e) G00[e] = BLK(Gl, l)[e] + Cm[e];
+        memcpy(BLK(ldos, l), G00, nn * sizeof(qto_c));
+        memcpy(A, Gn, nn * sizeof(qto_c));
+        g2(QTO_OP_N, QTO_OP_C, nm, D, H10, B);
+        g2(QTO_OP_N, QTO_OP_N, nm, B, A, Cm);
+        g2(QTO_OP_N, QTO_OP_N, nm, Cm, H10, A);
+        g2(QTO_OP_N, QTO_OP_C, nm, A, D, Cm);
+        for (size_t e = 0; e < nn; ++e) Gn[e] = BLK(Gln, l)[e] + Cm[e];
+        memcpy(A, BLK(Gln, l), nn * sizeof(qto_c));
+        g2(QTO_OP_N, QTO_OP_N, nm, GN0, H10, B);
+        g2(QTO_OP_N, QTO_OP_N, nm, B, A, Cm);
+        for (size_t e = 0; e < nn; ++e) Gn[e] += Cm[e];
+        g2(QTO_OP_N, QTO_OP_C, nm, A, H10, B);
+        g2(QTO_OP_N, QTO_OP_C, nm, B, GN0, Cm);
+        for (size_t e = 0; e < nn; ++e) Gn[e] += Cm[e];
+        memcpy(BLK(ndens, l), Gn, nn * sizeof(qto_c));
+        for (int j = 0; j < nm; ++j)
+            for (int i = 0; i < nm; ++i) {
+                size_t e = i + (size_t)j * nm;
+                BLK(pdens, l)[e] = Gn[e] + (G00[e] - conj(G00[j + (size_t)i * nm]));
+            }
+    }
+    /* left contact current :1332-1341 (TEMPr with mul) */
+    for (int j = 0; j < nm; ++j)
+        for (int i = 0; i < nm; ++i) {
+            size_t e = i + (size_t)j * nm;
+            Gp[e] = Gn[e] + (G00[e] - conj(G00[j + (size_t)i * nm]));
+        }
+    const double fLc = qto_ferm((E - mul) / (BOLTZ * tempr));
+    for (int j = 0; j < nm; ++j)
+        for (int i = 0; i < nm; ++i) {
+            size_t e = i + (size_t)j * nm;
+            A[e] = -(sigmal[e] - conj(sigmal[j + (size_t)i * nm])) * fLc;
+        }
+    g2(QTO_OP_N, QTO_OP_N, nm, A, Gp, B);
+    for (int j = 0; j < nm; ++j)
+        for (int i = 0; i < nm; ++i) {
+            size_t e = i + (size_t)j * nm;
+            A[e] = -(sigmal[e] - conj(sigmal[j + (size_t)i * nm])) * (fLc - 1.0);
+        }
+    g2(QTO_OP_N, QTO_OP_N, nm, A, Gn, Cm);
+    tim = 0.0;
+    for (int i = 0; i < nm; ++i) tim += creal(B[i + (size_t)i * nm] - Cm[i + (size_t)i * nm]);
+    *tre = tim;
+#undef BLK
+#undef SR
+#undef SL
+    free(H00); free(H10); free(A); free(B); free(Cm); free(D); free(S00); free(G00); free(GBB);
+    free(GN0); free(Gn); free(Gp); free(sig); free(sigmal); free(sigmar); free(Gl); free(Gln);
+    free(Glp);
+    return 0;
+}

@@ -91,6 +91,19 @@ int qto_rgf_forward(const qto_point* p, qto_result* r);
 /* rgf_mod.f90:272-472 */
 int qto_rgf_backward(const qto_point* p, qto_result* r);
 
+/* gw_mod.f90:1131-1345 (green_RGF_RS): RGF inside the GW loop on plain
+ * arrays.  All blocks nm x nm, column-major, slab s at offset s*nm*nm.
+ * Hii, H1i (nx blocks; H1i[s] couples slab s to s-1, H1i[0] the left lead,
+ * H1i[nx-1] also the right lead), sigma_* (nx blocks, may be NULL = zero).
+ * Outputs ndens/pdens/ldos (nx blocks), cur (nx real blocks), optional
+ * GRi1/GLi1/GGi1 (nx blocks, may be NULL).  TEMP[2], mu[2] as in the
+ * reference; lead decimation with eta=1e-3, nmax=100 and reset at nmax. */
+int qto_green_rgf_rs(const double* TEMP, int nm, int nx, double E, const double* mu,
+                     const qto_c* Hii, const qto_c* H1i, const qto_c* sigma_lesser_ph,
+                     const qto_c* sigma_r_ph, qto_c* ndens, qto_c* pdens, qto_c* ldos,
+                     double* tr, double* tre, double* cur, qto_c* GRi1, qto_c* GLi1,
+                     qto_c* GGi1);
+
 /* Batch driver for the CPU baseline: the E x k loop of negf_mod.f90:115-127
  * with one point per OpenMP thread.  Hii/H1i/Sii hold nk sets (set ik at
  * [ik*nx], [ik*(nx+1)], [ik*nx]); kidx[ip] selects the set of point ip.

@@ -152,7 +152,8 @@ __global__ __launch_bounds__(256) void current_partial_kernel(CurrentArgs a, int
     if (threadIdx.x == 0) a.partial[(int64_t)b * gridDim.x + blockIdx.x] = red[0];
 }
 
-__global__ void reduce_rows_kernel(const double* partial, int ncol, double* out, const int* active) {
+__global__ void reduce_rows_kernel(const double* partial, int ncol, double* out, const int* active,
+                                   double scale) {
     const int b = blockIdx.x;
     if (active && !active[b]) return;
     __shared__ double red[256];
@@ -164,7 +165,16 @@ __global__ void reduce_rows_kernel(const double* partial, int ncol, double* out,
         if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[b] = red[0];
+    if (threadIdx.x == 0) out[b] = scale * red[0];
+}
+
+__global__ __launch_bounds__(256) void real_diff_kernel(int n, double* out, int64_t sout, ZOp X1, ZOp X2) {
+    const int b = blockIdx.y;
+    const int64_t nn = (int64_t)n * n;
+    for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < nn; e += (int64_t)gridDim.x * 256) {
+        const int j = (int)(e / n), i = (int)(e - (int64_t)j * n);
+        out[b * sout + e] = X1.p[b * X1.sb + i + j * X1.ld].x - X2.p[b * X2.sb + i + j * X2.ld].x;
+    }
 }
 
 // ---------------------------------------------------------------- diagonals
@@ -362,7 +372,7 @@ void contact_current(hipStream_t s, const CurrentArgs& a) {
     hipLaunchKernelGGL(current_partial_kernel, dim3(nt * nt, a.batch), dim3(256), 0, s, a, nt);
     QT_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_rows_kernel, dim3(a.batch), dim3(256), 0, s, a.partial, nt * nt, a.out,
-                       a.active);
+                       a.active, a.scale);
     QT_HIP(hipGetLastError());
 }
 
@@ -379,6 +389,13 @@ void sancho_update(hipStream_t s, const SanchoUpdArgs& a) {
     hipLaunchKernelGGL(sancho_update_kernel, dim3(nblk, a.batch), dim3(256), 0, s, a);
     QT_HIP(hipGetLastError());
     hipLaunchKernelGGL(sancho_decide_kernel, dim3(a.batch), dim3(256), 0, s, a, nblk);
+    QT_HIP(hipGetLastError());
+}
+
+void real_diff(hipStream_t s, int n, int batch, double* out, int64_t sout, ZOp X1, ZOp X2) {
+    if (batch <= 0) return;
+    hipLaunchKernelGGL(real_diff_kernel, dim3(elem_blocks((int64_t)n * n), batch), dim3(256), 0, s, n,
+                       out, sout, X1, X2);
     QT_HIP(hipGetLastError());
 }
 

@@ -119,6 +119,7 @@ struct CurrentArgs {
     double* partial = nullptr;     // scratch [batch][tiles]
     double* out = nullptr;         // [batch] (device)
     const int* active = nullptr;
+    double scale = 1.0;            // sign convention of the caller
 };
 void contact_current(hipStream_t s, const CurrentArgs& a);
 
@@ -157,6 +158,9 @@ struct SanchoUpdArgs {
     const int* kidx = nullptr;
 };
 void sancho_update(hipStream_t s, const SanchoUpdArgs& a);
+
+// out[b] (real n x n) = Re(X1[b]) - Re(X2[b])
+void real_diff(hipStream_t s, int n, int batch, double* out, int64_t sout, ZOp X1, ZOp X2);
 
 // copies: dst[b] = src[b] (n x n), optional conjugate transpose
 void copy_blocks(hipStream_t s, int n, int batch, double2* dst, int64_t sdst, ZOp src,

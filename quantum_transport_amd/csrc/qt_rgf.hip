@@ -150,7 +150,8 @@ struct Work {
     double2* R = nullptr;    // 2B * 16 * nmax
     double* zre = nullptr;   // 2B energies (the batch twice, for the lead members)
     int* kidx = nullptr;     // 2B
-    double* fF = nullptr;    // B  occupation of the first contact
+    double* fF = nullptr;    // B  occupation injected at the first contact
+    double* fFc = nullptr;   // B  occupation of the first contact's current
     double* fL = nullptr;    // B  occupation of the last contact
     double* nfF = nullptr;   // B  -fF
     double* nfL = nullptr;   // B  -fL
@@ -207,6 +208,7 @@ static void work_carve(Arena& a, Work& w, const qt_ham& h, int B, bool sigma) {
     w.zre = a.take<double>((size_t)2 * B);
     w.kidx = a.take<int>((size_t)2 * B);
     w.fF = a.take<double>(B);
+    w.fFc = a.take<double>(B);
     w.fL = a.take<double>(B);
     w.nfF = a.take<double>(B);
     w.nfL = a.take<double>(B);
@@ -422,9 +424,24 @@ struct FullOut {
     void* const* Jdens = nullptr;
     void* const* Gl = nullptr;
     void* const* Gln = nullptr;
+    void* const* cur = nullptr;   // QT_RGF_GW extras
+    void* const* GRi1 = nullptr;
+    void* const* GLi1 = nullptr;
+    void* const* GGi1 = nullptr;
     int nx = 0;
     int p0 = 0;  // first point of the batch
 };
+
+static void d2h_real_blocks(Ctx& c, void* const* dst, int nx, int p0, int B, int s, int n,
+                            const double* src, int64_t sb) {
+    if (!dst) return;
+    QT_HIP(hipStreamSynchronize(c.stream));
+    for (int b = 0; b < B; ++b) {
+        void* d = dst[(int64_t)(p0 + b) * nx + s];
+        if (!d) continue;
+        QT_HIP(hipMemcpy(d, src + b * sb, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost));
+    }
+}
 
 static void d2h_blocks(Ctx& c, void* const* dst, int nx, int p0, int B, int s, int n,
                        const double2* src, int64_t sb) {
@@ -447,6 +464,7 @@ struct Solve {
     bool fwd;
     bool fix;
     bool sigma;
+    bool gw;       // green_RGF_RS bookkeeping (QT_RGF_GW)
     FullOut full;
     double2* gr = nullptr;  // diag destinations (device), already offset to the batch
     double2* glo = nullptr;
@@ -513,6 +531,52 @@ struct Solve {
         if (jd) d2h_blocks(c, full.Jdens, full.nx, full.p0, B, s, n, jd, w.nn);
     }
 
+    // green_RGF_RS extras of slab sc (gw_mod.f90:1250-1291), with its solved
+    // neighbour's G (Gc) and G^< (Lc), X = Gl W^H, Y = X G^<, Q = W Gln:
+    //   GLi1 = G<_{l,l+1} = Y - (Gc Q)^H          (Gln anti-Hermitian)
+    //   A    = G<_{l+1,l} = Lc X^H + Gc Q
+    //   cur  = Re(W^H A) - Re(W GLi1),  GRi1 = GN0,  GGi1 = Gl^H W^H G^>_last
+    void gw_extras(int sc, int n, const ZOp& W, double2* X, double2* Y, double2* Q, double2* GN0,
+                   double2* Gc, double2* Lc, const Gemm& g) {
+        hipStream_t s = c.stream;
+        const int64_t nn = w.nn;
+        double2* const* T = w.T;
+        double2* R = T[10];
+        double2* Bm = T[11];
+        double2* Acur = T[14];
+        double2* P1 = T[13] + (int64_t)B * nn;
+        double2* P2 = T[12];
+        g(OP_N, OP_N, n, n, n, op_member(Gc, n, nn), op_member(Q, n, nn), R, n, nn);   // Gc Q
+        AntiArgs an;  // Bm = (R - R^H) + Y - R = Y - R^H
+        an.n = n;
+        an.batch = B;
+        an.out = Bm;
+        an.sout = nn;
+        an.X = op_member(R, n, nn);
+        an.aconst = 1.0;
+        an.Y = op_member(Y, n, nn);
+        an.c1 = 1.0;
+        an.Z = op_member(R, n, nn);
+        an.c2 = -1.0;
+        antiherm(s, an);
+        g(OP_N, OP_C, n, n, n, op_member(Lc, n, nn), op_member(X, n, nn), Acur, n, nn, 1.0,
+          op_member(R, n, nn), 1.0);                                                    // Lc X^H + Gc Q
+        if (full.cur) {
+            g(OP_C, OP_N, n, n, n, W, op_member(Acur, n, nn), P1, n, nn);
+            g(OP_N, OP_N, n, n, n, W, op_member(Bm, n, nn), P2, n, nn);
+            double* curb = (double*)T[2];  // P is consumed by now; 2 nn doubles per member
+            real_diff(s, n, B, curb, nn, op_member(P1, n, nn), op_member(P2, n, nn));
+            d2h_real_blocks(c, full.cur, full.nx, full.p0, B, sc, n, curb, nn);
+        }
+        d2h_blocks(c, full.GLi1, full.nx, full.p0, B, sc, n, Bm, nn);
+        d2h_blocks(c, full.GRi1, full.nx, full.p0, B, sc, n, GN0, nn);
+        if (full.GGi1) {
+            g(OP_C, OP_C, n, n, n, opGl(sc), W, R, n, nn);                                // Gl^H W^H
+            g(OP_N, OP_N, n, n, n, op_member(R, n, nn), op_member(T[13], n, nn), P2, n, nn);
+            d2h_blocks(c, full.GGi1, full.nx, full.p0, B, sc, n, P2, nn);
+        }
+    }
+
     void run(qt_timing& tm) {
         hipStream_t s = c.stream;
         const int nx = h.nx;
@@ -521,7 +585,8 @@ struct Solve {
         const int sl = fwd ? nx - 1 : 0;  // last slab of pass 1
         const int step = fwd ? 1 : -1;
         const int nF = h.nm[sf], nL = h.nm[sl];
-        const bool want_j = full.any && full.Jdens;
+        const bool want_j = full.any && full.Jdens && !gw;
+        const bool want_gw = gw && full.any && (full.cur || full.GLi1 || full.GRi1 || full.GGi1);
         double2* const* T = w.T;
         Gemm g{s, &c.timer, B, w.kidx};
         QT_HIP(hipEventRecord(c.ev[0], s));
@@ -713,8 +778,22 @@ struct Solve {
             ca.f = w.fL;
             ca.partial = w.partial;
             ca.out = fwd ? w.tr : w.tre;
+            ca.scale = gw ? -1.0 : 1.0;  // gw_mod.f90:1245
             contact_current(s, ca);
             diag_out(sl, Gc, Lc);
+            if (want_gw && full.GGi1) {
+                // G^> of this slab, used by every GGi1 block (gw_mod.f90:1237, 1276)
+                AntiArgs gp;
+                gp.n = n;
+                gp.batch = B;
+                gp.out = T[13];
+                gp.sout = nn;
+                gp.X = op_member(Gc, n, nn);
+                gp.aconst = 1.0;
+                gp.Y = op_member(Lc, n, nn);
+                gp.c1 = 1.0;
+                antiherm(s, gp);
+            }
             const double2* jd = nullptr;
             if (want_j) {
                 // Jdens = Om (G^< + f (G - G^H)),  Om = -(Sigma - Sigma^H)
@@ -761,6 +840,7 @@ struct Solve {
             g(OP_N, OP_C, n, n, nq, op_member(Y, n, nn), op_member(X, n, nn), U, n, nn, 1.0,
               opGln(sc), 1.0);                                                                 // U = Gln + Y X^H
             g(OP_N, OP_N, nq, n, n, W, opGln(sc), Q, nq, nn);                                  // Q = W Gln
+            if (want_gw) gw_extras(sc, n, W, X, Y, Q, GN0, Gc, Lc, g);
             g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(Q, nq, nn), Y, n, nn);    // Tm = GN0 Q
             // G< = U + Tm - Tm^H: the reference's last term Gln W^H GN0^H equals
             // -(GN0 W Gln)^H because Gln is anti-Hermitian
@@ -797,7 +877,7 @@ struct Solve {
             ca.Sig = op_member(w.sigF, nF, nn);
             ca.G = op_member(Gc, nF, nn);
             ca.Gl = op_member(Lc, nF, nn);
-            ca.f = w.fF;
+            ca.f = w.fFc;
             ca.partial = w.partial;
             ca.out = fwd ? w.tre : w.tr;
             contact_current(s, ca);
@@ -992,10 +1072,12 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
     if (!h || !p || npts < 0 || (npts > 0 && !En) || !out) return QT_ERR_INVALID;
     if (npts == 0) return QT_OK;
     if (p->precision != QT_PREC_FP64) return QT_ERR_UNSUPPORTED;
-    const bool fwd = p->variant == QT_RGF_FORWARD;
-    if (!fwd && p->variant != QT_RGF_BACKWARD) return QT_ERR_INVALID;
+    if (p->variant != QT_RGF_FORWARD && p->variant != QT_RGF_BACKWARD && p->variant != QT_RGF_GW)
+        return QT_ERR_INVALID;
+    const bool gw = p->variant == QT_RGF_GW;
+    const bool fwd = p->variant != QT_RGF_BACKWARD;
     const bool fix = (p->flags & QT_F_COUPLING_FIX) != 0;
-    if (!h->uniform && (!fwd || !fix)) return QT_ERR_UNSUPPORTED;
+    if (!h->uniform && (!fwd || !fix || gw)) return QT_ERR_UNSUPPORTED;
     if ((sigma_lesser_ph == nullptr) != (sigma_r_ph == nullptr)) return QT_ERR_INVALID;
     for (int i = 0; i < npts; ++i)
         if (kidx && (kidx[i] < 0 || kidx[i] >= h->nk)) return QT_ERR_INVALID;
@@ -1020,7 +1102,7 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
         int64_t od = 0;
         for (int s = 0; s < h->nx; ++s) { doff[s] = od; od += h->nm[s]; }
         const int64_t ndiag = od;
-        std::vector<double> hz(2 * B), hf(4 * B);
+        std::vector<double> hz(2 * B), hf(5 * B);
         std::vector<int> hk(2 * B);
         std::vector<int> hinfo(B), hit(2 * B);
         std::vector<double> htr(B), htre(B);
@@ -1033,13 +1115,23 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
                 hz[b] = hz[B + b] = En[ip];
                 hk[b] = hk[B + b] = kidx ? kidx[ip] : 0;
                 // occupations: rgf_mod.f90:100,182 (forward) / :321,395 (backward):
-                // the first contact uses (mur, TEMPr), the last one (mul, TEMPl)
-                const double fF = ferm((En[ip] - p->mur) / (BOLTZ * p->tempr));
-                const double fL = ferm((En[ip] - p->mul) / (BOLTZ * p->templ));
+                // the first contact uses (mur, TEMPr), the last one (mul, TEMPl);
+                // gw_mod.f90:1185,1232,1333: left (mul, TEMPl) injected, its
+                // current with (mul, TEMPr), right (mur, TEMPl)
+                double fF, fFc, fL;
+                if (gw) {
+                    fF = ferm((En[ip] - p->mul) / (BOLTZ * p->templ));
+                    fFc = ferm((En[ip] - p->mul) / (BOLTZ * p->tempr));
+                    fL = ferm((En[ip] - p->mur) / (BOLTZ * p->templ));
+                } else {
+                    fF = fFc = ferm((En[ip] - p->mur) / (BOLTZ * p->tempr));
+                    fL = ferm((En[ip] - p->mul) / (BOLTZ * p->templ));
+                }
                 hf[b] = fF;
                 hf[B + b] = fL;
                 hf[2 * B + b] = -fF;
                 hf[3 * B + b] = -fL;
+                hf[4 * B + b] = fFc;
             }
             QT_HIP(hipMemcpyAsync(w.zre, hz.data(), sizeof(double) * 2 * B, hipMemcpyHostToDevice, c.stream));
             QT_HIP(hipMemcpyAsync(w.kidx, hk.data(), sizeof(int) * 2 * B, hipMemcpyHostToDevice, c.stream));
@@ -1047,6 +1139,7 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
             QT_HIP(hipMemcpyAsync(w.fL, hf.data() + B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
             QT_HIP(hipMemcpyAsync(w.nfF, hf.data() + 2 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
             QT_HIP(hipMemcpyAsync(w.nfL, hf.data() + 3 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.fFc, hf.data() + 4 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
             fill_int(c.stream, w.infoacc, B, 0);
             if (sigma) {
                 // H00 = Hii + sigma_r_ph * S,  SLS = sigma_lesser_ph * S  (rgf_mod.f90:120-121, 132)
@@ -1084,7 +1177,7 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
                     shifted(c.stream, sh);
                 }
             }
-            Solve sv{c, *h, *p, w, B, fwd, fix, sigma};
+            Solve sv{c, *h, *p, w, B, fwd, fix, sigma, gw};
             sv.doff = doff;
             const bool want_diag = out->gr_diag || out->gl_diag || out->gg_diag;
             if (want_diag) {
@@ -1103,7 +1196,8 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
                     sv.gg = w.diag + 2 * B * ndiag;
                 }
             }
-            sv.full.any = out->G_r || out->G_lesser || out->G_greater || out->Jdens || out->Gl || out->Gln;
+            sv.full.any = out->G_r || out->G_lesser || out->G_greater || out->Jdens || out->Gl || out->Gln ||
+                          (gw && (out->cur || out->GRi1 || out->GLi1 || out->GGi1));
             if (sv.full.any) {
                 if (nb < B) {
                     // full-block outputs are written per real point only
@@ -1114,16 +1208,24 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
                 sv.full.Jdens = out->Jdens;
                 sv.full.Gl = out->Gl;
                 sv.full.Gln = out->Gln;
+                if (gw) {
+                    sv.full.cur = out->cur;
+                    sv.full.GRi1 = out->GRi1;
+                    sv.full.GLi1 = out->GLi1;
+                    sv.full.GGi1 = out->GGi1;
+                }
                 sv.full.nx = h->nx;
                 sv.full.p0 = p0;
             }
             if (sv.full.any && nb < B) {
                 // restrict host block copies to the real points of this batch
-                static thread_local std::vector<void*> bufs[6];
-                void* const* src[6] = {out->G_r, out->G_lesser, out->G_greater, out->Jdens, out->Gl, out->Gln};
-                void* const** dst[6] = {&sv.full.G_r, &sv.full.G_lesser, &sv.full.G_greater,
-                                        &sv.full.Jdens, &sv.full.Gl, &sv.full.Gln};
-                for (int q = 0; q < 6; ++q) {
+                static thread_local std::vector<void*> bufs[10];
+                void* const* src[10] = {out->G_r, out->G_lesser, out->G_greater, out->Jdens, out->Gl, out->Gln,
+                                        sv.full.cur, sv.full.GRi1, sv.full.GLi1, sv.full.GGi1};
+                void* const** dst[10] = {&sv.full.G_r, &sv.full.G_lesser, &sv.full.G_greater,
+                                         &sv.full.Jdens, &sv.full.Gl, &sv.full.Gln, &sv.full.cur,
+                                         &sv.full.GRi1, &sv.full.GLi1, &sv.full.GGi1};
+                for (int q = 0; q < 10; ++q) {
                     if (!src[q]) continue;
                     bufs[q].assign((size_t)(p0 + B) * h->nx, nullptr);
                     for (int ip = p0; ip < p0 + nb; ++ip)

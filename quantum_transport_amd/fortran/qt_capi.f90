@@ -7,7 +7,7 @@ module qt_capi
     implicit none
 
     integer(c_int), parameter :: QT_OK = 0, QT_ERR_SINGULAR = 5
-    integer(c_int32_t), parameter :: QT_RGF_FORWARD = 0, QT_RGF_BACKWARD = 1
+    integer(c_int32_t), parameter :: QT_RGF_FORWARD = 0, QT_RGF_BACKWARD = 1, QT_RGF_GW = 2
     integer(c_int32_t), parameter :: QT_PREC_FP64 = 0
     integer(c_int32_t), parameter :: QT_F_COUPLING_FIX = 1, QT_F_DEVICE_OUTPUTS = 2, QT_F_LEAD_RESET = 4
     integer(c_int32_t), parameter :: QT_INFO_SINGULAR = 1, QT_INFO_LEAD_NMAX = 2
@@ -27,6 +27,7 @@ module qt_capi
         type(c_ptr) :: gr_diag = c_null_ptr, gl_diag = c_null_ptr, gg_diag = c_null_ptr
         type(c_ptr) :: G_r = c_null_ptr, G_lesser = c_null_ptr, G_greater = c_null_ptr
         type(c_ptr) :: Jdens = c_null_ptr, Gl = c_null_ptr, Gln = c_null_ptr
+        type(c_ptr) :: cur = c_null_ptr, GRi1 = c_null_ptr, GLi1 = c_null_ptr, GGi1 = c_null_ptr
     end type qt_rgf_outputs
 
     interface

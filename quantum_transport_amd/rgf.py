@@ -24,6 +24,7 @@ LIBPATH = os.path.join(HERE, "libqtrgf.so")
 
 QT_RGF_FORWARD = 0
 QT_RGF_BACKWARD = 1
+QT_RGF_GW = 2
 QT_PREC_FP64 = 0
 QT_F_COUPLING_FIX = 0x1
 QT_F_DEVICE_OUTPUTS = 0x2
@@ -57,7 +58,9 @@ class Outputs(C.Structure):
                 ("gr_diag", C.c_void_p), ("gl_diag", C.c_void_p), ("gg_diag", C.c_void_p),
                 ("G_r", C.POINTER(C.c_void_p)), ("G_lesser", C.POINTER(C.c_void_p)),
                 ("G_greater", C.POINTER(C.c_void_p)), ("Jdens", C.POINTER(C.c_void_p)),
-                ("Gl", C.POINTER(C.c_void_p)), ("Gln", C.POINTER(C.c_void_p))]
+                ("Gl", C.POINTER(C.c_void_p)), ("Gln", C.POINTER(C.c_void_p)),
+                ("cur", C.POINTER(C.c_void_p)), ("GRi1", C.POINTER(C.c_void_p)),
+                ("GLi1", C.POINTER(C.c_void_p)), ("GGi1", C.POINTER(C.c_void_p))]
 
 
 class Timing(C.Structure):
@@ -167,8 +170,8 @@ def make_params(variant: str = "forward", mul: float = 0.0, mur: float = 0.0,
                 lead_reset: bool = False, device_outputs: bool = False) -> Params:
     flags = (QT_F_COUPLING_FIX if coupling_fix else 0) | (QT_F_DEVICE_OUTPUTS if device_outputs else 0) \
         | (QT_F_LEAD_RESET if lead_reset else 0)
-    return Params(QT_RGF_FORWARD if variant == "forward" else QT_RGF_BACKWARD, QT_PREC_FP64, flags,
-                  lead_nmax, mul, mur, templ, tempr, lead_eta, lead_tol)
+    code = {"forward": QT_RGF_FORWARD, "backward": QT_RGF_BACKWARD, "gw": QT_RGF_GW}[variant]
+    return Params(code, QT_PREC_FP64, flags, lead_nmax, mul, mur, templ, tempr, lead_eta, lead_tol)
 
 
 def solve_batch(H: Hamiltonian, En, kidx=None, *, variant: str = "forward", mul: float = 0.0,
@@ -371,3 +374,64 @@ def invert_batched(A: np.ndarray):
     if st not in (0, QT_ERR_SINGULAR):
         check(st)
     return np.transpose(Af, (0, 2, 1)), info
+
+
+def green_rgf_rs(TEMP, nm: int, nx: int, E: float, mu, Hii, H1i, sigma_lesser_ph=None,
+                 sigma_greater_ph=None, sigma_r_ph=None, offdiag: bool = False) -> dict:
+    """green_RGF_RS (gw_mod.f90:1131-1345) on the device for one energy.
+    Hii, H1i, sigma_*: (nm, nm, nx) arrays (Fortran order, as the reference
+    passes them); sigma_greater_ph is accepted and unused, as in the
+    reference.  Returns ndens, pdens, ldos (nm, nm, nx), cur (real), tr, tre
+    and, with offdiag, GRi1, GLi1, GGi1."""
+    Hii = np.asarray(Hii, dtype=np.complex128)
+    H1i = np.asarray(H1i, dtype=np.complex128)
+    hii = [Hii[:, :, s] for s in range(nx)]
+    h1i = [H1i[:, :, s] for s in range(nx)] + [H1i[:, :, nx - 1]]  # right lead: H1i(nx)
+    H = Hamiltonian([hii], [h1i])
+    init()
+    keep = []
+
+    def table(nblk, dtype):
+        blocks = [np.zeros((nm, nm), dtype, order="F") for _ in range(nblk)]
+        arr = _ptrs(blocks)
+        keep.append(arr)
+        return blocks, C.cast(arr, C.POINTER(C.c_void_p))
+
+    o = Outputs()
+    tr = np.zeros(1)
+    tre = np.zeros(1)
+    info = np.zeros(1, np.int32)
+    o.tr = tr.ctypes.data_as(C.POINTER(C.c_double))
+    o.tre = tre.ctypes.data_as(C.POINTER(C.c_double))
+    o.info = info.ctypes.data_as(C.POINTER(C.c_int32))
+    res = {}
+    for key, field, dt in (("ldos", "G_r", np.complex128), ("ndens", "G_lesser", np.complex128),
+                           ("pdens", "G_greater", np.complex128), ("cur", "cur", np.float64)):
+        blocks, ptr = table(nx, dt)
+        res[key] = blocks
+        setattr(o, field, ptr)
+    if offdiag:
+        for key in ("GRi1", "GLi1", "GGi1"):
+            blocks, ptr = table(nx, np.complex128)
+            res[key] = blocks
+            setattr(o, key, ptr)
+    sl = sr = None
+    if sigma_r_ph is not None:
+        sls = [_f(np.asarray(sigma_lesser_ph)[:, :, s]) for s in range(nx)]
+        srs = [_f(np.asarray(sigma_r_ph)[:, :, s]) for s in range(nx)]
+        keep += [sls, srs]
+        sl, sr = _ptrs(sls), _ptrs(srs)
+    p = make_params("gw", mu[0], mu[1], TEMP[0], TEMP[1], lead_eta=1e-3, lead_nmax=100,
+                    lead_reset=True)
+    En = np.array([E], dtype=np.float64)
+    k = np.zeros(1, np.int32)
+    try:
+        st = lib().qt_rgf_solve(H.handle, C.byref(p), C.c_int32(1), En.ctypes.data_as(C.c_void_p),
+                                k.ctypes.data_as(C.c_void_p), sl, sr, C.byref(o))
+        if st != QT_ERR_SINGULAR:
+            check(st)
+    finally:
+        H.close()
+    out = {key: np.stack(v, axis=2) for key, v in res.items()}
+    out.update(tr=float(tr[0]), tre=float(tre[0]), info=int(info[0]))
+    return out

@@ -1,0 +1,64 @@
+"""Host-side input preparation: deterministic generator, ham_dat round trip,
+w90_MAT_DEF block convention (wannierHam3d.f90:273-301), energy grid
+(negf_mod.f90:100-101)."""
+import numpy as np
+
+from quantum_transport_amd import hamiltonian as hm
+
+
+def test_splitmix_known_values():
+    # SplitMix64 reference outputs for seed 0 (Steele et al. 2014 / Vigna's splitmix64.c)
+    r = hm.SplitMix64(0).u64(3)
+    assert [int(x) for x in r] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_generator_is_deterministic_and_hermitian():
+    a, da = hm.synthetic_device(24, 3, seed=5, kz=(0.0, 0.9), tz_density=0.1)
+    b, db = hm.synthetic_device(24, 3, seed=5, kz=(0.0, 0.9), tz_density=0.1)
+    for ik in range(2):
+        H = da.Hii[ik][0]
+        np.testing.assert_array_equal(H, db.Hii[ik][0])
+        np.testing.assert_allclose(H, H.conj().T, atol=1e-15)
+    # H_ii(kz) = H_ii + T_z e^{ikz} + h.c.
+    Tz = a.Hr[:, :, 1, 0, 0]
+    H0 = a.Hr[:, :, 1, 0, 1]
+    np.testing.assert_allclose(da.Hii[1][0], H0 + Tz * np.exp(0.9j) + Tz.conj().T * np.exp(-0.9j),
+                               atol=1e-14)
+    # the coupling is H_{i,i-1} = Hr(x=+1) = Hr(x=-1)^H, the same on every slab and lead
+    np.testing.assert_array_equal(da.H1i[0][0], a.Hr[:, :, 2, 0, 1])
+    np.testing.assert_allclose(a.Hr[:, :, 2, 0, 1], a.Hr[:, :, 0, 0, 1].conj().T)
+    assert len(da.H1i[0]) == 4
+    # about 8 in-block neighbours per site, hopping around -2.7 eV
+    off = H0 - np.diag(np.diag(H0))
+    deg = (np.abs(off) > 0).sum(1)
+    assert 5 <= deg.mean() <= 9
+    assert abs(np.real(off[np.abs(off) > 0]).mean() + 2.7) < 0.2
+
+
+def test_ham_dat_round_trip(tmp_path):
+    w = hm.synthetic_wannier(10, 3, tz_density=0.2)
+    hm.write_ham_dat(tmp_path / "ham_dat", w)
+    r = hm.read_ham_dat(str(tmp_path / "ham_dat"))
+    np.testing.assert_array_equal(r.Hr, w.Hr)
+    assert (r.xmin, r.ymin, r.zmin) == (w.xmin, w.ymin, w.zmin)
+    np.testing.assert_array_equal(r.centres, w.centres)
+
+
+def test_mat_def_blocks_of_ns_cells(golden_dir):
+    """ns cells per slab: block (i,k) of Hii is Hr(x=i-k), of H1i Hr(x=i-k+ns)."""
+    w = hm.read_ham_dat(f"{golden_dir}/ham_dat")
+    Hii, H1i = w.mat_def(0.0, 0.0, 0.0, 3)
+    nb = w.nb
+    x0 = -w.xmin
+    for i in range(3):
+        for k in range(3):
+            np.testing.assert_array_equal(Hii[i * nb:(i + 1) * nb, k * nb:(k + 1) * nb],
+                                          w.Hr[:, :, x0 + i - k, 0, 0])
+            np.testing.assert_array_equal(H1i[i * nb:(i + 1) * nb, k * nb:(k + 1) * nb],
+                                          w.Hr[:, :, x0 + i - k + 3, 0, 0])
+    np.testing.assert_allclose(Hii, Hii.conj().T, atol=1e-12)
+
+
+def test_energy_grid():
+    E = hm.energy_grid(-6.0, -1.0, 200)
+    assert E[0] == -6.0 and abs(E[-1] + 1.0) < 1e-14 and len(E) == 200

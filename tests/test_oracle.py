@@ -311,3 +311,81 @@ def test_corr_conv_match_fft_branch(oracle_mod, n):
     Y = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     np.testing.assert_allclose(oracle_mod.corr1d2(X, Y), _fft_branch_corr(X, Y), atol=1e-12)
     np.testing.assert_allclose(oracle_mod.conv1d2(X, Y), _fft_branch_conv(X, Y), atol=1e-12)
+
+
+# --------------------------------------------------------------- green_RGF_RS
+
+
+def _gw_inputs(seed=21, n=8, nx=5):
+    """identical slabs plus block-diagonal scattering self-energies"""
+    rng = np.random.default_rng(seed)
+    _, dev = hm.synthetic_device(n, nx, seed=seed, coupling_density=0.4)
+    Hii = np.stack(dev.Hii[0], axis=2)
+    H1i = np.stack(dev.H1i[0][:nx], axis=2)
+    sr = np.zeros((n, n, nx), complex)
+    sl = np.zeros((n, n, nx), complex)
+    for s in range(nx):
+        a = 0.03 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        sr[:, :, s] = (a + a.T) / 2 - 0.04j * np.eye(n)  # complex symmetric, Im < 0
+        b = 0.05 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        sl[:, :, s] = 1j * (b @ b.conj().T)               # anti-Hermitian
+    return dev, Hii, H1i, sr, sl
+
+
+@pytest.mark.parametrize("E", [-0.2, 0.45])
+def test_green_rgf_rs_dense(oracle_mod, E):
+    """gw_mod.f90:1131-1345 against the dense device inverse with
+    scattering self-energies: G (ldos), G^< (ndens), G^> (pdens), the bond
+    current Re(H_{l,l+1} G<_{l+1,l}) - Re(H_{l+1,l} G<_{l,l+1}) (cur), the
+    off-diagonal blocks G_{l,l+1} (GRi1) and G<_{l,l+1} (GLi1), and the two
+    contact currents.  Lead decimation at eta=1e-3 (gw_mod.f90:1401)."""
+    n, nx = 8, 5
+    dev, Hii, H1i, sr, sl = _gw_inputs(n=n, nx=nx)
+    mu, T = [0.05, -0.15], [300.0, 280.0]
+    out = oracle_mod.green_rgf_rs(Hii, H1i, E, mu, T, sigma_lesser_ph=sl, sigma_r_ph=sr)
+    W = H1i[:, :, 0]
+    H00L = Hii[:, :, 0] + sr[:, :, 0]
+    H00R = Hii[:, :, -1] + sr[:, :, -1]
+    gL, _, _, _ = oracle_mod.sancho(None, H00L, W.conj().T, E, eta=1e-3, nmax=100, reset_on_fail=True)
+    gR, _, _, _ = oracle_mod.sancho(None, H00R, W, E, eta=1e-3, nmax=100, reset_on_fail=True)
+    sigL = W @ gL @ W.conj().T
+    sigR = W.conj().T @ gR @ W
+    fL = _dense.ferm((E - mu[0]) / (BOLTZ * T[0]))   # left injection (mul, TEMPl)
+    fR = _dense.ferm((E - mu[1]) / (BOLTZ * T[0]))   # right injection (mur, TEMPl)
+    Hd, off = _dense.assemble([Hii[:, :, s] for s in range(nx)], [W] * (nx + 1))
+    N = Hd.shape[0]
+    Sig = np.zeros_like(Hd)
+    Sl = np.zeros_like(Hd)
+    for s in range(nx):
+        Sig[off[s]:off[s + 1], off[s]:off[s + 1]] += sr[:, :, s]
+        Sl[off[s]:off[s + 1], off[s]:off[s + 1]] += sl[:, :, s]
+    Sig[:n, :n] += sigL
+    Sig[-n:, -n:] += sigR
+    Sl[:n, :n] += -(sigL - sigL.conj().T) * fL
+    Sl[-n:, -n:] += -(sigR - sigR.conj().T) * fR
+    G = np.linalg.inv(E * np.eye(N) - Hd - Sig)
+    Gl = G @ Sl @ G.conj().T
+    blk = lambda M, a, b: M[off[a]:off[a + 1], off[b]:off[b + 1]]
+    scale = np.abs(G).max()
+    for s in range(nx):
+        assert np.abs(out["ldos"][:, :, s] - blk(G, s, s)).max() <= 1e-10 * scale
+        assert np.abs(out["ndens"][:, :, s] - blk(Gl, s, s)).max() <= 1e-10 * scale
+        Gg = blk(Gl, s, s) + blk(G, s, s) - blk(G, s, s).conj().T
+        assert np.abs(out["pdens"][:, :, s] - Gg).max() <= 1e-10 * scale
+    for s in range(nx - 1):
+        assert np.abs(out["GRi1"][:, :, s] - blk(G, s, s + 1)).max() <= 1e-10 * scale
+        assert np.abs(out["GLi1"][:, :, s] - blk(Gl, s, s + 1)).max() <= 1e-10 * scale
+        cur = (W.conj().T @ blk(Gl, s + 1, s)).real - (W @ blk(Gl, s, s + 1)).real
+        assert np.abs(out["cur"][:, :, s] - cur).max() <= 1e-10 * scale
+    # contact currents: right with (mur, TEMPl) and a minus sign (:1232,1245),
+    # left with (mul, TEMPr) (:1333-1341)
+    fRc = fR
+    fLc = _dense.ferm((E - mu[0]) / (BOLTZ * T[1]))
+    def spectral(sig, Gd, Gld, f):
+        om = -(sig - sig.conj().T)
+        Gg = Gld + Gd - Gd.conj().T
+        return np.trace(om * f @ Gg - om * (f - 1.0) @ Gld).real
+    tr_ref = -spectral(sigR, blk(G, nx - 1, nx - 1), blk(Gl, nx - 1, nx - 1), fRc)
+    tre_ref = spectral(sigL, blk(G, 0, 0), blk(Gl, 0, 0), fLc)
+    assert abs(out["tr"] - tr_ref) <= 1e-9 * max(1.0, abs(tr_ref))
+    assert abs(out["tre"] - tre_ref) <= 1e-9 * max(1.0, abs(tre_ref))
