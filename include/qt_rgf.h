@@ -172,6 +172,25 @@ int qt_zgemm_batched(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_t k, 
  * else info[b] = 0.  Returns QT_ERR_SINGULAR when any block failed. */
 int qt_invert_batched(int32_t n, int32_t batch, void* A, int32_t* info);
 
+/* ---- energy-axis correlation / convolution ---------------------------- */
+
+/* Batched corr1d2 (fft_mod.f90:58-91, 'fft' branch):
+ *   Z_s(i) = sum_e X_s(e) Y_s(e - l),  l = i - n/2  (0-based i; lag 0 at n/2),
+ * for nseries series pairs; series s occupies elements [s*n, s*n + n) of
+ * X, Y and Z (complex128), i.e. the energy-fastest layout of
+ * G_lesser_by_blocks(:, ik, ix, i) in gw_mod.f90:246-255.  n <= 2048.
+ * flags: QT_F_DEVICE_OUTPUTS -> X, Y, Z are device pointers, else host.
+ * The reference's result for odd n is undefined (non-conformant array
+ * assignment); here odd n keeps the same definition. */
+int qt_corr1d2_batched(int32_t n, int64_t nseries, const void* X, const void* Y, void* Z,
+                       int32_t flags);
+
+/* Batched conv1d2 (fft_mod.f90:129-168, 'fft' branch):
+ *   Z_s(e) = sum_{l=-n/2}^{n-1-n/2} X_s(e - l) Y_s(l + n/2),  X_s = 0 outside [0, n),
+ * same layout and flags as qt_corr1d2_batched. */
+int qt_conv1d2_batched(int32_t n, int64_t nseries, const void* X, const void* Y, void* Z,
+                       int32_t flags);
+
 /* ---- device memory helpers (outputs kept resident) -------------------- */
 int qt_device_alloc(size_t bytes, void** ptr);
 int qt_device_free(void* ptr);

@@ -785,12 +785,14 @@ void qto_corr1d2(int n, const qto_c* X, const qto_c* Y, qto_c* Z) {
     }
 }
 
-/* Z(e) = sum_{l=-n/2}^{n/2-1} X(e - l) Y(l + n/2), X = 0 outside [0, n).
- * fft_mod.f90:153-166: Y rotated so lag 0 is sample 0, X zero-padded. */
+/* Z(e) = sum_{l=-n/2}^{n-1-n/2} X(e - l) Y(l + n/2), X = 0 outside [0, n).
+ * fft_mod.f90:153-166: Y rotated so lag 0 is sample 0, X zero-padded.  For
+ * even n the upper lag is n/2-1 as in the reference; for odd n (where the
+ * reference assignment is non-conformant) every sample of Y is used. */
 void qto_conv1d2(int n, const qto_c* X, const qto_c* Y, qto_c* Z) {
     for (int e = 0; e < n; ++e) {
         qto_c s = 0.0;
-        for (int l = -n / 2; l <= n / 2 - 1; ++l) {
+        for (int l = -n / 2; l <= n - 1 - n / 2; ++l) {
             int q = e - l;
             if (q >= 0 && q < n) s += X[q] * Y[l + n / 2];
         }

@@ -159,6 +159,12 @@ struct SanchoUpdArgs {
 };
 void sancho_update(hipStream_t s, const SanchoUpdArgs& a);
 
+// Energy-axis correlation (mode 0, corr1d2) or convolution (mode 1, conv1d2)
+// of nseries series pairs, series s at X + s*n (qt_fft.hip).  n <= 2048.
+void corr_conv(hipStream_t s, int n, int mode, const double2* X, const double2* Y, double2* Z,
+               int64_t nseries);
+void fft_release();  // frees the twiddle tables
+
 // out[b] (real n x n) = Re(X1[b]) - Re(X2[b])
 void real_diff(hipStream_t s, int n, int batch, double* out, int64_t sout, ZOp X1, ZOp X2);
 

@@ -956,6 +956,7 @@ int qt_finish(void) {
     Ctx* c = g_ctx;
     (void)hipStreamSynchronize(c->stream);
     c->arena.release();
+    fft_release();
     if (c->timer.created)
         for (int k = 0; k < 2; ++k)
             for (int i = 0; i < 4096; ++i) (void)hipEventDestroy(c->timer.ev[k][i]);
@@ -1445,6 +1446,54 @@ int qt_invert_batched(int32_t n, int32_t batch, void* A, int32_t* info) {
     } catch (const hip_error& e) {
         return status_of(e);
     }
+}
+
+static int corr_conv_entry(int mode, int32_t n, int64_t nseries, const void* X, const void* Y,
+                           void* Z, int32_t flags) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (n <= 0 || nseries < 0 || (nseries > 0 && (!X || !Y || !Z))) return QT_ERR_INVALID;
+    if (n > 2048) return QT_ERR_UNSUPPORTED;
+    if (nseries == 0) return QT_OK;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        if (flags & QT_F_DEVICE_OUTPUTS) {
+            corr_conv(c.stream, n, mode, (const double2*)X, (const double2*)Y, (double2*)Z, nseries);
+            QT_HIP(hipStreamSynchronize(c.stream));
+            return QT_OK;
+        }
+        // host buffers: staged through the arena in chunks of at most 256 MiB
+        const int64_t per = (int64_t)n * 3 * (int64_t)sizeof(double2);
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nseries, (256ll << 20) / per));
+        c.arena.reserve((size_t)(chunk * per) + 4096);
+        c.arena.reset();
+        double2* dX = c.arena.take<double2>(chunk * n);
+        double2* dY = c.arena.take<double2>(chunk * n);
+        double2* dZ = c.arena.take<double2>(chunk * n);
+        for (int64_t s0 = 0; s0 < nseries; s0 += chunk) {
+            const int64_t cnt = std::min(chunk, nseries - s0);
+            const size_t bytes = sizeof(double2) * (size_t)(cnt * n);
+            QT_HIP(hipMemcpy(dX, (const double2*)X + s0 * n, bytes, hipMemcpyHostToDevice));
+            QT_HIP(hipMemcpy(dY, (const double2*)Y + s0 * n, bytes, hipMemcpyHostToDevice));
+            corr_conv(c.stream, n, mode, dX, dY, dZ, cnt);
+            QT_HIP(hipStreamSynchronize(c.stream));
+            QT_HIP(hipMemcpy((double2*)Z + s0 * n, dZ, bytes, hipMemcpyDeviceToHost));
+        }
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+int qt_corr1d2_batched(int32_t n, int64_t nseries, const void* X, const void* Y, void* Z,
+                       int32_t flags) {
+    return corr_conv_entry(0, n, nseries, X, Y, Z, flags);
+}
+
+int qt_conv1d2_batched(int32_t n, int64_t nseries, const void* X, const void* Y, void* Z,
+                       int32_t flags) {
+    return corr_conv_entry(1, n, nseries, X, Y, Z, flags);
 }
 
 int qt_device_alloc(size_t bytes, void** ptr) {

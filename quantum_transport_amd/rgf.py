@@ -36,7 +36,8 @@ QT_ERR_SINGULAR = 5
 EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham_destroy",
             "qt_rgf_solve", "qt_sancho", "qt_device_alloc", "qt_device_free", "qt_memcpy_h2d",
             "qt_memcpy_d2h", "qt_synchronize", "qt_set_kernel_timing", "qt_last_timing",
-            "qt_zgemm_batched", "qt_invert_batched"]
+            "qt_zgemm_batched", "qt_invert_batched", "qt_corr1d2_batched",
+            "qt_conv1d2_batched"]
 
 
 class QTError(RuntimeError):
@@ -93,6 +94,9 @@ def lib() -> C.CDLL:
         L.qt_device_free.argtypes = [C.c_void_p]
         L.qt_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.qt_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        for name in ("qt_corr1d2_batched", "qt_conv1d2_batched"):
+            getattr(L, name).argtypes = [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_int32]
         _lib = L
     return _lib
 
@@ -374,6 +378,38 @@ def invert_batched(A: np.ndarray):
     if st not in (0, QT_ERR_SINGULAR):
         check(st)
     return np.transpose(Af, (0, 2, 1)), info
+
+
+def _corr_conv(name: str, X, Y) -> np.ndarray:
+    init()
+    X = np.ascontiguousarray(X, dtype=np.complex128)
+    Y = np.ascontiguousarray(Y, dtype=np.complex128)
+    if X.shape != Y.shape or X.ndim < 1:
+        raise ValueError("X and Y must have the same shape (..., n)")
+    n = X.shape[-1]
+    Z = np.empty_like(X)
+    check(getattr(lib(), name)(n, X.size // n if n else 0, X.ctypes.data_as(C.c_void_p),
+                               Y.ctypes.data_as(C.c_void_p), Z.ctypes.data_as(C.c_void_p), 0))
+    return Z
+
+
+def corr1d2(X, Y) -> np.ndarray:
+    """Energy-axis correlation of every series pair along the last axis
+    (corr1d2, fft_mod.f90:58-91): Z[..., i] = sum_e X[..., e] Y[..., e - i + n//2]."""
+    return _corr_conv("qt_corr1d2_batched", X, Y)
+
+
+def conv1d2(X, Y) -> np.ndarray:
+    """Energy-axis convolution along the last axis (conv1d2,
+    fft_mod.f90:129-168): Z[..., e] = sum_l X[..., e - l] Y[..., l + n//2]."""
+    return _corr_conv("qt_conv1d2_batched", X, Y)
+
+
+def corr_conv_device(mode: str, n: int, nseries: int, X: "DeviceBuffer", Y: "DeviceBuffer",
+                     Z: "DeviceBuffer") -> None:
+    """corr1d2 / conv1d2 on device-resident series (mode 'corr' or 'conv')."""
+    name = {"corr": "qt_corr1d2_batched", "conv": "qt_conv1d2_batched"}[mode]
+    check(getattr(lib(), name)(n, nseries, X.ptr, Y.ptr, Z.ptr, QT_F_DEVICE_OUTPUTS))
 
 
 def green_rgf_rs(TEMP, nm: int, nx: int, E: float, mu, Hii, H1i, sigma_lesser_ph=None,

@@ -313,6 +313,21 @@ def test_corr_conv_match_fft_branch(oracle_mod, n):
     np.testing.assert_allclose(oracle_mod.conv1d2(X, Y), _fft_branch_conv(X, Y), atol=1e-12)
 
 
+@pytest.mark.parametrize("n", [1, 3, 8, 33])
+def test_corr_conv_match_numpy_convolve(oracle_mod, n):
+    """Odd lengths too (the reference's FFT branch is only conformant for
+    even n): correlation = full linear convolution with the reversed Y,
+    centred on lag 0; convolution = full linear convolution shifted by n//2."""
+    rng = np.random.default_rng(50 + n)
+    X = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    Y = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    i = np.arange(n)
+    np.testing.assert_allclose(oracle_mod.corr1d2(X, Y),
+                               np.convolve(X, Y[::-1])[n - 1 + i - n // 2], atol=1e-12)
+    np.testing.assert_allclose(oracle_mod.conv1d2(X, Y), np.convolve(Y, X)[i + n // 2],
+                               atol=1e-12)
+
+
 # --------------------------------------------------------------- green_RGF_RS
 
 
