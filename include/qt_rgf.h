@@ -191,6 +191,35 @@ int qt_corr1d2_batched(int32_t n, int64_t nseries, const void* X, const void* Y,
 int qt_conv1d2_batched(int32_t n, int64_t nseries, const void* X, const void* Y, void* Z,
                        int32_t flags);
 
+/* ---- GW polarisation / self-energy assembly --------------------------- */
+
+/* Arrays are element-major complex128 of nm*nm*nx*nk*nen samples, the
+ * by_blocks view of gw_mod.f90:246-255: series (ij, ix, ik) starts at
+ * ((ij*nx + ix)*nk + ik)*nen with ij = row + col*nm (0-based), nk = nky*nkz.
+ * Entries with |row - col| > ndiag are written as zero.  G enters as
+ * i*Im(G) (gw_mod.f90:444-446).  dEn = En(2) - En(1).  nen <= 2048.
+ * flags: QT_F_DEVICE_OUTPUTS -> every array is a device pointer. */
+
+/* Replaces the P loop of gw_ephoton_3d_ijs (gw_mod.f90:444-550):
+ *   P^<_ij(q) = dE sum_k corr(G^<_ij(k), G^>_ji(k')),
+ *   P^>_ij(q) = dE sum_k corr(G^>_ij(k), G^<_ji(k')),
+ *   P^R_ij(q) = dE sum_k [corr(G^<_ij(k), conj G^R_ij(k')) + corr(G^R_ij(k), G^<_ji(k'))],
+ * dE = -i dEn/(2 pi) spindeg/nk, k' = k - q + nk/2 folded (gw_mod.f90:478-489). */
+int qt_gw_polarization(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t nm, int32_t ndiag,
+                       double dEn, double spindeg, const void* Gr, const void* Gl, const void* Gg,
+                       void* Pr, void* Pl, void* Pg, int32_t flags);
+
+/* Replaces the Sigma loop of gw_ephoton_3d_ijs (gw_mod.f90:641-743): the
+ * new Sigma^R, Sigma^<, Sigma^> from conv1d2 sums over q of G(k') and W(q),
+ * dE = i dEn/(2 pi)/nk, Sigma^R = Re Sigma^R + i Im(Sigma^> - Sigma^<)/2,
+ * band zeroing and the (S + S^T)/2 symmetrisation. */
+int qt_gw_selfenergy(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t nm, int32_t ndiag,
+                     double dEn, const void* Gr, const void* Gl, const void* Gg, const void* Wr,
+                     const void* Wl, const void* Wg, void* Sr, void* Sl, void* Sg, int32_t flags);
+
+/* S += alpha (Snew - S) over n complex128 samples (gw_mod.f90:747-749). */
+int qt_gw_mix(int64_t n, double alpha, void* S, const void* Snew, int32_t flags);
+
 /* ---- device memory helpers (outputs kept resident) -------------------- */
 int qt_device_alloc(size_t bytes, void** ptr);
 int qt_device_free(void* ptr);

@@ -11,6 +11,9 @@ Functions mirror the reference routines they restate:
 * :func:`rgf`          rgf_mod.f90:46-269 (forward) and :272-472 (backward)
 * :func:`rgf_batch`    the E x k loop of negf_mod.f90:115-127
 * :func:`corr1d2`, :func:`conv1d2`  fft_mod.f90:58-91, 129-168
+* :func:`green_rgf_rs` gw_mod.f90:1131-1345
+* :func:`gw_polarization`, :func:`gw_selfenergy`, :func:`gw_mix`
+                       gw_mod.f90:444-519, 641-743, 747-749
 """
 from __future__ import annotations
 
@@ -254,3 +257,45 @@ def green_rgf_rs(Hii, H1i, E: float, mu, TEMP, sigma_lesser_ph=None, sigma_r_ph=
         raise ValueError(f"oracle green_rgf_rs returned {st}")
     out.update(tr=tr.value, tre=tre.value)
     return out
+
+
+def _gw_arrays(*arrs):
+    return [np.ascontiguousarray(a, dtype=np.complex128) for a in arrs]
+
+
+def gw_polarization(Gr, Gl, Gg, nky: int, nkz: int, ndiag: int, dEn: float, spindeg: float,
+                    flavour: str = "plain"):
+    """P^R, P^<, P^> (gw_mod.f90:444-519) of element-major arrays of shape
+    (nm*nm, nx, nk, nen), element ij = row + col*nm."""
+    Gr, Gl, Gg = _gw_arrays(Gr, Gl, Gg)
+    nij, nx, nk, nen = Gr.shape
+    nm = int(round(nij ** 0.5))
+    assert nm * nm == nij and nk == nky * nkz
+    P = [np.zeros_like(Gr) for _ in range(3)]
+    lib(flavour).qto_gw_polarization(
+        C.c_int(nen), C.c_int(nky), C.c_int(nkz), C.c_int(nx), C.c_int(nm), C.c_int(ndiag),
+        C.c_double(dEn), C.c_double(spindeg), *(a.ctypes.data_as(C.c_void_p) for a in (Gr, Gl, Gg)),
+        *(p.ctypes.data_as(C.c_void_p) for p in P))
+    return tuple(P)
+
+
+def gw_selfenergy(Gr, Gl, Gg, Wr, Wl, Wg, nky: int, nkz: int, ndiag: int, dEn: float,
+                  flavour: str = "plain"):
+    """new Sigma^R, Sigma^<, Sigma^> (gw_mod.f90:641-743), same layout."""
+    Gr, Gl, Gg, Wr, Wl, Wg = _gw_arrays(Gr, Gl, Gg, Wr, Wl, Wg)
+    nij, nx, nk, nen = Gr.shape
+    nm = int(round(nij ** 0.5))
+    S = [np.zeros_like(Gr) for _ in range(3)]
+    lib(flavour).qto_gw_selfenergy(
+        C.c_int(nen), C.c_int(nky), C.c_int(nkz), C.c_int(nx), C.c_int(nm), C.c_int(ndiag),
+        C.c_double(dEn), *(a.ctypes.data_as(C.c_void_p) for a in (Gr, Gl, Gg, Wr, Wl, Wg)),
+        *(s.ctypes.data_as(C.c_void_p) for s in S))
+    return tuple(S)
+
+
+def gw_mix(S, Snew, alpha: float, flavour: str = "plain"):
+    S = np.ascontiguousarray(S, dtype=np.complex128).copy()
+    Snew = np.ascontiguousarray(Snew, dtype=np.complex128)
+    lib(flavour).qto_gw_mix(C.c_int64(S.size), C.c_double(alpha), S.ctypes.data_as(C.c_void_p),
+                            Snew.ctypes.data_as(C.c_void_p))
+    return S

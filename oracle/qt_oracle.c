@@ -995,3 +995,159 @@ int qto_green_rgf_rs(const double* TEMP, int nm, int nx, double E, const double*
     free(Glp);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* GW polarisation / self-energy assembly, gw_mod.f90:444-749          */
+/* ------------------------------------------------------------------ */
+
+/* the reference's constant (static.f90:15) */
+#define QTO_GW_PI 3.14159265359
+
+/* k' paired with (k, q), gw_mod.f90:478-489 and :656-667 (1-based there;
+ * 0-based here): k'_z = k_z - q_z + nkz/2 folded into the zone. */
+static int kd_index(int ik, int iq, int nky, int nkz) {
+    int ikz = ik % nkz + 1, iky = ik / nkz + 1;
+    int iqz = iq % nkz + 1, iqy = iq / nkz + 1;
+    int ikzd = ikz - iqz + nkz / 2;
+    int ikyd = iky - iqy + nky / 2;
+    if (ikzd < 1) ikzd += nkz;
+    if (ikzd > nkz) ikzd -= nkz;
+    if (ikyd < 1) ikyd += nky;
+    if (ikyd > nky) ikyd -= nky;
+    if (nky == 1) ikyd = 1;
+    if (nkz == 1) ikzd = 1;
+    return (ikzd - 1) + (ikyd - 1) * nkz;
+}
+
+static int in_band(int row, int col, int nm, int ndiag) {
+    int l = row - ndiag < 0 ? 0 : row - ndiag;
+    int h = row + ndiag > nm - 1 ? nm - 1 : row + ndiag;
+    return col >= l && col <= h;
+}
+
+/* series (ij, ix, ik) of an element-major array */
+#define SER(A, ij, ix, ik) ((A) + ((((size_t)(ij) * nx + (ix)) * nk + (ik)) * (size_t)nen))
+
+/* i * Im(v), the preprocessing of gw_mod.f90:444-446 */
+static void imag_part(int n, const qto_c* x, qto_c* y, int conjugate) {
+    for (int e = 0; e < n; ++e) y[e] = (conjugate ? -1.0 : 1.0) * cimag(x[e]) * I;
+}
+
+void qto_gw_polarization(int nen, int nky, int nkz, int nx, int nm, int ndiag, double dEn,
+                         double spindeg, const qto_c* Gr, const qto_c* Gl, const qto_c* Gg,
+                         qto_c* Pr, qto_c* Pl, qto_c* Pg) {
+    const int nk = nky * nkz;
+    const size_t total = (size_t)nm * nm * nx * nk * nen;
+    memset(Pr, 0, sizeof(qto_c) * total);
+    memset(Pl, 0, sizeof(qto_c) * total);
+    memset(Pg, 0, sizeof(qto_c) * total);
+    qto_c *a = zalloc(nen), *b = zalloc(nen), *z = zalloc(nen);
+    /* dE, gw_mod.f90:516 */
+    const qto_c dE = (-dEn / 2.0 / QTO_GW_PI) * I * spindeg / (double)nk;
+    for (int col = 0; col < nm; ++col)
+        for (int row = 0; row < nm; ++row) {
+            if (!in_band(row, col, nm, ndiag)) continue;
+            const int ij = row + col * nm, ji = col + row * nm;
+            for (int ix = 0; ix < nx; ++ix)
+                for (int iq = 0; iq < nk; ++iq)
+                    for (int ik = 0; ik < nk; ++ik) {
+                        const int ikd = kd_index(ik, iq, nky, nkz);
+                        qto_c* pl = SER(Pl, ij, ix, iq);
+                        qto_c* pg = SER(Pg, ij, ix, iq);
+                        qto_c* pr = SER(Pr, ij, ix, iq);
+                        /* P< += corr(G<_ij(k), G>_ji(k')) */
+                        imag_part(nen, SER(Gl, ij, ix, ik), a, 0);
+                        imag_part(nen, SER(Gg, ji, ix, ikd), b, 0);
+                        qto_corr1d2(nen, a, b, z);
+                        for (int e = 0; e < nen; ++e) pl[e] += z[e];
+                        /* P> += corr(G>_ij(k), G<_ji(k')) */
+                        imag_part(nen, SER(Gg, ij, ix, ik), a, 0);
+                        imag_part(nen, SER(Gl, ji, ix, ikd), b, 0);
+                        qto_corr1d2(nen, a, b, z);
+                        for (int e = 0; e < nen; ++e) pg[e] += z[e];
+                        /* P^R += corr(G<_ij(k), conj G^R_ij(k')) + corr(G^R_ij(k), G<_ji(k')) */
+                        imag_part(nen, SER(Gl, ij, ix, ik), a, 0);
+                        imag_part(nen, SER(Gr, ij, ix, ikd), b, 1);
+                        qto_corr1d2(nen, a, b, z);
+                        for (int e = 0; e < nen; ++e) pr[e] += z[e];
+                        imag_part(nen, SER(Gr, ij, ix, ik), a, 0);
+                        imag_part(nen, SER(Gl, ji, ix, ikd), b, 0);
+                        qto_corr1d2(nen, a, b, z);
+                        for (int e = 0; e < nen; ++e) pr[e] += z[e];
+                    }
+        }
+    for (size_t t = 0; t < total; ++t) {
+        Pl[t] *= dE;
+        Pg[t] *= dE;
+        Pr[t] *= dE;
+    }
+    free(a); free(b); free(z);
+}
+
+void qto_gw_selfenergy(int nen, int nky, int nkz, int nx, int nm, int ndiag, double dEn,
+                       const qto_c* Gr, const qto_c* Gl, const qto_c* Gg, const qto_c* Wr,
+                       const qto_c* Wl, const qto_c* Wg, qto_c* Sr, qto_c* Sl, qto_c* Sg) {
+    const int nk = nky * nkz;
+    const size_t total = (size_t)nm * nm * nx * nk * nen;
+    qto_c *tr = zalloc(total), *tl = zalloc(total), *tg = zalloc(total);
+    qto_c *a = zalloc(nen), *z = zalloc(nen);
+    for (int col = 0; col < nm; ++col)
+        for (int row = 0; row < nm; ++row) {
+            if (!in_band(row, col, nm, ndiag)) continue;
+            const int ij = row + col * nm;
+            for (int ix = 0; ix < nx; ++ix)
+                for (int ik = 0; ik < nk; ++ik)
+                    for (int iq = 0; iq < nk; ++iq) {
+                        const int ikd = kd_index(ik, iq, nky, nkz);
+                        qto_c* sl = SER(tl, ij, ix, ik);
+                        qto_c* sg = SER(tg, ij, ix, ik);
+                        qto_c* sr = SER(tr, ij, ix, ik);
+                        /* gw_mod.f90:672-681 */
+                        imag_part(nen, SER(Gl, ij, ix, ikd), a, 0);
+                        qto_conv1d2(nen, a, SER(Wl, ij, ix, iq), z);
+                        for (int e = 0; e < nen; ++e) sl[e] += z[e];
+                        qto_conv1d2(nen, a, SER(Wr, ij, ix, iq), z);
+                        for (int e = 0; e < nen; ++e) sr[e] += z[e];
+                        imag_part(nen, SER(Gg, ij, ix, ikd), a, 0);
+                        qto_conv1d2(nen, a, SER(Wg, ij, ix, iq), z);
+                        for (int e = 0; e < nen; ++e) sg[e] += z[e];
+                        imag_part(nen, SER(Gr, ij, ix, ikd), a, 0);
+                        qto_conv1d2(nen, a, SER(Wl, ij, ix, iq), z);
+                        for (int e = 0; e < nen; ++e) sr[e] += z[e];
+                        qto_conv1d2(nen, a, SER(Wr, ij, ix, iq), z);
+                        for (int e = 0; e < nen; ++e) sr[e] += z[e];
+                    }
+        }
+    /* :694-698 */
+    const qto_c dE = (dEn / 2.0 / QTO_GW_PI) * I / (double)nk;
+    for (size_t t = 0; t < total; ++t) {
+        tl[t] *= dE;
+        tg[t] *= dE;
+        tr[t] *= dE;
+        tr[t] = creal(tr[t]) + I * (cimag(tg[t] - tl[t]) / 2.0);
+    }
+    /* band zeroing (:705-726) leaves out-of-band entries zero; symmetrise
+     * (:730-743): S_ij = (S_ij + S_ji) / 2 */
+    for (int col = 0; col < nm; ++col)
+        for (int row = 0; row < nm; ++row) {
+            const int ij = row + col * nm, ji = col + row * nm;
+            for (int ix = 0; ix < nx; ++ix)
+                for (int ik = 0; ik < nk; ++ik) {
+                    const qto_c *r1 = SER(tr, ij, ix, ik), *r2 = SER(tr, ji, ix, ik);
+                    const qto_c *l1 = SER(tl, ij, ix, ik), *l2 = SER(tl, ji, ix, ik);
+                    const qto_c *g1 = SER(tg, ij, ix, ik), *g2_ = SER(tg, ji, ix, ik);
+                    qto_c *ro = SER(Sr, ij, ix, ik), *lo = SER(Sl, ij, ix, ik), *go = SER(Sg, ij, ix, ik);
+                    for (int e = 0; e < nen; ++e) {
+                        ro[e] = (r1[e] + r2[e]) / 2.0;
+                        lo[e] = (l1[e] + l2[e]) / 2.0;
+                        go[e] = (g1[e] + g2_[e]) / 2.0;
+                    }
+                }
+        }
+    free(tr); free(tl); free(tg); free(a); free(z);
+}
+
+void qto_gw_mix(int64_t n, double alpha, qto_c* S, const qto_c* Snew) {
+    for (int64_t t = 0; t < n; ++t) S[t] = S[t] + alpha * (Snew[t] - S[t]);
+}
+#undef SER

@@ -16,6 +16,7 @@
 #define QT_ORACLE_H
 
 #include <complex.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -122,6 +123,23 @@ int qto_rgf_batch(int variant, int nx, const int* nm, int nk,
  * branch evaluates through a length 2n-1 DFT. */
 void qto_corr1d2(int n, const qto_c* X, const qto_c* Y, qto_c* Z);
 void qto_conv1d2(int n, const qto_c* X, const qto_c* Y, qto_c* Z);
+
+/* GW assembly on element-major arrays (the by_blocks view of
+ * gw_mod.f90:246-255): series (ij, ix, ik) of an array A starts at
+ * A + ((ij*nx + ix)*nk + ik)*nen, ij = row + col*nm (0-based), nk = nky*nkz.
+ * Only |row - col| <= ndiag is assembled; other entries are zero.  G enters
+ * as i*Im(G) (gw_mod.f90:444-446).  dEn = En(2) - En(1). */
+/* P^<, P^>, P^R, gw_mod.f90:465-519 */
+void qto_gw_polarization(int nen, int nky, int nkz, int nx, int nm, int ndiag, double dEn,
+                         double spindeg, const qto_c* Gr, const qto_c* Gl, const qto_c* Gg,
+                         qto_c* Pr, qto_c* Pl, qto_c* Pg);
+/* new Sigma^R, Sigma^<, Sigma^> including the Sigma^R imaginary-part rule,
+ * band zeroing and (S + S^T)/2, gw_mod.f90:641-743 */
+void qto_gw_selfenergy(int nen, int nky, int nkz, int nx, int nm, int ndiag, double dEn,
+                       const qto_c* Gr, const qto_c* Gl, const qto_c* Gg, const qto_c* Wr,
+                       const qto_c* Wl, const qto_c* Wg, qto_c* Sr, qto_c* Sl, qto_c* Sg);
+/* S += alpha (Snew - S), gw_mod.f90:747-749 */
+void qto_gw_mix(int64_t n, double alpha, qto_c* S, const qto_c* Snew);
 
 /* Number of threads the batch driver will use, and the BLAS flavour
  * ("plain" restatement or "mkl"). */

@@ -5,86 +5,64 @@
 // The reference zero-pads two length-n series to 2n-1 and multiplies their
 // DFTs (one MKL descriptor per call).  Here one workgroup handles one series
 // pair: both are placed in a length-L buffer (L = the power of two >= 2n-1,
-// which yields the same linear correlation / convolution), transformed by a
-// radix-2 Stockham FFT in LDS (ping-pong buffers, natural order out),
-// multiplied pointwise, inverse-transformed and the n wanted lags written.
+// which yields the same linear correlation / convolution), transformed by
+// the LDS FFT of qt_fft.h, multiplied pointwise, inverse-transformed and the
+// n wanted lags written.
 //   corr: Z[i] = sum_e X[e] Y[e - l],            l = i - n/2
 //         = (X (*) Yf)[l mod L],  Yf[(-q) mod L] = Y[q]
-//   conv: Z[e] = sum_{l=-n/2}^{n/2-1} X[e - l] Y[l + n/2]
+//   conv: Z[e] = sum_{l=-n/2}^{n-1-n/2} X[e - l] Y[l + n/2]
 //         = (Yr (*) X)[e],        Yr[l mod L] = Y[l + n/2]
-// Twiddles W_L^k = exp(-2 pi i k / L) come from a table built once per L.
+// Twiddles W_L^t = exp(-2 pi i t / L), t < L, come from a table built once per L.
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <vector>
 
+#include "qt_fft.h"
 #include "qt_internal.h"
 
 namespace qt {
 
 namespace {
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// In-place (ping-pong) Stockham radix-2 FFT of length L held in LDS buffer
-// `a`; `b` is scratch.  Returns the buffer holding the result.  sign = -1:
-// forward (twiddle W^k), +1: inverse (conj(W^k)), unnormalised.
-template <int L, int NT>
-__device__ double2* stockham(double2* a, double2* b, const double2* __restrict__ tw, int sign) {
-    for (int Ns = 1; Ns < L; Ns <<= 1) {
-        __syncthreads();
-        for (int j = threadIdx.x; j < L / 2; j += NT) {
-            const int t = j & (Ns - 1);
-            double2 w = tw[t * (L / (2 * Ns))];
-            if (sign > 0) w.y = -w.y;
-            const double2 a0 = a[j];
-            const double2 a1 = cmul(a[j + L / 2], w);
-            const int d = ((j - t) << 1) + t;  // expand(j, Ns, 2)
-            b[d] = make_double2(a0.x + a1.x, a0.y + a1.y);
-            b[d + Ns] = make_double2(a0.x - a1.x, a0.y - a1.y);
-        }
-        double2* t2 = a;
-        a = b;
-        b = t2;
-    }
-    __syncthreads();
-    return a;
-}
+using fft::pad;
 
 // mode 0: corr1d2, mode 1: conv1d2
 template <int L, int NT>
 __global__ __launch_bounds__(NT) void corrconv_kernel(int n, int mode, const double2* __restrict__ X,
                                                       const double2* __restrict__ Y, double2* __restrict__ Z,
                                                       const double2* __restrict__ tw, int64_t nseries) {
-    constexpr int PER = L / NT;
-    extern __shared__ double2 lds[];
-    double2* A = lds;
-    double2* Bf = lds + L;
+    constexpr int PER = L >= NT ? L / NT : 1;
+    extern __shared__ double2 s[];
     const int64_t sidx = blockIdx.x;
     if (sidx >= nseries) return;
     const double2* x = X + sidx * n;
     const double2* y = Y + sidx * n;
     const int h = n / 2;
+    const int tid = threadIdx.x;
     // ---- transform of the first operand: X (corr) or the rotated Y (conv)
-    for (int i = threadIdx.x; i < L; i += NT) {
+    for (int i = tid; i < L; i += NT) {
         double2 v = make_double2(0.0, 0.0);
         if (mode == 0) {
             if (i < n) v = x[i];
         } else {
-            // Yr[l mod L] = Y[l + n/2], l in [-n/2, n/2-1]
+            // Yr[l mod L] = Y[l + n/2], l in [-n/2, n-1-n/2]
             if (i < n - h) v = y[i + h];
             else if (i >= L - h) v = y[i - (L - h)];
         }
-        A[i] = v;
+        s[pad(i)] = v;
     }
-    double2* r = stockham<L, NT>(A, Bf, tw, -1);
+    __syncthreads();
+    fft::transform<L, NT, false>(s, tw);
     double2 fx[PER];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) fx[q] = r[threadIdx.x + q * NT];
+    for (int q = 0; q < PER; ++q) {
+        const int i = tid + q * NT;
+        fx[q] = i < L ? s[pad(i)] : make_double2(0.0, 0.0);
+    }
     __syncthreads();
     // ---- second operand: the flipped Y (corr) or X (conv)
-    for (int i = threadIdx.x; i < L; i += NT) {
+    for (int i = tid; i < L; i += NT) {
         double2 v = make_double2(0.0, 0.0);
         if (mode == 0) {
             // Yf[(-q) mod L] = Y[q]
@@ -93,27 +71,28 @@ __global__ __launch_bounds__(NT) void corrconv_kernel(int n, int mode, const dou
         } else {
             if (i < n) v = x[i];
         }
-        A[i] = v;
+        s[pad(i)] = v;
     }
-    r = stockham<L, NT>(A, Bf, tw, -1);
+    __syncthreads();
+    fft::transform<L, NT, false>(s, tw);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const int i = threadIdx.x + q * NT;
-        r[i] = cmul(r[i], fx[q]);
+        const int i = tid + q * NT;
+        if (i < L) s[pad(i)] = fft::cmul(s[pad(i)], fx[q]);
     }
-    double2* other = (r == A) ? Bf : A;
-    r = stockham<L, NT>(r, other, tw, +1);
+    __syncthreads();
+    fft::transform<L, NT, true>(s, tw);
     const double inv = 1.0 / (double)L;
     double2* z = Z + sidx * n;
-    for (int i = threadIdx.x; i < n; i += NT) {
+    for (int i = tid; i < n; i += NT) {
         int src;
         if (mode == 0) {
-            const int l = i - h;            // lag
+            const int l = i - h;  // lag
             src = l >= 0 ? l : l + L;
         } else {
             src = i;
         }
-        const double2 v = r[src];
+        const double2 v = s[pad(src)];
         z[i] = make_double2(v.x * inv, v.y * inv);
     }
 }
@@ -121,34 +100,39 @@ __global__ __launch_bounds__(NT) void corrconv_kernel(int n, int mode, const dou
 std::mutex tw_mu;
 std::map<int, double2*> tw_tables;
 
-const double2* twiddles(int L) {
+}  // namespace
+
+const double2* fft_twiddles(int L) {
     std::lock_guard<std::mutex> lk(tw_mu);
     auto it = tw_tables.find(L);
     if (it != tw_tables.end()) return it->second;
-    std::vector<double2> h(L / 2);
-    for (int k = 0; k < L / 2; ++k) {
+    std::vector<double2> h(L);
+    for (int k = 0; k < L; ++k) {
         const double ang = -2.0 * M_PI * (double)k / (double)L;
         h[k] = make_double2(std::cos(ang), std::sin(ang));
     }
     double2* d = nullptr;
-    QT_HIP(hipMalloc((void**)&d, sizeof(double2) * (L / 2 > 0 ? L / 2 : 1)));
-    QT_HIP(hipMemcpy(d, h.data(), sizeof(double2) * (L / 2), hipMemcpyHostToDevice));
+    QT_HIP(hipMalloc((void**)&d, sizeof(double2) * L));
+    QT_HIP(hipMemcpy(d, h.data(), sizeof(double2) * L, hipMemcpyHostToDevice));
     tw_tables[L] = d;
     return d;
 }
 
+
+namespace {
+
 template <int L>
 void launch_cc(hipStream_t s, int n, int mode, const double2* X, const double2* Y, double2* Z,
                int64_t nseries) {
-    constexpr int NT = L >= 512 ? 256 : (L >= 2 ? L / 2 : 1);
-    const size_t lds = sizeof(double2) * 2 * L;
+    constexpr int NT = fft::threads_for(L);
+    const size_t lds = sizeof(double2) * fft::padded_len(L);
     static bool attr = false;
     if (!attr) {
         QT_HIP(hipFuncSetAttribute((const void*)corrconv_kernel<L, NT>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         attr = true;
     }
-    const double2* tw = twiddles(L);
+    const double2* tw = fft_twiddles(L);
     for (int64_t s0 = 0; s0 < nseries; s0 += 0x7fffffff) {
         const int64_t cnt = std::min<int64_t>(0x7fffffff, nseries - s0);
         hipLaunchKernelGGL((corrconv_kernel<L, NT>), dim3((unsigned)cnt), dim3(NT), lds, s, n, mode,

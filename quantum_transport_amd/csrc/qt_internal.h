@@ -164,6 +164,24 @@ void sancho_update(hipStream_t s, const SanchoUpdArgs& a);
 void corr_conv(hipStream_t s, int n, int mode, const double2* X, const double2* Y, double2* Z,
                int64_t nseries);
 void fft_release();  // frees the twiddle tables
+// device table exp(-2 pi i t / L), t < L (built once per L, kept until fft_release)
+const double2* fft_twiddles(int L);
+
+// GW assembly on element-major arrays (qt_gw.hip): series (ij, ix, ik) at
+// ((ij*nx + ix)*nk + ik)*nen, ij = row + col*nm.  elems: in-band ij to
+// assemble (polarisation); pairs: in-band ij with row <= col (self-energy,
+// which writes both ij and ji).  Out-of-band outputs are left untouched.
+struct GwArgs {
+    int nen = 0, nky = 1, nkz = 1, nx = 0, nm = 0;
+    double dEn = 0.0, spindeg = 1.0;
+    const double2 *Gr = nullptr, *Gl = nullptr, *Gg = nullptr;
+    const double2 *Wr = nullptr, *Wl = nullptr, *Wg = nullptr;
+    double2 *Pr = nullptr, *Pl = nullptr, *Pg = nullptr;
+    double2 *Sr = nullptr, *Sl = nullptr, *Sg = nullptr;
+};
+void gw_polarization(hipStream_t s, const GwArgs& a, const int* elems, int nelem);
+void gw_selfenergy(hipStream_t s, const GwArgs& a, const int* pairs, int npair);
+void gw_mix(hipStream_t s, int64_t n, double alpha, double2* S, const double2* Snew);
 
 // out[b] (real n x n) = Re(X1[b]) - Re(X2[b])
 void real_diff(hipStream_t s, int n, int batch, double* out, int64_t sout, ZOp X1, ZOp X2);

@@ -1496,6 +1496,183 @@ int qt_conv1d2_batched(int32_t n, int64_t nseries, const void* X, const void* Y,
     return corr_conv_entry(1, n, nseries, X, Y, Z, flags);
 }
 
+// in-band elements |row - col| <= ndiag (all when upper_only == false, else
+// row <= col), as ij = row + col*nm
+static std::vector<int> band_elements(int nm, int ndiag, bool upper_only) {
+    std::vector<int> v;
+    for (int col = 0; col < nm; ++col)
+        for (int row = 0; row < nm; ++row) {
+            if (std::abs(row - col) > ndiag) continue;
+            if (upper_only && row > col) continue;
+            v.push_back(row + col * nm);
+        }
+    return v;
+}
+
+static bool gw_shape_ok(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t nm, int32_t ndiag) {
+    if (nen <= 0 || nky <= 0 || nkz <= 0 || nx <= 0 || nm <= 0 || ndiag < 0) return false;
+    const int64_t nij = (int64_t)nm * nm;
+    return nij <= 0x7fffffff && nij * nx <= 0x7fffffff;
+}
+
+int qt_gw_polarization(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t nm, int32_t ndiag,
+                       double dEn, double spindeg, const void* Gr, const void* Gl, const void* Gg,
+                       void* Pr, void* Pl, void* Pg, int32_t flags) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (!gw_shape_ok(nen, nky, nkz, nx, nm, ndiag) || !Gr || !Gl || !Gg || !Pr || !Pl || !Pg)
+        return QT_ERR_INVALID;
+    if (nen > 2048) return QT_ERR_UNSUPPORTED;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        const std::vector<int> el = band_elements(nm, ndiag, false);
+        const int64_t total = (int64_t)nm * nm * nx * nky * nkz * nen;
+        const size_t bytes = sizeof(double2) * (size_t)total;
+        const bool dev = (flags & QT_F_DEVICE_OUTPUTS) != 0;
+        c.arena.reserve(sizeof(int) * el.size() + (dev ? 0 : 6 * bytes) + 8192);
+        c.arena.reset();
+        int* del = c.arena.take<int>((int64_t)el.size());
+        QT_HIP(hipMemcpyAsync(del, el.data(), sizeof(int) * el.size(), hipMemcpyHostToDevice, c.stream));
+        GwArgs a;
+        a.nen = nen;
+        a.nky = nky;
+        a.nkz = nkz;
+        a.nx = nx;
+        a.nm = nm;
+        a.dEn = dEn;
+        a.spindeg = spindeg;
+        if (dev) {
+            a.Gr = (const double2*)Gr;
+            a.Gl = (const double2*)Gl;
+            a.Gg = (const double2*)Gg;
+            a.Pr = (double2*)Pr;
+            a.Pl = (double2*)Pl;
+            a.Pg = (double2*)Pg;
+        } else {
+            double2* g[3];
+            for (auto& p : g) p = c.arena.take<double2>(total);
+            a.Pr = c.arena.take<double2>(total);
+            a.Pl = c.arena.take<double2>(total);
+            a.Pg = c.arena.take<double2>(total);
+            QT_HIP(hipMemcpyAsync(g[0], Gr, bytes, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(g[1], Gl, bytes, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(g[2], Gg, bytes, hipMemcpyHostToDevice, c.stream));
+            a.Gr = g[0];
+            a.Gl = g[1];
+            a.Gg = g[2];
+        }
+        for (double2* p : {a.Pr, a.Pl, a.Pg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
+        gw_polarization(c.stream, a, del, (int)el.size());
+        QT_HIP(hipStreamSynchronize(c.stream));
+        if (!dev) {
+            QT_HIP(hipMemcpy(Pr, a.Pr, bytes, hipMemcpyDeviceToHost));
+            QT_HIP(hipMemcpy(Pl, a.Pl, bytes, hipMemcpyDeviceToHost));
+            QT_HIP(hipMemcpy(Pg, a.Pg, bytes, hipMemcpyDeviceToHost));
+        }
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+int qt_gw_selfenergy(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t nm, int32_t ndiag,
+                     double dEn, const void* Gr, const void* Gl, const void* Gg, const void* Wr,
+                     const void* Wl, const void* Wg, void* Sr, void* Sl, void* Sg, int32_t flags) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (!gw_shape_ok(nen, nky, nkz, nx, nm, ndiag) || !Gr || !Gl || !Gg || !Wr || !Wl || !Wg || !Sr ||
+        !Sl || !Sg)
+        return QT_ERR_INVALID;
+    if (nen > 2048) return QT_ERR_UNSUPPORTED;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        const std::vector<int> pr = band_elements(nm, ndiag, true);
+        const int64_t total = (int64_t)nm * nm * nx * nky * nkz * nen;
+        const size_t bytes = sizeof(double2) * (size_t)total;
+        const bool dev = (flags & QT_F_DEVICE_OUTPUTS) != 0;
+        c.arena.reserve(sizeof(int) * pr.size() + (dev ? 0 : 9 * bytes) + 8192);
+        c.arena.reset();
+        int* dpr = c.arena.take<int>((int64_t)pr.size());
+        QT_HIP(hipMemcpyAsync(dpr, pr.data(), sizeof(int) * pr.size(), hipMemcpyHostToDevice, c.stream));
+        GwArgs a;
+        a.nen = nen;
+        a.nky = nky;
+        a.nkz = nkz;
+        a.nx = nx;
+        a.nm = nm;
+        a.dEn = dEn;
+        if (dev) {
+            a.Gr = (const double2*)Gr;
+            a.Gl = (const double2*)Gl;
+            a.Gg = (const double2*)Gg;
+            a.Wr = (const double2*)Wr;
+            a.Wl = (const double2*)Wl;
+            a.Wg = (const double2*)Wg;
+            a.Sr = (double2*)Sr;
+            a.Sl = (double2*)Sl;
+            a.Sg = (double2*)Sg;
+        } else {
+            const void* src[6] = {Gr, Gl, Gg, Wr, Wl, Wg};
+            double2* in[6];
+            for (int i = 0; i < 6; ++i) {
+                in[i] = c.arena.take<double2>(total);
+                QT_HIP(hipMemcpyAsync(in[i], src[i], bytes, hipMemcpyHostToDevice, c.stream));
+            }
+            a.Gr = in[0];
+            a.Gl = in[1];
+            a.Gg = in[2];
+            a.Wr = in[3];
+            a.Wl = in[4];
+            a.Wg = in[5];
+            a.Sr = c.arena.take<double2>(total);
+            a.Sl = c.arena.take<double2>(total);
+            a.Sg = c.arena.take<double2>(total);
+        }
+        for (double2* p : {a.Sr, a.Sl, a.Sg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
+        gw_selfenergy(c.stream, a, dpr, (int)pr.size());
+        QT_HIP(hipStreamSynchronize(c.stream));
+        if (!dev) {
+            QT_HIP(hipMemcpy(Sr, a.Sr, bytes, hipMemcpyDeviceToHost));
+            QT_HIP(hipMemcpy(Sl, a.Sl, bytes, hipMemcpyDeviceToHost));
+            QT_HIP(hipMemcpy(Sg, a.Sg, bytes, hipMemcpyDeviceToHost));
+        }
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+int qt_gw_mix(int64_t n, double alpha, void* S, const void* Snew, int32_t flags) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (n < 0 || (n > 0 && (!S || !Snew))) return QT_ERR_INVALID;
+    if (n == 0) return QT_OK;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        if (flags & QT_F_DEVICE_OUTPUTS) {
+            gw_mix(c.stream, n, alpha, (double2*)S, (const double2*)Snew);
+            QT_HIP(hipStreamSynchronize(c.stream));
+            return QT_OK;
+        }
+        const size_t bytes = sizeof(double2) * (size_t)n;
+        c.arena.reserve(2 * bytes + 4096);
+        c.arena.reset();
+        double2* dS = c.arena.take<double2>(n);
+        double2* dN = c.arena.take<double2>(n);
+        QT_HIP(hipMemcpyAsync(dS, S, bytes, hipMemcpyHostToDevice, c.stream));
+        QT_HIP(hipMemcpyAsync(dN, Snew, bytes, hipMemcpyHostToDevice, c.stream));
+        gw_mix(c.stream, n, alpha, dS, dN);
+        QT_HIP(hipStreamSynchronize(c.stream));
+        QT_HIP(hipMemcpy(S, dS, bytes, hipMemcpyDeviceToHost));
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
 int qt_device_alloc(size_t bytes, void** ptr) {
     if (!g_ctx) return QT_ERR_NOT_INIT;
     try {

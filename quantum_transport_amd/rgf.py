@@ -37,7 +37,7 @@ EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham
             "qt_rgf_solve", "qt_sancho", "qt_device_alloc", "qt_device_free", "qt_memcpy_h2d",
             "qt_memcpy_d2h", "qt_synchronize", "qt_set_kernel_timing", "qt_last_timing",
             "qt_zgemm_batched", "qt_invert_batched", "qt_corr1d2_batched",
-            "qt_conv1d2_batched"]
+            "qt_conv1d2_batched", "qt_gw_polarization", "qt_gw_selfenergy", "qt_gw_mix"]
 
 
 class QTError(RuntimeError):
@@ -97,6 +97,11 @@ def lib() -> C.CDLL:
         for name in ("qt_corr1d2_batched", "qt_conv1d2_batched"):
             getattr(L, name).argtypes = [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_int32]
+        L.qt_gw_polarization.argtypes = [C.c_int32] * 6 + [C.c_double] * 2 + [C.c_void_p] * 6 \
+            + [C.c_int32]
+        L.qt_gw_selfenergy.argtypes = [C.c_int32] * 6 + [C.c_double] + [C.c_void_p] * 9 \
+            + [C.c_int32]
+        L.qt_gw_mix.argtypes = [C.c_int64, C.c_double, C.c_void_p, C.c_void_p, C.c_int32]
         _lib = L
     return _lib
 
@@ -410,6 +415,56 @@ def corr_conv_device(mode: str, n: int, nseries: int, X: "DeviceBuffer", Y: "Dev
     """corr1d2 / conv1d2 on device-resident series (mode 'corr' or 'conv')."""
     name = {"corr": "qt_corr1d2_batched", "conv": "qt_conv1d2_batched"}[mode]
     check(getattr(lib(), name)(n, nseries, X.ptr, Y.ptr, Z.ptr, QT_F_DEVICE_OUTPUTS))
+
+
+def _gw_host(arrs):
+    out = [np.ascontiguousarray(a, dtype=np.complex128) for a in arrs]
+    if any(a.shape != out[0].shape or a.ndim != 4 for a in out):
+        raise ValueError("GW arrays must share the shape (nm*nm, nx, nk, nen)")
+    nij = out[0].shape[0]
+    nm = int(round(nij ** 0.5))
+    if nm * nm != nij:
+        raise ValueError("first axis must be nm*nm")
+    return out, nm
+
+
+def gw_polarization(Gr, Gl, Gg, nky: int, nkz: int, ndiag: int, dEn: float, spindeg: float):
+    """P^R, P^<, P^> of gw_ephoton_3d_ijs (gw_mod.f90:444-550) on the device.
+    Arrays: element-major (nm*nm, nx, nky*nkz, nen), element ij = row + col*nm."""
+    init()
+    (Gr, Gl, Gg), nm = _gw_host((Gr, Gl, Gg))
+    _, nx, nk, nen = Gr.shape
+    if nk != nky * nkz:
+        raise ValueError("third axis must be nky*nkz")
+    P = [np.empty_like(Gr) for _ in range(3)]
+    check(lib().qt_gw_polarization(nen, nky, nkz, nx, nm, ndiag, dEn, spindeg,
+                                   *(a.ctypes.data for a in (Gr, Gl, Gg)),
+                                   *(p.ctypes.data for p in P), 0))
+    return tuple(P)
+
+
+def gw_selfenergy(Gr, Gl, Gg, Wr, Wl, Wg, nky: int, nkz: int, ndiag: int, dEn: float):
+    """new Sigma^R, Sigma^<, Sigma^> (gw_mod.f90:641-743) on the device."""
+    init()
+    arrs, nm = _gw_host((Gr, Gl, Gg, Wr, Wl, Wg))
+    _, nx, nk, nen = arrs[0].shape
+    if nk != nky * nkz:
+        raise ValueError("third axis must be nky*nkz")
+    S = [np.empty_like(arrs[0]) for _ in range(3)]
+    check(lib().qt_gw_selfenergy(nen, nky, nkz, nx, nm, ndiag, dEn, *(a.ctypes.data for a in arrs),
+                                 *(x.ctypes.data for x in S), 0))
+    return tuple(S)
+
+
+def gw_mix(S, Snew, alpha: float) -> np.ndarray:
+    """S + alpha (Snew - S) (gw_mod.f90:747-749) on the device."""
+    init()
+    S = np.ascontiguousarray(S, dtype=np.complex128).copy()
+    Snew = np.ascontiguousarray(Snew, dtype=np.complex128)
+    if S.shape != Snew.shape:
+        raise ValueError("shape mismatch")
+    check(lib().qt_gw_mix(S.size, alpha, S.ctypes.data, Snew.ctypes.data, 0))
+    return S
 
 
 def green_rgf_rs(TEMP, nm: int, nx: int, E: float, mu, Hii, H1i, sigma_lesser_ph=None,

@@ -404,3 +404,110 @@ def test_green_rgf_rs_dense(oracle_mod, E):
     tre_ref = spectral(sigL, blk(G, 0, 0), blk(Gl, 0, 0), fLc)
     assert abs(out["tr"] - tr_ref) <= 1e-9 * max(1.0, abs(tr_ref))
     assert abs(out["tre"] - tre_ref) <= 1e-9 * max(1.0, abs(tre_ref))
+
+
+# --------------------------------------------------------------- GW assembly
+
+def _np_corr(x, y):
+    n = len(x)
+    return np.convolve(x, y[::-1])[n - 1 + np.arange(n) - n // 2]
+
+
+def _np_conv(x, y):
+    n = len(x)
+    return np.convolve(y, x)[np.arange(n) + n // 2]
+
+
+def _gw_random(rng, nm, nx, nk, nen, count):
+    sh = (nm * nm, nx, nk, nen)
+    return [rng.standard_normal(sh) + 1j * rng.standard_normal(sh) for _ in range(count)]
+
+
+def test_gw_polarization_single_k(oracle_mod):
+    """nk = 1: P_ij = dE * correlations of i*Im(G) series (gw_mod.f90:444-519),
+    built here from numpy's convolution; |row - col| > ndiag stays zero."""
+    nm, nx, nen, ndiag = 4, 2, 10, 1
+    rng = np.random.default_rng(3)
+    Gr, Gl, Gg = _gw_random(rng, nm, nx, 1, nen, 3)
+    dEn, spin = 0.05, 2.0
+    Pr, Pl, Pg = oracle_mod.gw_polarization(Gr, Gl, Gg, 1, 1, ndiag, dEn, spin)
+    dE = -1j * dEn / (2 * 3.14159265359) * spin
+    im = lambda a: 1j * a.imag
+    for col in range(nm):
+        for row in range(nm):
+            ij, ji = row + col * nm, col + row * nm
+            for ix in range(nx):
+                if abs(row - col) > ndiag:
+                    assert not Pl[ij, ix].any() and not Pr[ij, ix].any()
+                    continue
+                gl, gg, gr = im(Gl[ij, ix, 0]), im(Gg[ij, ix, 0]), im(Gr[ij, ix, 0])
+                glt, ggt = im(Gl[ji, ix, 0]), im(Gg[ji, ix, 0])
+                np.testing.assert_allclose(Pl[ij, ix, 0], dE * _np_corr(gl, ggt), atol=1e-13)
+                np.testing.assert_allclose(Pg[ij, ix, 0], dE * _np_corr(gg, glt), atol=1e-13)
+                np.testing.assert_allclose(
+                    Pr[ij, ix, 0], dE * (_np_corr(gl, np.conj(gr)) + _np_corr(gr, glt)), atol=1e-13)
+
+
+def test_gw_polarization_momentum_pairing(oracle_mod):
+    """With G^< living only at k = a and G^> only at k = b, P^<(q) is nonzero
+    only at the transfer q that pairs (a, q) with b: in the reference's
+    1-based indices k'z = kz - qz + nkz/2 (folded), i.e. 0-based
+    q = a - b + nkz/2 - 1 mod nkz (gw_mod.f90:478-489).  The k-sum is
+    scaled by spindeg/nk."""
+    nm, nx, nen, nkz = 2, 1, 6, 4
+    rng = np.random.default_rng(4)
+    Gr = np.zeros((nm * nm, nx, nkz, nen), complex)
+    Gl = np.zeros_like(Gr)
+    Gg = np.zeros_like(Gr)
+    for a, b in ((0, 0), (1, 3), (3, 1), (2, 0)):
+        Gl[:] = 0
+        Gg[:] = 0
+        Gl[:, :, a] = 1j * rng.standard_normal((nm * nm, nx, nen))
+        Gg[:, :, b] = 1j * rng.standard_normal((nm * nm, nx, nen))
+        _, Pl, _ = oracle_mod.gw_polarization(Gr, Gl, Gg, 1, nkz, nm, 0.1, 1.0)
+        q = (a - b + nkz // 2 - 1) % nkz
+        for iq in range(nkz):
+            assert bool(np.abs(Pl[:, :, iq]).max() > 0) == (iq == q), (a, b, iq)
+        dE = -1j * 0.1 / (2 * 3.14159265359) / nkz
+        np.testing.assert_allclose(Pl[1, 0, q], dE * _np_corr(Gl[1, 0, a], Gg[2, 0, b]), atol=1e-13)
+
+
+def test_gw_selfenergy_single_k(oracle_mod):
+    """nk = 1: Sigma^<,> = dE conv(i Im G, W), Sigma^R = Re(dE (conv(G<,W^R) +
+    conv(G^R,W<) + conv(G^R,W^R))) + i Im(Sigma^> - Sigma^<)/2, then
+    (S + S^T)/2 over the element pair (gw_mod.f90:641-743)."""
+    nm, nx, nen, ndiag = 3, 2, 8, 1
+    rng = np.random.default_rng(6)
+    Gr, Gl, Gg, Wr, Wl, Wg = _gw_random(rng, nm, nx, 1, nen, 6)
+    dEn = 0.02
+    Sr, Sl, Sg = oracle_mod.gw_selfenergy(Gr, Gl, Gg, Wr, Wl, Wg, 1, 1, ndiag, dEn)
+    dE = 1j * dEn / (2 * 3.14159265359)
+    im = lambda a: 1j * a.imag
+
+    def raw(ij, ix):
+        if abs(ij % nm - ij // nm) > ndiag:
+            return np.zeros((3, nen), complex)
+        gl, gg, gr = im(Gl[ij, ix, 0]), im(Gg[ij, ix, 0]), im(Gr[ij, ix, 0])
+        sl = dE * _np_conv(gl, Wl[ij, ix, 0])
+        sg = dE * _np_conv(gg, Wg[ij, ix, 0])
+        sr = dE * (_np_conv(gl, Wr[ij, ix, 0]) + _np_conv(gr, Wl[ij, ix, 0]) +
+                   _np_conv(gr, Wr[ij, ix, 0]))
+        sr = sr.real + 1j * (sg - sl).imag / 2
+        return np.array([sr, sl, sg])
+
+    for col in range(nm):
+        for row in range(nm):
+            ij, ji = row + col * nm, col + row * nm
+            for ix in range(nx):
+                ref = (raw(ij, ix) + raw(ji, ix)) / 2
+                for got, r in zip((Sr, Sl, Sg), ref):
+                    np.testing.assert_allclose(got[ij, ix, 0], r, atol=1e-13)
+
+
+def test_gw_mix(oracle_mod):
+    rng = np.random.default_rng(1)
+    S = rng.standard_normal(20) + 1j * rng.standard_normal(20)
+    N = rng.standard_normal(20) + 1j * rng.standard_normal(20)
+    np.testing.assert_allclose(oracle_mod.gw_mix(S, N, 0.0), S)
+    np.testing.assert_allclose(oracle_mod.gw_mix(S, N, 1.0), N)
+    np.testing.assert_allclose(oracle_mod.gw_mix(S, N, 0.25), 0.75 * S + 0.25 * N)
