@@ -217,6 +217,20 @@ int qt_gw_selfenergy(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t 
                      double dEn, const void* Gr, const void* Gl, const void* Gg, const void* Wr,
                      const void* Wl, const void* Wg, void* Sr, void* Sl, void* Sg, int32_t flags);
 
+/* W solve for nw frequencies, replaces green_rgf_calc_w (called at
+ * gw_mod.f90:592-599 with Vii(:,:,:,iq), V1i(:,:,:,iq) and the P / W blocks
+ * of each energy; the reference does not define it):
+ *   W^R = V (I - P^R V)^-1,  W^< = W^R P^< W^R^H,  W^> = W^R P^> W^R^H,
+ * diagonal blocks, closed system (no boundary term).  V is Hermitian and
+ * block tridiagonal: Vii (nm x nm x nx) holds V_ii, V1i(:,:,i) = V_{i,i-1}
+ * for i >= 2 (the H1i convention of green_RGF_RS, gw_mod.f90:1194-1196).
+ * P^R, P^<, P^> are block diagonal.  PR/PL/PG/WR/WL/WG: nw consecutive
+ * (nm, nm, nx) stacks (the energy-major by_energies layout, gw_mod.f90:246).
+ * Returns QT_ERR_SINGULAR when an inverted block was singular (it is zeroed). */
+int qt_gw_wsolve_batched(int32_t nm, int32_t nx, int32_t nw, const void* Vii, const void* V1i,
+                         const void* PR, const void* PL, const void* PG, void* WR, void* WL, void* WG,
+                         int32_t flags);
+
 /* S += alpha (Snew - S) over n complex128 samples (gw_mod.f90:747-749). */
 int qt_gw_mix(int64_t n, double alpha, void* S, const void* Snew, int32_t flags);
 

@@ -299,3 +299,17 @@ def gw_mix(S, Snew, alpha: float, flavour: str = "plain"):
     lib(flavour).qto_gw_mix(C.c_int64(S.size), C.c_double(alpha), S.ctypes.data_as(C.c_void_p),
                             Snew.ctypes.data_as(C.c_void_p))
     return S
+
+
+def gw_wsolve(Vii, V1i, PR, PL, PG, flavour: str = "plain"):
+    """W^R, W^<, W^> diagonal blocks (the W solve called at gw_mod.f90:597) on
+    (nm, nm, nx) Fortran-ordered stacks; V1i[:, :, i] = V_{i,i-1}."""
+    arrs = [np.asfortranarray(a, dtype=np.complex128) for a in (Vii, V1i, PR, PL, PG)]
+    nm, _, nx = arrs[0].shape
+    W = [np.zeros((nm, nm, nx), np.complex128, order="F") for _ in range(3)]
+    st = lib(flavour).qto_gw_wsolve(C.c_int(nm), C.c_int(nx),
+                                    *(a.ctypes.data_as(C.c_void_p) for a in arrs),
+                                    *(x.ctypes.data_as(C.c_void_p) for x in W))
+    if st != 0:
+        raise ValueError("singular block in the W solve")
+    return tuple(W)

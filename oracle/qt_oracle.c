@@ -1151,3 +1151,197 @@ void qto_gw_mix(int64_t n, double alpha, qto_c* S, const qto_c* Snew) {
     for (int64_t t = 0; t < n; ++t) S[t] = S[t] + alpha * (Snew[t] - S[t]);
 }
 #undef SER
+
+/* ------------------------------------------------------------------ */
+/* W solve (called at gw_mod.f90:597, not defined in the reference)    */
+/* ------------------------------------------------------------------ */
+
+/* W^R = V (I - P^R V)^-1 and W^<,> = W^R P^<,> W^R^H for a block-tridiagonal
+ * Hermitian V (Vii(:,:,i) = V_ii, V1i(:,:,i) = V_{i,i-1} for i >= 2, the
+ * H1i convention of green_RGF_RS, gw_mod.f90:1194-1196) and block-diagonal
+ * P (diagonal blocks only, as assembled by the P loop).  With
+ * A = I - P^R V (block tridiagonal), G = A^-1 and Z = G P^< G^H:
+ *   W^R_ii = sum_j V_ij G_ji,  W^<_ii = sum_{j,k} V_ij Z_jk V_ik^H,
+ * j, k in {i-1, i, i+1}.  G and Z come from a left-connected forward sweep
+ * and a backward sweep that also yields the distance-2 blocks of Z.  The
+ * system is closed (no boundary self-energy for W). */
+typedef struct {
+    int nm, nx;
+    const qto_c *Vii, *V1i;
+    qto_c *g, *zl, *zg;     /* left-connected, per slab */
+    qto_c *Gd, *Gu, *Gl;    /* G_ii, G_{i,i+1}, G_{i+1,i} */
+    qto_c *Z[2][5];         /* Z^<,> diagonals -2..2: Z[.][2+d] slab i holds Z_{i,i+d} */
+} wsolve_t;
+
+#define WB(p, i) ((p) + (size_t)(i) * nn)
+
+/* V_ij for |i - j| <= 1, NULL otherwise; tmp receives V1i^H when needed */
+static const qto_c* vblock(const wsolve_t* w, int i, int j, qto_c* tmp) {
+    const int nm = w->nm;
+    const size_t nn = (size_t)nm * nm;
+    if (i == j) return WB(w->Vii, i);
+    if (i == j + 1) return WB(w->V1i, i);
+    if (j == i + 1) {
+        dagger(WB(w->V1i, j), nm, nm, tmp);
+        return tmp;
+    }
+    return NULL;
+}
+
+/* A_ij = delta_ij I - P^R_i V_ij into out */
+static void ablock(const wsolve_t* w, const qto_c* PR, int i, int j, qto_c* out, qto_c* tmp) {
+    const int nm = w->nm;
+    const size_t nn = (size_t)nm * nm;
+    const qto_c* v = vblock(w, i, j, tmp);
+    g2(QTO_OP_N, QTO_OP_N, nm, WB(PR, i), v, out);
+    for (size_t t = 0; t < nn; ++t) out[t] = -out[t];
+    if (i == j)
+        for (int d = 0; d < nm; ++d) out[d + (size_t)d * nm] += 1.0;
+}
+
+static void zadd(size_t n, qto_c* a, const qto_c* b, qto_c s) {
+    for (size_t t = 0; t < n; ++t) a[t] += s * b[t];
+}
+
+int qto_gw_wsolve(int nm, int nx, const qto_c* Vii, const qto_c* V1i, const qto_c* PR,
+                  const qto_c* PL, const qto_c* PG, qto_c* WR, qto_c* WL, qto_c* WG) {
+    const size_t nn = (size_t)nm * nm;
+    int status = 0;
+    wsolve_t w;
+    memset(&w, 0, sizeof w);
+    w.nm = nm;
+    w.nx = nx;
+    w.Vii = Vii;
+    w.V1i = V1i;
+    w.g = zalloc(nn * nx);
+    w.zl = zalloc(nn * nx);
+    w.zg = zalloc(nn * nx);
+    w.Gd = zalloc(nn * nx);
+    w.Gu = zalloc(nn * nx);
+    w.Gl = zalloc(nn * nx);
+    for (int s = 0; s < 2; ++s)
+        for (int d = 0; d < 5; ++d) w.Z[s][d] = zalloc(nn * nx);
+    qto_c *A1 = zalloc(nn), *A2 = zalloc(nn), *T1 = zalloc(nn), *T2 = zalloc(nn), *T3 = zalloc(nn);
+    qto_c *vt = zalloc(nn), *vt2 = zalloc(nn);
+    const qto_c* PX[2] = {PL, PG};
+    /* forward: g_i = (A_ii - A_{i,i-1} g_{i-1} A_{i-1,i})^-1,
+     *          z_i = g_i (P_i + A_{i,i-1} z_{i-1} A_{i,i-1}^H) g_i^H */
+    for (int i = 0; i < nx; ++i) {
+        ablock(&w, PR, i, i, T1, vt);
+        if (i > 0) {
+            ablock(&w, PR, i, i - 1, A1, vt);
+            ablock(&w, PR, i - 1, i, A2, vt);
+            g2(QTO_OP_N, QTO_OP_N, nm, A1, WB(w.g, i - 1), T2);
+            g2(QTO_OP_N, QTO_OP_N, nm, T2, A2, T3);
+            zadd(nn, T1, T3, -1.0);
+        }
+        if (qto_invert(T1, nm) != 0) status = 1;
+        memcpy(WB(w.g, i), T1, sizeof(qto_c) * nn);
+        for (int s = 0; s < 2; ++s) {
+            qto_c* z = s ? w.zg : w.zl;
+            memcpy(T2, WB(PX[s], i), sizeof(qto_c) * nn);
+            if (i > 0) {
+                g2(QTO_OP_N, QTO_OP_N, nm, A1, WB(z, i - 1), T3);
+                g2(QTO_OP_N, QTO_OP_C, nm, T3, A1, vt2);
+                zadd(nn, T2, vt2, 1.0);
+            }
+            g2(QTO_OP_N, QTO_OP_N, nm, WB(w.g, i), T2, T3);
+            g2(QTO_OP_N, QTO_OP_C, nm, T3, WB(w.g, i), WB(z, i));
+        }
+    }
+    /* backward */
+    memcpy(WB(w.Gd, nx - 1), WB(w.g, nx - 1), sizeof(qto_c) * nn);
+    memcpy(WB(w.Z[0][2], nx - 1), WB(w.zl, nx - 1), sizeof(qto_c) * nn);
+    memcpy(WB(w.Z[1][2], nx - 1), WB(w.zg, nx - 1), sizeof(qto_c) * nn);
+    for (int i = nx - 2; i >= 0; --i) {
+        ablock(&w, PR, i, i + 1, A1, vt);   /* A_{i,i+1} */
+        ablock(&w, PR, i + 1, i, A2, vt);   /* A_{i+1,i} */
+        const qto_c* gi = WB(w.g, i);
+        /* G_{i,i+1} = -g_i A_{i,i+1} G_{i+1,i+1} */
+        g2(QTO_OP_N, QTO_OP_N, nm, gi, A1, T1);            /* T1 = g A01 */
+        g2(QTO_OP_N, QTO_OP_N, nm, T1, WB(w.Gd, i + 1), WB(w.Gu, i));
+        zadd(nn, WB(w.Gu, i), WB(w.Gu, i), -2.0);
+        /* G_{i+1,i} = -G_{i+1,i+1} A_{i+1,i} g_i */
+        g2(QTO_OP_N, QTO_OP_N, nm, A2, gi, T2);            /* T2 = A10 g */
+        g2(QTO_OP_N, QTO_OP_N, nm, WB(w.Gd, i + 1), T2, WB(w.Gl, i));
+        zadd(nn, WB(w.Gl, i), WB(w.Gl, i), -2.0);
+        /* G_ii = g_i - G_{i,i+1} A_{i+1,i} g_i */
+        g2(QTO_OP_N, QTO_OP_N, nm, WB(w.Gu, i), T2, T3);
+        memcpy(WB(w.Gd, i), gi, sizeof(qto_c) * nn);
+        zadd(nn, WB(w.Gd, i), T3, -1.0);
+        for (int s = 0; s < 2; ++s) {
+            const qto_c* zi = WB(s ? w.zg : w.zl, i);
+            qto_c** Z = w.Z[s];
+            /* Z_{i,i+1} = -g A01 Z_{i+1,i+1} - z_i A10^H G_{i+1,i+1}^H */
+            g2(QTO_OP_N, QTO_OP_N, nm, T1, WB(Z[2], i + 1), WB(Z[3], i));
+            g2(QTO_OP_N, QTO_OP_C, nm, zi, A2, T3);        /* T3 = z A10^H */
+            g2(QTO_OP_N, QTO_OP_C, nm, T3, WB(w.Gd, i + 1), vt2);
+            zadd(nn, WB(Z[3], i), vt2, 1.0);
+            zadd(nn, WB(Z[3], i), WB(Z[3], i), -2.0);
+            /* Z_{i+1,i} = -Z_{i+1,i+1} A01^H g^H - G_{i+1,i+1} A10 z_i */
+            /* A01^H g^H = (g A01)^H = T1^H */
+            g2(QTO_OP_N, QTO_OP_C, nm, WB(Z[2], i + 1), T1, WB(Z[1], i + 1));
+            g2(QTO_OP_N, QTO_OP_N, nm, A2, zi, vt);        /* A10 z */
+            g2(QTO_OP_N, QTO_OP_N, nm, WB(w.Gd, i + 1), vt, vt2);
+            zadd(nn, WB(Z[1], i + 1), vt2, 1.0);
+            zadd(nn, WB(Z[1], i + 1), WB(Z[1], i + 1), -2.0);
+            if (i + 2 < nx) {
+                /* Z_{i,i+2} = -g A01 Z_{i+1,i+2} - z_i A10^H G_{i+2,i+1}^H */
+                g2(QTO_OP_N, QTO_OP_N, nm, T1, WB(Z[3], i + 1), WB(Z[4], i));
+                g2(QTO_OP_N, QTO_OP_C, nm, T3, WB(w.Gl, i + 1), vt2);
+                zadd(nn, WB(Z[4], i), vt2, 1.0);
+                zadd(nn, WB(Z[4], i), WB(Z[4], i), -2.0);
+                /* Z_{i+2,i} = -Z_{i+2,i+1} A01^H g^H - G_{i+2,i+1} A10 z_i */
+                g2(QTO_OP_N, QTO_OP_C, nm, WB(Z[1], i + 2), T1, WB(Z[0], i + 2));
+                g2(QTO_OP_N, QTO_OP_N, nm, WB(w.Gl, i + 1), vt, vt2);
+                zadd(nn, WB(Z[0], i + 2), vt2, 1.0);
+                zadd(nn, WB(Z[0], i + 2), WB(Z[0], i + 2), -2.0);
+            }
+            /* Z_ii = -g A01 Z_{i+1,i} + z_i M^H,  M = I + g A01 G_{i+1,i+1} A10 */
+            g2(QTO_OP_N, QTO_OP_N, nm, T1, WB(Z[1], i + 1), WB(Z[2], i));
+            zadd(nn, WB(Z[2], i), WB(Z[2], i), -2.0);
+            g2(QTO_OP_N, QTO_OP_N, nm, WB(w.Gu, i), A2, vt);   /* G_{i,i+1} A10 = -g A01 G A10 */
+            /* M = I - G_{i,i+1} A10 */
+            for (size_t t = 0; t < nn; ++t) vt[t] = -vt[t];
+            for (int d = 0; d < nm; ++d) vt[d + (size_t)d * nm] += 1.0;
+            g2(QTO_OP_N, QTO_OP_C, nm, zi, vt, vt2);
+            zadd(nn, WB(Z[2], i), vt2, 1.0);
+        }
+    }
+    /* W_ii */
+    for (int i = 0; i < nx; ++i) {
+        qto_c* wr = WB(WR, i);
+        memset(wr, 0, sizeof(qto_c) * nn);
+        for (int j = i - 1; j <= i + 1; ++j) {
+            if (j < 0 || j >= nx) continue;
+            const qto_c* v = vblock(&w, i, j, vt);
+            const qto_c* Gji = j == i ? WB(w.Gd, i) : (j == i - 1 ? WB(w.Gu, i - 1) : WB(w.Gl, i));
+            g2(QTO_OP_N, QTO_OP_N, nm, v, Gji, T1);
+            zadd(nn, wr, T1, 1.0);
+        }
+        for (int s = 0; s < 2; ++s) {
+            qto_c* wx = WB(s ? WG : WL, i);
+            memset(wx, 0, sizeof(qto_c) * nn);
+            for (int k = i - 1; k <= i + 1; ++k) {
+                if (k < 0 || k >= nx) continue;
+                memset(T2, 0, sizeof(qto_c) * nn);   /* Y_k = sum_j V_ij Z_jk */
+                for (int j = i - 1; j <= i + 1; ++j) {
+                    if (j < 0 || j >= nx) continue;
+                    const qto_c* v = vblock(&w, i, j, vt);
+                    const qto_c* zjk = WB(w.Z[s][2 + (k - j)], j);
+                    g2(QTO_OP_N, QTO_OP_N, nm, v, zjk, T1);
+                    zadd(nn, T2, T1, 1.0);
+                }
+                const qto_c* vk = vblock(&w, i, k, vt);
+                g2(QTO_OP_N, QTO_OP_C, nm, T2, vk, T1);
+                zadd(nn, wx, T1, 1.0);
+            }
+        }
+    }
+    free(w.g); free(w.zl); free(w.zg); free(w.Gd); free(w.Gu); free(w.Gl);
+    for (int s = 0; s < 2; ++s)
+        for (int d = 0; d < 5; ++d) free(w.Z[s][d]);
+    free(A1); free(A2); free(T1); free(T2); free(T3); free(vt); free(vt2);
+    return status;
+}
+#undef WB

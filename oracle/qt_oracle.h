@@ -138,6 +138,13 @@ void qto_gw_polarization(int nen, int nky, int nkz, int nx, int nm, int ndiag, d
 void qto_gw_selfenergy(int nen, int nky, int nkz, int nx, int nm, int ndiag, double dEn,
                        const qto_c* Gr, const qto_c* Gl, const qto_c* Gg, const qto_c* Wr,
                        const qto_c* Wl, const qto_c* Wg, qto_c* Sr, qto_c* Sl, qto_c* Sg);
+/* W solve for one frequency (the routine called at gw_mod.f90:597, which the
+ * reference does not define): W^R = V (I - P^R V)^-1, W^<,> = W^R P^<,> W^R^H,
+ * diagonal blocks, on (nm,nm,nx) column-major stacks; V1i(:,:,i) = V_{i,i-1}
+ * (i >= 2), V Hermitian, P block-diagonal, closed system.  Returns nonzero
+ * when a block inversion was singular. */
+int qto_gw_wsolve(int nm, int nx, const qto_c* Vii, const qto_c* V1i, const qto_c* PR,
+                  const qto_c* PL, const qto_c* PG, qto_c* WR, qto_c* WL, qto_c* WG);
 /* S += alpha (Snew - S), gw_mod.f90:747-749 */
 void qto_gw_mix(int64_t n, double alpha, qto_c* S, const qto_c* Snew);
 

@@ -1673,6 +1673,283 @@ int qt_gw_mix(int64_t n, double alpha, void* S, const void* Snew, int32_t flags)
     }
 }
 
+// ---------------------------------------------------------------- W solve
+// W^R = V (I - P^R V)^-1 and W^<,> = W^R P^<,> W^R^H (diagonal blocks) for B
+// frequencies at once.  V (Vii, V1i: nm x nm x nx, V1i[i] = V_{i,i-1}) is
+// shared; P and W are member-major (member w, slab i at (w*nx + i)*nn).
+// With A = I - P^R V, G = A^-1, Z = G P^< G^H:
+//   forward   g_i = (A_ii - A_{i,i-1} g_{i-1} A_{i-1,i})^-1,
+//             z_i = g_i (P_i + A_{i,i-1} z_{i-1} A_{i,i-1}^H) g_i^H;
+//   backward  G_{i,i+1} = -g_i A_{i,i+1} G_{i+1,i+1}, G_{i+1,i} = -G_{i+1,i+1} A_{i+1,i} g_i,
+//             G_ii = g_i - G_{i,i+1} A_{i+1,i} g_i,
+//             Z_{i,j} = -g_i A_{i,i+1} Z_{i+1,j} - z_i A_{i+1,i}^H G_{j,i+1}^H   (j > i),
+//             Z_{j,i} = -Z_{j,i+1} (g_i A_{i,i+1})^H - G_{j,i+1} A_{i+1,i} z_i   (j > i),
+//             Z_ii = -g_i A_{i,i+1} Z_{i+1,i} + z_i (I - G_{i,i+1} A_{i+1,i})^H;
+//   W^R_ii = sum_j V_ij G_ji,  W^<_ii = sum_k (sum_j V_ij Z_jk) V_ik^H.
+struct WSolve {
+    Ctx& c;
+    int nm, nx, B;
+    int64_t nn;
+    const double2 *V, *V1, *Id;
+    Work w;
+    Gemm mm;
+    // slab-major internal blocks: slab i of member b at base + (i*B + b)*nn
+    double2 *Aii, *Aup, *Alo, *g, *z[2], *Gd, *Gu, *Gl, *Z[2][5], *T[5];
+    WSolve(Ctx& c_, int nm_, int nx_, int B_, const double2* V_, const double2* V1_, const double2* Id_)
+        : c(c_), nm(nm_), nx(nx_), B(B_), nn((int64_t)nm_ * nm_), V(V_), V1(V1_), Id(Id_) {
+        mm.s = c.stream;
+        mm.t = &c.timer;
+        mm.batch = B;
+        mm.kidx = nullptr;
+    }
+    static size_t bytes(int nm, int nx, int B) {
+        const size_t nn = (size_t)nm * nm;
+        const size_t blk = sizeof(double2) * nn * B;
+        const size_t per = (blk + 256) * (size_t)(19 * nx) + (blk + 256) * 5;
+        return per + sizeof(int) * (size_t)B * (nm + 2) + sizeof(double2) * (size_t)B * 16 * nm + 8192;
+    }
+    void carve(Arena& a) {
+        auto slabs = [&](int count) { return a.take<double2>((size_t)count * B * nn); };
+        Aii = slabs(nx);
+        Aup = slabs(nx);
+        Alo = slabs(nx);
+        g = slabs(nx);
+        z[0] = slabs(nx);
+        z[1] = slabs(nx);
+        Gd = slabs(nx);
+        Gu = slabs(nx);
+        Gl = slabs(nx);
+        for (int s = 0; s < 2; ++s)
+            for (int d = 0; d < 5; ++d) Z[s][d] = slabs(nx);
+        for (auto& t : T) t = slabs(1);
+        w.piv = a.take<int>((size_t)B * nm);
+        w.info = a.take<int>((size_t)B);
+        w.infoacc = a.take<int>((size_t)B);
+        w.R = a.take<double2>((size_t)B * 16 * nm);
+    }
+    double2* at(double2* base, int i) const { return base + (int64_t)i * B * nn; }
+    ZOp mem(const double2* base, int i) const { return op_member(at(const_cast<double2*>(base), i), nm, nn); }
+    ZOp tmp(int t) const { return op_member(T[t], nm, nn); }
+    ZOp vii(int i) const { return op_member(V + i * nn, nm, 0); }
+    ZOp v1(int i) const { return op_member(V1 + i * nn, nm, 0); }
+    // C = alpha op(A) op(B) + beta D on all members
+    void g2(int opa, int opb, ZOp A, ZOp Bo, double2* C, double alpha = 1.0, ZOp D = ZOp(), double beta = 0.0) {
+        mm(opa, opb, nm, nm, nm, A, Bo, C, nm, nn, alpha, D, beta);
+    }
+    // V_ij as an operand with its op (OP_N or OP_C of a stored block)
+    void vop(int i, int j, ZOp& o, int& op) const {
+        if (i == j) {
+            o = vii(i);
+            op = OP_N;
+        } else if (i == j + 1) {
+            o = v1(i);
+            op = OP_N;
+        } else {
+            o = v1(j);
+            op = OP_C;
+        }
+    }
+    void run(const double2* PR, const double2* PX[2], double2* WR, double2* WX[2]) {
+        const int64_t ms = (int64_t)nx * nn;  // member stride of P / W
+        auto P = [&](const double2* base, int i) { return op_member(base + i * nn, nm, ms); };
+        ZOp Iop = op_member(Id, nm, 0);
+        fill_int(c.stream, w.infoacc, B, 0);
+        // A blocks: A_ii = I - P_i V_ii, A_{i,i+1} = -P_i V_{i+1,i}^H, A_{i+1,i} = -P_{i+1} V_{i+1,i}
+        for (int i = 0; i < nx; ++i) {
+            g2(OP_N, OP_N, P(PR, i), vii(i), at(Aii, i), -1.0, Iop, 1.0);
+            if (i + 1 < nx) {
+                g2(OP_N, OP_C, P(PR, i), v1(i + 1), at(Aup, i), -1.0);
+                g2(OP_N, OP_N, P(PR, i + 1), v1(i + 1), at(Alo, i), -1.0);
+            }
+        }
+        // forward sweep (Alo[i-1] = A_{i,i-1}, Aup[i-1] = A_{i-1,i})
+        for (int i = 0; i < nx; ++i) {
+            if (i == 0) {
+                copy_blocks(c.stream, nm, B, T[0], nn, mem(Aii, 0), nullptr, 0, false);
+            } else {
+                g2(OP_N, OP_N, mem(Alo, i - 1), mem(g, i - 1), T[1]);
+                g2(OP_N, OP_N, tmp(1), mem(Aup, i - 1), T[0], -1.0, mem(Aii, i), 1.0);
+            }
+            inv_batch(c, w, nm, B, T[0], nn, at(g, i), nn, nullptr);
+            note_inv_info(c, w, B, B);
+            for (int s = 0; s < 2; ++s) {
+                ZOp rhs = P(PX[s], i);
+                if (i > 0) {
+                    g2(OP_N, OP_N, mem(Alo, i - 1), mem(z[s], i - 1), T[1]);
+                    g2(OP_N, OP_C, tmp(1), mem(Alo, i - 1), T[2], 1.0, P(PX[s], i), 1.0);
+                    rhs = tmp(2);
+                }
+                g2(OP_N, OP_N, mem(g, i), rhs, T[3]);
+                g2(OP_N, OP_C, tmp(3), mem(g, i), at(z[s], i));
+            }
+        }
+        // backward sweep
+        copy_blocks(c.stream, nm, B, at(Gd, nx - 1), nn, mem(g, nx - 1), nullptr, 0, false);
+        for (int s = 0; s < 2; ++s)
+            copy_blocks(c.stream, nm, B, at(Z[s][2], nx - 1), nn, mem(z[s], nx - 1), nullptr, 0, false);
+        for (int i = nx - 2; i >= 0; --i) {
+            g2(OP_N, OP_N, mem(g, i), mem(Aup, i), T[0]);                        // T0 = g A01
+            g2(OP_N, OP_N, tmp(0), mem(Gd, i + 1), at(Gu, i), -1.0);             // G_{i,i+1}
+            g2(OP_N, OP_N, mem(Alo, i), mem(g, i), T[1]);                        // T1 = A10 g
+            g2(OP_N, OP_N, mem(Gd, i + 1), tmp(1), at(Gl, i), -1.0);             // G_{i+1,i}
+            g2(OP_N, OP_N, mem(Gu, i), tmp(1), at(Gd, i), -1.0, mem(g, i), 1.0); // G_ii
+            // M^H with M = I - G_{i,i+1} A10
+            g2(OP_N, OP_N, mem(Gu, i), mem(Alo, i), T[4], -1.0, Iop, 1.0);
+            for (int s = 0; s < 2; ++s) {
+                double2** Zs = Z[s];
+                const ZOp zi = mem(z[s], i);
+                g2(OP_N, OP_C, zi, mem(Alo, i), T[2]);                             // T2 = z A10^H
+                g2(OP_N, OP_N, mem(Alo, i), zi, T[3]);                             // T3 = A10 z
+                // Z_{i,i+1}
+                g2(OP_N, OP_C, tmp(2), mem(Gd, i + 1), at(Zs[3], i), -1.0);
+                g2(OP_N, OP_N, tmp(0), mem(Zs[2], i + 1), at(Zs[3], i), -1.0, mem(Zs[3], i), 1.0);
+                // Z_{i+1,i}
+                g2(OP_N, OP_N, mem(Gd, i + 1), tmp(3), at(Zs[1], i + 1), -1.0);
+                g2(OP_N, OP_C, mem(Zs[2], i + 1), tmp(0), at(Zs[1], i + 1), -1.0, mem(Zs[1], i + 1), 1.0);
+                if (i + 2 < nx) {
+                    // Z_{i,i+2}
+                    g2(OP_N, OP_C, tmp(2), mem(Gl, i + 1), at(Zs[4], i), -1.0);
+                    g2(OP_N, OP_N, tmp(0), mem(Zs[3], i + 1), at(Zs[4], i), -1.0, mem(Zs[4], i), 1.0);
+                    // Z_{i+2,i}
+                    g2(OP_N, OP_N, mem(Gl, i + 1), tmp(3), at(Zs[0], i + 2), -1.0);
+                    g2(OP_N, OP_C, mem(Zs[1], i + 2), tmp(0), at(Zs[0], i + 2), -1.0, mem(Zs[0], i + 2), 1.0);
+                }
+                // Z_ii
+                g2(OP_N, OP_C, zi, tmp(4), at(Zs[2], i));
+                g2(OP_N, OP_N, tmp(0), mem(Zs[1], i + 1), at(Zs[2], i), -1.0, mem(Zs[2], i), 1.0);
+            }
+        }
+        // W blocks
+        for (int i = 0; i < nx; ++i) {
+            double2* wr = WR + i * nn;
+            bool first = true;
+            for (int j = i - 1; j <= i + 1; ++j) {
+                if (j < 0 || j >= nx) continue;
+                ZOp vo;
+                int op;
+                vop(i, j, vo, op);
+                const double2* Gji = j == i ? at(Gd, i) : (j == i - 1 ? at(Gu, i - 1) : at(Gl, i));
+                mm(op, OP_N, nm, nm, nm, vo, op_member(Gji, nm, nn), wr, nm, ms, 1.0,
+                   first ? ZOp() : op_member(wr, nm, ms), 1.0);
+                first = false;
+            }
+            for (int s = 0; s < 2; ++s) {
+                double2* wx = WX[s] + i * nn;
+                bool firstk = true;
+                for (int k = i - 1; k <= i + 1; ++k) {
+                    if (k < 0 || k >= nx) continue;
+                    bool firstj = true;
+                    for (int j = i - 1; j <= i + 1; ++j) {
+                        if (j < 0 || j >= nx) continue;
+                        ZOp vo;
+                        int op;
+                        vop(i, j, vo, op);
+                        mm(op, OP_N, nm, nm, nm, vo, mem(Z[s][2 + (k - j)], j), T[0], nm, nn, 1.0,
+                           firstj ? ZOp() : tmp(0), 1.0);
+                        firstj = false;
+                    }
+                    // wx += T0 V_ik^H
+                    ZOp vo;
+                    int op;
+                    vop(i, k, vo, op);
+                    const int opd = op == OP_N ? OP_C : OP_N;  // (V_ik)^H of the stored block
+                    mm(OP_N, opd, nm, nm, nm, tmp(0), vo, wx, nm, ms, 1.0,
+                       firstk ? ZOp() : op_member(wx, nm, ms), 1.0);
+                    firstk = false;
+                }
+            }
+        }
+    }
+};
+
+int qt_gw_wsolve_batched(int32_t nm, int32_t nx, int32_t nw, const void* Vii, const void* V1i,
+                         const void* PR, const void* PL, const void* PG, void* WR, void* WL, void* WG,
+                         int32_t flags) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (nm <= 0 || nx <= 0 || nw < 0 || !Vii || (nx > 1 && !V1i)) return QT_ERR_INVALID;
+    if (nw > 0 && (!PR || !PL || !PG || !WR || !WL || !WG)) return QT_ERR_INVALID;
+    if (nm > 1024) return QT_ERR_UNSUPPORTED;
+    if (nw == 0) return QT_OK;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        const bool dev = (flags & QT_F_DEVICE_OUTPUTS) != 0;
+        const int64_t nn = (int64_t)nm * nm;
+        const size_t vbytes = sizeof(double2) * (size_t)(nn * nx);
+        // batch from free memory (each member also stages P and W when host buffers are given)
+        size_t fr = 0, tot = 0;
+        QT_HIP(hipMemGetInfo(&fr, &tot));
+        const size_t per = WSolve::bytes(nm, nx, 1) + (dev ? 0 : 6 * vbytes);
+        int B = (int)std::max<int64_t>(1, std::min<int64_t>(nw, (int64_t)(0.7 * (double)fr) / (int64_t)per));
+        B = std::min(B, 256);
+        const size_t need = WSolve::bytes(nm, nx, B) + (dev ? 0 : 6 * vbytes * (size_t)B) + 2 * vbytes +
+                            sizeof(double2) * nn + 16384;
+        c.arena.reserve(need);
+        c.arena.reset();
+        double2* dV = c.arena.take<double2>(nn * nx);
+        double2* dV1 = c.arena.take<double2>(nn * nx);
+        double2* dI = c.arena.take<double2>(nn);
+        QT_HIP(hipMemcpyAsync(dV, Vii, vbytes, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c.stream));
+        if (nx > 1)
+            QT_HIP(hipMemcpyAsync(dV1, V1i, vbytes, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                  c.stream));
+        {
+            std::vector<double2> id((size_t)nn, make_double2(0.0, 0.0));
+            for (int d = 0; d < nm; ++d) id[(size_t)d * nm + d] = make_double2(1.0, 0.0);
+            QT_HIP(hipMemcpyAsync(dI, id.data(), sizeof(double2) * nn, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipStreamSynchronize(c.stream));
+        }
+        double2* stage[6] = {};
+        if (!dev)
+            for (auto& p : stage) p = c.arena.take<double2>(nn * nx * B);
+        int st = QT_OK;
+        for (int w0 = 0; w0 < nw; w0 += B) {
+            const int cnt = std::min(B, nw - w0);
+            WSolve ws(c, nm, nx, cnt, dV, dV1, dI);
+            const size_t mark = c.arena.used;
+            ws.carve(c.arena);
+            const size_t off = (size_t)w0 * nn * nx;
+            const size_t bytes = vbytes * cnt;
+            const double2 *pr, *px[2];
+            double2 *wr, *wx[2];
+            if (dev) {
+                pr = (const double2*)PR + off;
+                px[0] = (const double2*)PL + off;
+                px[1] = (const double2*)PG + off;
+                wr = (double2*)WR + off;
+                wx[0] = (double2*)WL + off;
+                wx[1] = (double2*)WG + off;
+            } else {
+                QT_HIP(hipMemcpyAsync(stage[0], (const double2*)PR + off, bytes, hipMemcpyHostToDevice, c.stream));
+                QT_HIP(hipMemcpyAsync(stage[1], (const double2*)PL + off, bytes, hipMemcpyHostToDevice, c.stream));
+                QT_HIP(hipMemcpyAsync(stage[2], (const double2*)PG + off, bytes, hipMemcpyHostToDevice, c.stream));
+                pr = stage[0];
+                px[0] = stage[1];
+                px[1] = stage[2];
+                wr = stage[3];
+                wx[0] = stage[4];
+                wx[1] = stage[5];
+            }
+            ws.run(pr, px, wr, wx);
+            QT_HIP(hipStreamSynchronize(c.stream));
+            std::vector<int> info(cnt);
+            QT_HIP(hipMemcpy(info.data(), ws.w.infoacc, sizeof(int) * cnt, hipMemcpyDeviceToHost));
+            for (int v : info)
+                if (v) st = QT_ERR_SINGULAR;
+            if (!dev) {
+                QT_HIP(hipMemcpy((double2*)WR + off, stage[3], bytes, hipMemcpyDeviceToHost));
+                QT_HIP(hipMemcpy((double2*)WL + off, stage[4], bytes, hipMemcpyDeviceToHost));
+                QT_HIP(hipMemcpy((double2*)WG + off, stage[5], bytes, hipMemcpyDeviceToHost));
+            }
+            c.arena.used = mark;
+        }
+        return st;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
 int qt_device_alloc(size_t bytes, void** ptr) {
     if (!g_ctx) return QT_ERR_NOT_INIT;
     try {

@@ -37,7 +37,8 @@ EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham
             "qt_rgf_solve", "qt_sancho", "qt_device_alloc", "qt_device_free", "qt_memcpy_h2d",
             "qt_memcpy_d2h", "qt_synchronize", "qt_set_kernel_timing", "qt_last_timing",
             "qt_zgemm_batched", "qt_invert_batched", "qt_corr1d2_batched",
-            "qt_conv1d2_batched", "qt_gw_polarization", "qt_gw_selfenergy", "qt_gw_mix"]
+            "qt_conv1d2_batched", "qt_gw_polarization", "qt_gw_selfenergy", "qt_gw_mix",
+            "qt_gw_wsolve_batched"]
 
 
 class QTError(RuntimeError):
@@ -102,6 +103,7 @@ def lib() -> C.CDLL:
         L.qt_gw_selfenergy.argtypes = [C.c_int32] * 6 + [C.c_double] + [C.c_void_p] * 9 \
             + [C.c_int32]
         L.qt_gw_mix.argtypes = [C.c_int64, C.c_double, C.c_void_p, C.c_void_p, C.c_int32]
+        L.qt_gw_wsolve_batched.argtypes = [C.c_int32] * 3 + [C.c_void_p] * 8 + [C.c_int32]
         _lib = L
     return _lib
 
@@ -465,6 +467,26 @@ def gw_mix(S, Snew, alpha: float) -> np.ndarray:
         raise ValueError("shape mismatch")
     check(lib().qt_gw_mix(S.size, alpha, S.ctypes.data, Snew.ctypes.data, 0))
     return S
+
+
+def gw_wsolve(Vii, V1i, PR, PL, PG):
+    """W^R, W^<, W^> diagonal blocks for a batch of frequencies on the device.
+    Vii, V1i: (nm, nm, nx); PR/PL/PG: (nm, nm, nx, nw) (Fortran order, the
+    by_energies layout); returns (WR, WL, WG) of the same shape."""
+    init()
+    Vii = np.asfortranarray(Vii, dtype=np.complex128)
+    V1i = np.asfortranarray(V1i, dtype=np.complex128)
+    P = [np.asfortranarray(p, dtype=np.complex128) for p in (PR, PL, PG)]
+    nm, _, nx = Vii.shape
+    if P[0].ndim == 3:
+        P = [p[..., None] for p in P]
+    nw = P[0].shape[3]
+    W = [np.zeros_like(P[0]) for _ in range(3)]
+    st = lib().qt_gw_wsolve_batched(nm, nx, nw, Vii.ctypes.data, V1i.ctypes.data,
+                                    *(p.ctypes.data for p in P), *(x.ctypes.data for x in W), 0)
+    if st != QT_ERR_SINGULAR:
+        check(st)
+    return tuple(W)
 
 
 def green_rgf_rs(TEMP, nm: int, nx: int, E: float, mu, Hii, H1i, sigma_lesser_ph=None,

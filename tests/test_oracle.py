@@ -511,3 +511,42 @@ def test_gw_mix(oracle_mod):
     np.testing.assert_allclose(oracle_mod.gw_mix(S, N, 0.0), S)
     np.testing.assert_allclose(oracle_mod.gw_mix(S, N, 1.0), N)
     np.testing.assert_allclose(oracle_mod.gw_mix(S, N, 0.25), 0.75 * S + 0.25 * N)
+
+
+# --------------------------------------------------------------- W solve
+
+def _w_inputs(seed, nm, nx):
+    rng = np.random.default_rng(seed)
+    cr = lambda s=1.0: s * (rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm)))
+    Vii = np.stack([(lambda a: (a + a.conj().T) / 2)(cr()) for _ in range(nx)], axis=2)
+    V1i = np.stack([cr(0.3) for _ in range(nx)], axis=2)
+    PR, PL, PG = (np.stack([cr(0.2) for _ in range(nx)], axis=2) for _ in range(3))
+    return Vii, V1i, PR, PL, PG
+
+
+@pytest.mark.parametrize("nm,nx", [(1, 1), (3, 2), (4, 6), (5, 7)])
+def test_gw_wsolve_dense(oracle_mod, nm, nx):
+    """Diagonal blocks of W^R = V (I - P^R V)^-1 and W^<,> = W^R P^<,> W^R^H
+    with the full device matrices and numpy's dense inverse; also
+    W^R = (I - V P^R)^-1 V (the other order of the definition)."""
+    Vii, V1i, PR, PL, PG = _w_inputs(nm * 10 + nx, nm, nx)
+    N = nm * nx
+    V = np.zeros((N, N), complex)
+    P = {k: np.zeros((N, N), complex) for k in ("R", "L", "G")}
+    for i in range(nx):
+        sl = slice(i * nm, (i + 1) * nm)
+        V[sl, sl] = Vii[:, :, i]
+        P["R"][sl, sl], P["L"][sl, sl], P["G"][sl, sl] = PR[:, :, i], PL[:, :, i], PG[:, :, i]
+        if i > 0:
+            up = slice((i - 1) * nm, i * nm)
+            V[sl, up] = V1i[:, :, i]
+            V[up, sl] = V1i[:, :, i].conj().T
+    WR = V @ np.linalg.inv(np.eye(N) - P["R"] @ V)
+    np.testing.assert_allclose(WR, np.linalg.inv(np.eye(N) - V @ P["R"]) @ V, atol=1e-12)
+    WL = WR @ P["L"] @ WR.conj().T
+    WG = WR @ P["G"] @ WR.conj().T
+    got = oracle_mod.gw_wsolve(Vii, V1i, PR, PL, PG)
+    for g, ref in zip(got, (WR, WL, WG)):
+        for i in range(nx):
+            sl = slice(i * nm, (i + 1) * nm)
+            np.testing.assert_allclose(g[:, :, i], ref[sl, sl], atol=1e-12 * max(1, np.abs(ref).max()))
