@@ -5,12 +5,51 @@
 ! off-diagonal blocks.  The lead decimation uses z = E + i*1e-3, at most 100
 ! steps and the reset at nmax of gw_mod.f90:1400-1445 there; the right lead
 ! couples through H1i(:,:,nx).
+!
+! The SCBA stages of gw_ephoton_3d_ijs are provided as device calls on the
+! reference's buffers: gw_polarization (P loop, gw_mod.f90:444-550),
+! gw_selfenergy (Sigma loop, :641-743), gw_mix (:747-749) on the
+! element-major by_blocks arrays (nen, nk, nx, nm*nm), and green_rgf_calc_w
+! (the W solve called at :597) on the energy-major (nm, nm, nx, nw) blocks.
 module gw_mod
     use iso_c_binding
     use qt_capi
     implicit none
     private
-    public :: green_RGF_RS
+    public :: green_RGF_RS, green_rgf_calc_w, gw_polarization, gw_selfenergy, gw_mix
+
+    interface
+        integer(c_int) function qt_gw_polarization(nen, nky, nkz, nx, nm, ndiag, dEn, spindeg, Gr, Gl, Gg, &
+                                                   Pr, Pl, Pg, flags) bind(C, name="qt_gw_polarization")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            integer(c_int32_t), value :: nen, nky, nkz, nx, nm, ndiag, flags
+            real(c_double), value :: dEn, spindeg
+            type(c_ptr), value :: Gr, Gl, Gg, Pr, Pl, Pg
+        end function qt_gw_polarization
+
+        integer(c_int) function qt_gw_selfenergy(nen, nky, nkz, nx, nm, ndiag, dEn, Gr, Gl, Gg, Wr, Wl, Wg, &
+                                                 Sr, Sl, Sg, flags) bind(C, name="qt_gw_selfenergy")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            integer(c_int32_t), value :: nen, nky, nkz, nx, nm, ndiag, flags
+            real(c_double), value :: dEn
+            type(c_ptr), value :: Gr, Gl, Gg, Wr, Wl, Wg, Sr, Sl, Sg
+        end function qt_gw_selfenergy
+
+        integer(c_int) function qt_gw_mix(n, alpha, S, Snew, flags) bind(C, name="qt_gw_mix")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            integer(c_int64_t), value :: n
+            real(c_double), value :: alpha
+            type(c_ptr), value :: S, Snew
+            integer(c_int32_t), value :: flags
+        end function qt_gw_mix
+
+        integer(c_int) function qt_gw_wsolve_batched(nm, nx, nw, Vii, V1i, PR, PL, PG, WR, WL, WG, flags) &
+            bind(C, name="qt_gw_wsolve_batched")
+            import :: c_int, c_int32_t, c_ptr
+            integer(c_int32_t), value :: nm, nx, nw, flags
+            type(c_ptr), value :: Vii, V1i, PR, PL, PG, WR, WL, WG
+        end function qt_gw_wsolve_batched
+    end interface
 
 contains
 
@@ -89,5 +128,79 @@ contains
         tr = tr1(1)
         tre = tre1(1)
     end subroutine green_RGF_RS
+
+    ! P^<, P^>, P^R of every in-band element (|row - col| <= ndiag) from
+    ! G^R, G^<, G^> (i*Im G is used), k-sums over nky*nkz points; dEn = En(2)-En(1)
+    subroutine gw_polarization(nen, nky, nkz, nx, nm, ndiag, dEn, spindeg, G_r, G_lesser, G_greater, &
+                               P_retarded, P_lesser, P_greater)
+        integer, intent(in) :: nen, nky, nkz, nx, nm, ndiag
+        real(8), intent(in) :: dEn, spindeg
+        complex(8), intent(in), dimension(nen, nky*nkz, nx, nm*nm), target :: G_r, G_lesser, G_greater
+        complex(8), intent(out), dimension(nen, nky*nkz, nx, nm*nm), target :: P_retarded, P_lesser, P_greater
+        call qt_ensure_init()
+        call qt_check(qt_gw_polarization(int(nen, c_int32_t), int(nky, c_int32_t), int(nkz, c_int32_t), &
+                                         int(nx, c_int32_t), int(nm, c_int32_t), int(ndiag, c_int32_t), &
+                                         real(dEn, c_double), real(spindeg, c_double), c_loc(G_r), &
+                                         c_loc(G_lesser), c_loc(G_greater), c_loc(P_retarded), &
+                                         c_loc(P_lesser), c_loc(P_greater), 0_c_int32_t), 'gw_polarization')
+    end subroutine gw_polarization
+
+    ! new Sigma^R, Sigma^<, Sigma^> (band zeroing and (S + S^T)/2 included)
+    subroutine gw_selfenergy(nen, nky, nkz, nx, nm, ndiag, dEn, G_r, G_lesser, G_greater, W_retarded, &
+                             W_lesser, W_greater, Sigma_r_new, Sigma_lesser_new, Sigma_greater_new)
+        integer, intent(in) :: nen, nky, nkz, nx, nm, ndiag
+        real(8), intent(in) :: dEn
+        complex(8), intent(in), dimension(nen, nky*nkz, nx, nm*nm), target :: G_r, G_lesser, G_greater
+        complex(8), intent(in), dimension(nen, nky*nkz, nx, nm*nm), target :: W_retarded, W_lesser, W_greater
+        complex(8), intent(out), dimension(nen, nky*nkz, nx, nm*nm), target :: Sigma_r_new, Sigma_lesser_new, &
+                                                                              Sigma_greater_new
+        call qt_ensure_init()
+        call qt_check(qt_gw_selfenergy(int(nen, c_int32_t), int(nky, c_int32_t), int(nkz, c_int32_t), &
+                                       int(nx, c_int32_t), int(nm, c_int32_t), int(ndiag, c_int32_t), &
+                                       real(dEn, c_double), c_loc(G_r), c_loc(G_lesser), c_loc(G_greater), &
+                                       c_loc(W_retarded), c_loc(W_lesser), c_loc(W_greater), &
+                                       c_loc(Sigma_r_new), c_loc(Sigma_lesser_new), c_loc(Sigma_greater_new), &
+                                       0_c_int32_t), 'gw_selfenergy')
+    end subroutine gw_selfenergy
+
+    ! Sigma = Sigma + alpha_mix (Sigma_new - Sigma)
+    subroutine gw_mix(n, alpha_mix, Sigma, Sigma_new)
+        integer(8), intent(in) :: n
+        real(8), intent(in) :: alpha_mix
+        complex(8), intent(inout), target :: Sigma(n)
+        complex(8), intent(in), target :: Sigma_new(n)
+        call qt_ensure_init()
+        call qt_check(qt_gw_mix(int(n, c_int64_t), real(alpha_mix, c_double), c_loc(Sigma), c_loc(Sigma_new), &
+                                0_c_int32_t), 'gw_mix')
+    end subroutine gw_mix
+
+    ! W^R = V (I - P^R V)^-1, W^<,> = W^R P^<,> W^R^H for nw frequencies in one
+    ! call (the reference calls it per energy, gw_mod.f90:592-599); nbc = 0
+    ! (closed system) is the only boundary treatment provided
+    subroutine green_rgf_calc_w(nbc, nm, nx, Vii, V1i, p_lesser, p_greater, p_retarded, &
+                                w_lesser, w_greater, w_retarded, nw)
+        integer, intent(in) :: nbc, nm, nx
+        integer, intent(in), optional :: nw
+        complex(8), intent(in), dimension(nm, nm, nx), target :: Vii, V1i
+        complex(8), intent(in), dimension(nm, nm, nx, *), target :: p_lesser, p_greater, p_retarded
+        complex(8), intent(inout), dimension(nm, nm, nx, *), target :: w_lesser, w_greater, w_retarded
+        integer(c_int) :: rc
+        integer :: n
+        if (nbc /= 0) then
+            write (*, *) 'green_rgf_calc_w: only nbc = 0 (closed system) is provided'
+            error stop 1
+        end if
+        n = 1
+        if (present(nw)) n = nw
+        call qt_ensure_init()
+        rc = qt_gw_wsolve_batched(int(nm, c_int32_t), int(nx, c_int32_t), int(n, c_int32_t), c_loc(Vii), &
+                                  c_loc(V1i), c_loc(p_retarded), c_loc(p_lesser), c_loc(p_greater), &
+                                  c_loc(w_retarded), c_loc(w_lesser), c_loc(w_greater), 0_c_int32_t)
+        if (rc == QT_ERR_SINGULAR) then
+            write (*, *) 'green_rgf_calc_w: singular block (zeroed)'
+        else
+            call qt_check(rc, 'green_rgf_calc_w')
+        end if
+    end subroutine green_rgf_calc_w
 
 end module gw_mod

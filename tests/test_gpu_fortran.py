@@ -84,3 +84,38 @@ def test_fortran_fft_mod_batched(oracle_mod):
         for s in range(ns):
             r = ref(X[:, s], Y[:, s])
             assert np.abs(Z[:, s] - r).max() <= 1e-12 * np.abs(r).max(), (sym, s)
+
+
+@pytest.mark.gpu
+def test_fortran_gw_stages(oracle_mod):
+    """gw_mod's gw_polarization and green_rgf_calc_w through the Fortran
+    module ABI match the oracle."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "quantum_transport_amd", "fortran", "_build",
+                              "libqtfortran.so"))
+    rng = np.random.default_rng(9)
+    cr = lambda *sh, s=1.0: s * (rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+    # P loop on (nen, nk, nx, nm*nm) Fortran arrays = C-order (nm*nm, nx, nk, nen)
+    nm, nx, nen, ndiag = 3, 2, 20, 1
+    G = [np.asfortranarray(cr(nen, 1, nx, nm * nm)) for _ in range(3)]
+    P = [np.zeros_like(G[0], order="F") for _ in range(3)]
+    ints = [C.byref(C.c_int(v)) for v in (nen, 1, 1, nx, nm, ndiag)]
+    lib._QMgw_modPgw_polarization(*ints, C.byref(C.c_double(0.02)), C.byref(C.c_double(2.0)),
+                                  *(a.ctypes.data_as(C.c_void_p) for a in G + P))
+    ref = oracle_mod.gw_polarization(*(np.ascontiguousarray(g.T) for g in G), 1, 1, ndiag, 0.02, 2.0)
+    for got, r in zip(P, ref):
+        assert np.abs(got.T - r).max() <= 1e-12 * np.abs(r).max()
+    # W solve for 3 frequencies
+    nm, nx, nw = 4, 5, 3
+    Vii = cr(nm, nm, nx)
+    Vii = np.asfortranarray((Vii + Vii.conj().transpose(1, 0, 2)) / 2)
+    V1i = np.asfortranarray(cr(nm, nm, nx, s=0.3))
+    PL, PG, PR = (np.asfortranarray(cr(nm, nm, nx, nw, s=0.1)) for _ in range(3))
+    WL, WG, WR = (np.zeros((nm, nm, nx, nw), complex, order="F") for _ in range(3))
+    lib._QMgw_modPgreen_rgf_calc_w(C.byref(C.c_int(0)), C.byref(C.c_int(nm)), C.byref(C.c_int(nx)),
+                                   *(a.ctypes.data_as(C.c_void_p) for a in (Vii, V1i, PL, PG, PR, WL, WG, WR)),
+                                   C.byref(C.c_int(nw)))
+    for w in range(nw):
+        ref = oracle_mod.gw_wsolve(Vii, V1i, PR[..., w], PL[..., w], PG[..., w])
+        for got, r in zip((WR[..., w], WL[..., w], WG[..., w]), ref):
+            assert np.abs(got - r).max() <= 1e-10 * np.abs(r).max()
