@@ -67,6 +67,79 @@ def cpu_baseline(nb: int, slabs: int, threads: int, seed: int) -> dict:
     }
 
 
+def run_gw(args) -> dict:
+    """GW-SCBA energy-axis stage (P and Sigma assembly, gw_mod.f90:444-743) at
+    the GW shape: nm=256 orbitals per slab, every element of the diagonal
+    blocks, 2048 energies, one kz, on --gw-slabs slabs (the per-slab work of
+    the 50-slab device).  Arrays are device resident (torch allocations
+    handed to the library as device pointers)."""
+    import torch
+    from quantum_transport_amd import rgf
+
+    nm, nx, nen = 256, args.gw_slabs, 2048
+    rgf.init(0)
+    dev = torch.device("cuda", 0)
+    total = nm * nm * nx * nen
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+
+    def arr():
+        return torch.randn(total, dtype=torch.complex128, device=dev, generator=g)
+
+    G = [arr() for _ in range(3)]
+    Wt = [arr() for _ in range(3)]
+    out = [torch.empty(total, dtype=torch.complex128, device=dev) for _ in range(3)]
+    torch.cuda.synchronize()
+    L = rgf.lib()
+    ptr = lambda t: t.data_ptr()
+    dEn = 8.0 / (nen - 1)
+
+    def pol():
+        rgf.check(L.qt_gw_polarization(nen, 1, 1, nx, nm, nm, dEn, 2.0, *(ptr(t) for t in G),
+                                       *(ptr(t) for t in out), rgf.QT_F_DEVICE_OUTPUTS))
+
+    def sig():
+        rgf.check(L.qt_gw_selfenergy(nen, 1, 1, nx, nm, nm, dEn, *(ptr(t) for t in G),
+                                     *(ptr(t) for t in Wt), *(ptr(t) for t in out),
+                                     rgf.QT_F_DEVICE_OUTPUTS))
+
+    times = {}
+    for name, fn in (("polarization", pol), ("selfenergy", sig)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        times[name] = (time.perf_counter() - t0) / args.steps
+    series = nm * nm * nx
+    # algorithmic HBM bytes: P reads 5 series and writes 3 per element and slab,
+    # Sigma reads 6 and writes 3 (16 B per complex sample)
+    bytes_p = series * nen * 16 * 8
+    bytes_s = series * nen * 16 * 9
+    bw_p = bytes_p / times["polarization"] / 1e9
+    bw_s = bytes_s / times["selfenergy"] / 1e9
+    return {
+        "metric": "GW P + Sigma assembly, element-slabs/s @nm=256, 2048 E",
+        "value": round(series / (times["polarization"] + times["selfenergy"]), 1),
+        "unit": "element-slabs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * (times["polarization"] + times["selfenergy"]), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (torch normal samples)",
+        "config": {"workload": f"GW P and Sigma assembly, nm=256 (all elements), {nx} slabs, "
+                               f"2048 energies, 1 kz", "nm": nm, "slabs": nx, "energies": nen},
+        "roofline": {"bound": "hbm", "achieved": round(bw_p, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(bw_p / 8000.0, 4), "traffic": None,
+                     "kernel": "gw_pol_kernel (P assembly)"},
+        "stages": {"polarization_ms": round(1e3 * times["polarization"], 2),
+                   "selfenergy_ms": round(1e3 * times["selfenergy"], 2),
+                   "selfenergy_GBps": round(bw_s, 1)},
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,12 +151,19 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1002)
+    ap.add_argument("--workload", choices=("rgf", "gw"), default="rgf",
+                    help="rgf: the headline RGF sweep; gw: the GW energy-axis stage")
+    ap.add_argument("--gw-slabs", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the RGF path is measured per GPU "
               "(energy points shard across GPUs with no communication but a final sum); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
+
+    if args.workload == "gw":
+        print(json.dumps(run_gw(args)), flush=True)
+        return 0
 
     from quantum_transport_amd import hamiltonian as hm
     from quantum_transport_amd import rgf
