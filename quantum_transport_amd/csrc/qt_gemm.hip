@@ -16,6 +16,9 @@
 // Work mapping: the 1-D grid is ordered so that all tiles of one batch
 // member are dispatched to the same XCD (workgroups are dealt round-robin
 // over the 8 XCDs), keeping that member's operands in one L2.
+#include <cstdlib>
+#include <string>
+
 #include "qt_internal.h"
 
 namespace qt {
@@ -194,8 +197,206 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int til
     }
 }
 
+// Wider variant: 128x64 block tile, K staged 8 at a time (2 x 24 KB LDS),
+// each wave a 64x32 sub-tile (4x2 MFMA tiles): twice the MFMAs per LDS read
+// of the 64x64 kernel and fewer global loads per flop.
+constexpr int WBM = 128, WBN = 64, WBK = 8;
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, int tiles_m, int tiles) {
+    const int L = blockIdx.x;
+    const int lane8 = L % NXCD;
+    const int rest = L / NXCD;
+    const int tile = rest % tiles;
+    const int b = lane8 + NXCD * (rest / tiles);
+    if (b >= a.batch) return;
+    if (a.active && !a.active[b]) return;
+
+    const int kz = a.kidx ? a.kidx[b >> a.kshift] : 0;
+    const double2* __restrict__ Ap = a.A.p + b * a.A.sb + kz * a.A.sk;
+    const double2* __restrict__ Bp = a.B.p + b * a.B.sb + kz * a.B.sk;
+    const int64_t lda = a.A.ld, ldb = a.B.ld;
+    const int M = a.M, N = a.N, K = a.K;
+    const int m0 = (tile % tiles_m) * WBM;
+    const int n0 = (tile / tiles_m) * WBN;
+
+    __shared__ double2 As[2][WBK][WBM];
+    __shared__ double2 Bs[2][WBK][WBN];
+
+    const int tid = threadIdx.x;
+    double2 ra[4], rb[2];
+
+    auto loadA = [&](int k0) {
+        if (OPA == OP_N) {
+            const int kk = tid >> 5, ii = (tid & 31) * 4;
+            const int gk = k0 + kk;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gi = m0 + ii + q;
+                ra[q] = (gi < M && gk < K) ? Ap[gi + gk * lda] : make_double2(0.0, 0.0);
+            }
+        } else {
+            const int ii = tid >> 1, kk = (tid & 1) * 4;
+            const int gi = m0 + ii;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gk = k0 + kk + q;
+                ra[q] = (gi < M && gk < K) ? Ap[gk + gi * lda] : make_double2(0.0, 0.0);
+            }
+        }
+    };
+    auto storeA = [&](int buf) {
+        if (OPA == OP_N) {
+            const int kk = tid >> 5, ii = (tid & 31) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) As[buf][kk][ii + q] = ra[q];
+        } else {
+            const int ii = tid >> 1, kk = (tid & 1) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) As[buf][kk + q][ii] = OPA == OP_C ? cconj(ra[q]) : ra[q];
+        }
+    };
+    auto loadB = [&](int k0) {
+        if (OPB == OP_N) {
+            const int jj = tid >> 2, kk = (tid & 3) * 2;
+            const int gj = n0 + jj;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int gk = k0 + kk + q;
+                rb[q] = (gj < N && gk < K) ? Bp[gk + gj * ldb] : make_double2(0.0, 0.0);
+            }
+        } else {
+            const int kk = tid >> 5, jj = (tid & 31) * 2;
+            const int gk = k0 + kk;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int gj = n0 + jj + q;
+                rb[q] = (gj < N && gk < K) ? Bp[gj + gk * ldb] : make_double2(0.0, 0.0);
+            }
+        }
+    };
+    auto storeB = [&](int buf) {
+        if (OPB == OP_N) {
+            const int jj = tid >> 2, kk = (tid & 3) * 2;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) Bs[buf][kk + q][jj] = rb[q];
+        } else {
+            const int kk = tid >> 5, jj = (tid & 31) * 2;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) Bs[buf][kk][jj + q] = OPB == OP_C ? cconj(rb[q]) : rb[q];
+        }
+    };
+
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    double4_t accR[4][2], accI[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            accR[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+            accI[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+        }
+
+    const int nk = (K + WBK - 1) / WBK;
+    loadA(0);
+    loadB(0);
+    storeA(0);
+    storeB(0);
+    __syncthreads();
+
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) {
+            loadA((kt + 1) * WBK);
+            loadB((kt + 1) * WBK);
+        }
+#pragma unroll
+        for (int kq = 0; kq < WBK / 4; ++kq) {
+            const int k = kq * 4 + lk;
+            double2 av[4], bv[2];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = As[buf][k][wm * 64 + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bv[j] = Bs[buf][k][wn * 32 + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
+                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].x, accI[i][j], 0, 0, 0);
+                }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[j].y, av[i].y, accR[i][j], 0, 0, 0);
+                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].y, accI[i][j], 0, 0, 0);
+                }
+        }
+        if (kt + 1 < nk) {
+            storeA(buf ^ 1);
+            storeB(buf ^ 1);
+        }
+        __syncthreads();
+    }
+
+    double2* __restrict__ Cp = a.C + b * a.sbc + kz * a.skc;
+    const double2* __restrict__ Dp = a.D.p ? a.D.p + b * a.D.sb + kz * a.D.sk : nullptr;
+    const double alpha = a.alpha, beta = a.beta;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int gi = m0 + wm * 64 + i * 16 + li;
+        if (gi >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gj = n0 + wn * 32 + j * 16 + 4 * q + lk;
+                if (gj >= N || (gj >= a.skip_lo && gj < a.skip_hi)) continue;
+                double re = alpha * accR[i][j][q], im = alpha * accI[i][j][q];
+                if (Dp) {
+                    const double2 d = Dp[gi + gj * a.D.ld];
+                    re += beta * d.x;
+                    im += beta * d.y;
+                }
+                Cp[gi + gj * a.ldc] = make_double2(re, im);
+            }
+        }
+    }
+}
+
+template <int OPA, int OPB>
+void launch_wide(hipStream_t s, const ZGemmArgs& a) {
+    const int tiles_m = (a.M + WBM - 1) / WBM;
+    const int tiles_n = (a.N + WBN - 1) / WBN;
+    const int tiles = tiles_m * tiles_n;
+    const int batch_pad = (a.batch + NXCD - 1) / NXCD * NXCD;
+    const int64_t blocks = (int64_t)tiles * batch_pad;
+    if (blocks > 0x7fffffff) throw hip_error("zgemm: grid too large", 1);
+    hipLaunchKernelGGL((zgemm_mfma_wide_kernel<OPA, OPB>), dim3((unsigned)blocks), dim3(256), 0, s, a,
+                       tiles_m, tiles);
+    QT_HIP(hipGetLastError());
+}
+
+// tile variant: 0 = 64x64 (default), 1 = 128x64; QT_GEMM_TILE=128x64 selects the
+// wide kernel for problems with at least 128 rows
+int tile_variant() {
+    static const int v = [] {
+        const char* e = getenv("QT_GEMM_TILE");
+        return (e && std::string(e) == "128x64") ? 1 : 0;
+    }();
+    return v;
+}
+
 template <int OPA, int OPB>
 void launch(hipStream_t s, const ZGemmArgs& a) {
+    if (tile_variant() == 1 && a.M >= WBM) {
+        launch_wide<OPA, OPB>(s, a);
+        return;
+    }
     const int tiles_m = (a.M + BM - 1) / BM;
     const int tiles_n = (a.N + BN - 1) / BN;
     const int tiles = tiles_m * tiles_n;
