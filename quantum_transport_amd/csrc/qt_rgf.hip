@@ -721,11 +721,20 @@ struct Solve {
               op_member(T[1], n, nn), 1.0);                              // - U Gl U^H
             inv_batch(c, w, n, B, T[1], nn, gl(sc), sq(sc), nullptr);
             note_inv_info(c, w, B, B);
-            g(OP_N, OP_N, n, np, np, U, opGln(sp), T[2], n, nn);         // U Gln
-            g(OP_N, OP_C, n, n, np, op_member(T[2], n, nn), U, T[3], n, nn, 1.0, opSLS(sc),
-              1.0);                                                      // U Gln U^H + sl S
-            g(OP_N, OP_N, n, n, n, opGl(sc), op_member(T[3], n, nn), T[4], n, nn);
-            g(OP_N, OP_C, n, n, n, op_member(T[4], n, nn), opGl(sc), gln(sc), n, sq(sc));
+            if (sigma) {
+                g(OP_N, OP_N, n, np, np, U, opGln(sp), T[2], n, nn);     // U Gln
+                g(OP_N, OP_C, n, n, np, op_member(T[2], n, nn), U, T[3], n, nn, 1.0, opSLS(sc),
+                  1.0);                                                  // U Gln U^H + sl S
+                g(OP_N, OP_N, n, n, n, opGl(sc), op_member(T[3], n, nn), T[4], n, nn);
+                g(OP_N, OP_C, n, n, n, op_member(T[4], n, nn), opGl(sc), gln(sc), n, sq(sc));
+            } else {
+                // no interior injection: Gln = Gl U Gln' U^H Gl^H = V Gln' V^H with
+                // V = Gl U, three products instead of four
+                g(OP_N, OP_N, n, np, n, opGl(sc), U, T[2], n, nn);       // V = Gl U
+                g(OP_N, OP_N, n, np, np, op_member(T[2], n, nn), opGln(sp), T[3], n, nn);
+                g(OP_N, OP_C, n, n, np, op_member(T[3], n, nn), op_member(T[2], n, nn), gln(sc), n,
+                  sq(sc));                                               // (V Gln') V^H
+            }
             sp = sc;
         }
         // ---------------- last slab of pass 1 (rgf_mod.f90:172-202 / 385-415)

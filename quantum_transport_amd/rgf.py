@@ -362,7 +362,8 @@ def zgemm_batched(A: np.ndarray, B: np.ndarray, opa: int = 0, opb: int = 0, alph
     if R is None:
         Rf = np.zeros((batch, n, m), np.complex128)
     else:
-        Rf = np.ascontiguousarray(np.transpose(np.asarray(R, np.complex128), (0, 2, 1)))
+        # always a copy: the library writes the result into this buffer
+        Rf = np.array(np.transpose(np.asarray(R, np.complex128), (0, 2, 1)), order="C", copy=True)
     check(lib().qt_zgemm_batched(C.c_int32(opa), C.c_int32(opb), C.c_int32(m), C.c_int32(n),
                                  C.c_int32(k), C.c_int32(batch), C.c_double(alpha),
                                  Af.ctypes.data_as(C.c_void_p), C.c_int64(ar), C.c_int64(ar * ac),
@@ -378,7 +379,8 @@ def invert_batched(A: np.ndarray):
     init()
     A = np.asarray(A, dtype=np.complex128)
     batch, n = A.shape[0], A.shape[1]
-    Af = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))
+    # always a copy: the inverse is written in place
+    Af = np.array(np.transpose(A, (0, 2, 1)), order="C", copy=True)
     info = np.zeros(batch, np.int32)
     st = lib().qt_invert_batched(C.c_int32(n), C.c_int32(batch), Af.ctypes.data_as(C.c_void_p),
                                  info.ctypes.data_as(C.c_void_p))

@@ -34,6 +34,10 @@ def test_zgemm_vs_numpy(qt, opa, opb, mnk):
     got = qt.zgemm_batched(A, B, opa, opb, alpha=-1.0, beta=1.0, R=R0)
     ref = np.stack([R0[b] - OPS[opa](A[b]) @ OPS[opb](B[b]) for b in range(batch)])
     err = np.abs(got - ref).max() / np.abs(ref).max()
+    if not err < 1e-13:
+        for b in range(min(batch, 3)):
+            print("member", b, "got", got[b].ravel()[:4], "ref", ref[b].ravel()[:4],
+                  "R0", R0[b].ravel()[:4])
     assert err < 1e-13, err
 
 
