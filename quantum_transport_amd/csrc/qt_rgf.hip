@@ -850,19 +850,30 @@ struct Solve {
               opGln(sc), 1.0);                                                                 // U = Gln + Y X^H
             g(OP_N, OP_N, nq, n, n, W, opGln(sc), Q, nq, nn);                                  // Q = W Gln
             if (want_gw) gw_extras(sc, n, W, X, Y, Q, GN0, Gc, Lc, g);
-            g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(Q, nq, nn), Y, n, nn);    // Tm = GN0 Q
-            // G< = U + Tm - Tm^H: the reference's last term Gln W^H GN0^H equals
-            // -(GN0 W Gln)^H because Gln is anti-Hermitian
-            AntiArgs an;
-            an.n = n;
-            an.batch = B;
-            an.out = Ln;
-            an.sout = nn;
-            an.X = op_member(Y, n, nn);
-            an.aconst = 1.0;
-            an.Y = op_member(U, n, nn);
-            an.c1 = 1.0;
-            antiherm(s, an);
+            if (!sigma) {
+                g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(Q, nq, nn), Y, n, nn);  // Tm = GN0 Q
+                // G< = U + Tm - Tm^H: the reference's last term Gln W^H GN0^H equals
+                // -(GN0 W Gln)^H because Gln is anti-Hermitian when every injection
+                // is a contact term -(Sigma - Sigma^H) f
+                AntiArgs an;
+                an.n = n;
+                an.batch = B;
+                an.out = Ln;
+                an.sout = nn;
+                an.X = op_member(Y, n, nn);
+                an.aconst = 1.0;
+                an.Y = op_member(U, n, nn);
+                an.c1 = 1.0;
+                antiherm(s, an);
+            } else {
+                // scattering injections (sigma_lesser_ph S, or a Sigma^< that is not
+                // anti-Hermitian) break that symmetry: the reference's four terms
+                g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(Q, nq, nn), Y, n, nn, 1.0,
+                  op_member(U, n, nn), 1.0);                                                     // U + GN0 Q
+                g(OP_N, OP_C, n, nq, n, opGln(sc), W, P, n, nn);                                  // Gln W^H
+                g(OP_N, OP_C, n, n, nq, op_member(P, n, nn), op_member(GN0, n, nn), Ln, n, nn, 1.0,
+                  op_member(Y, n, nn), 1.0);                                                     // + Gln W^H GN0^H
+            }
             const double2* jd = nullptr;
             if (want_j) {
                 // A = G<(nbr) X^H + G(nbr) Q ; Jdens = -2 W^H A  (rgf_mod.f90:215-221)

