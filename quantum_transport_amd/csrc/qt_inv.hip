@@ -276,11 +276,10 @@ void run_panels(hipStream_t s, const InvArgs& a) {
 
 }  // namespace
 
-void zinv(hipStream_t s, const InvArgs& a, Timer* t) {
+namespace {
+
+void zinv_part(hipStream_t s, const InvArgs& a) {
     const int n = a.n;
-    if (n <= 0 || a.batch <= 0) return;
-    if (n > 1024) throw hip_error("zinv: block size above 1024", 6);
-    if (t && t->enabled) t->begin(s, 1);
     hipLaunchKernelGGL(zero_int_kernel, dim3((a.batch + 255) / 256), dim3(256), 0, s, a.info, a.batch);
     if (n <= 256)
         run_panels<32, 256>(s, a);
@@ -291,6 +290,61 @@ void zinv(hipStream_t s, const InvArgs& a, Timer* t) {
     hipLaunchKernelGGL(gj_finish_kernel, dim3((n + 7) / 8, a.batch), dim3(256), 0, s, a.A, a.sa, a.X,
                        a.sx, n, a.piv, a.info, a.active);
     QT_HIP(hipGetLastError());
+}
+
+InvArgs members(const InvArgs& a, int b0, int cnt) {
+    InvArgs h = a;
+    h.batch = cnt;
+    h.A = a.A + b0 * a.sa;
+    h.X = a.X + b0 * a.sx;
+    h.piv = a.piv + (int64_t)b0 * a.n;
+    h.info = a.info + b0;
+    h.active = a.active ? a.active + b0 : nullptr;
+    return h;
+}
+
+// a second stream (default priority) and two events for splitting a batch
+struct Side {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    int device = -1;
+};
+Side& side() {
+    static Side sd;
+    int dev = 0;
+    QT_HIP(hipGetDevice(&dev));
+    if (sd.device != dev) {
+        QT_HIP(hipStreamCreate(&sd.s));
+        QT_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
+        QT_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
+        sd.device = dev;
+    }
+    return sd;
+}
+
+}  // namespace
+
+// The panel kernel keeps one workgroup per member busy for a whole latency-
+// bound column sweep, leaving most CUs idle; the batch is therefore split in
+// two halves on two streams so that the panels of one half run while the
+// MFMA updates of the other half fill the remaining CUs.
+void zinv(hipStream_t s, const InvArgs& a, Timer* t) {
+    const int n = a.n;
+    if (n <= 0 || a.batch <= 0) return;
+    if (n > 1024) throw hip_error("zinv: block size above 1024", 6);
+    if (t && t->enabled) t->begin(s, 1);
+    if (a.batch >= 8 && n >= 64) {
+        Side& sd = side();
+        const int h0 = a.batch / 2;
+        QT_HIP(hipEventRecord(sd.fork, s));
+        QT_HIP(hipStreamWaitEvent(sd.s, sd.fork, 0));
+        zinv_part(sd.s, members(a, h0, a.batch - h0));
+        zinv_part(s, members(a, 0, h0));
+        QT_HIP(hipEventRecord(sd.join, sd.s));
+        QT_HIP(hipStreamWaitEvent(s, sd.join, 0));
+    } else {
+        zinv_part(s, a);
+    }
     if (t && t->enabled)
         t->end(s, 1, 8.0 * (double)n * n * n * (a.flop_batch >= 0 ? a.flop_batch : a.batch));
 }

@@ -53,6 +53,24 @@ def test_invert_vs_numpy(qt, n):
         assert err < 1e-11 * max(1.0, np.linalg.cond(A[b]) / 1e3), (n, b, err)
 
 
+@pytest.mark.parametrize("n,batch", [(64, 8), (96, 11), (300, 10)])
+def test_invert_split_batch(qt, n, batch):
+    """batches of 8 and more are split over two streams: every member, the
+    singular one included, must come back as with a single stream"""
+    rng = np.random.default_rng(n + batch)
+    A = _rand(rng, batch, n, n) + 2.0 * np.eye(n)
+    A[batch // 2 + 1][:, 3] = 0.0  # singular member in the second half
+    X, info = qt.invert_batched(A)
+    for b in range(batch):
+        if b == batch // 2 + 1:
+            assert info[b] != 0 and not X[b].any()
+            continue
+        assert info[b] == 0
+        ref = np.linalg.inv(A[b])
+        err = np.abs(X[b] - ref).max() / np.abs(ref).max()
+        assert err < 1e-11 * max(1.0, np.linalg.cond(A[b]) / 1e3), (n, b, err)
+
+
 def test_invert_pivoting_and_singular(qt):
     A = np.array([[0, 1, 2], [1, 0, 3], [4, -3, 8]], dtype=np.complex128)
     S = A.copy()
