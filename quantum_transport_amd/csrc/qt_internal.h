@@ -59,7 +59,7 @@ void zgemm(hipStream_t s, int opa, int opb, const ZGemmArgs& a, Timer* t);
 
 // Batched in-place-free inversion: Ainv[b] = inv(A[b]); A is destroyed.
 // piv: n ints per member, info: one int per member (0 or first zero pivot
-// column + 1), R: NB*n scratch per member.
+// column + 1).
 struct InvArgs {
     int n = 0, batch = 0;
     double2* A = nullptr;       // work matrices, lda = n, stride sa
@@ -68,7 +68,6 @@ struct InvArgs {
     int64_t sx = 0;
     int* piv = nullptr;         // n per member
     int* info = nullptr;        // 1 per member
-    double2* R = nullptr;       // scratch, 16*n per member
     const int* active = nullptr;
     int flop_batch = -1;        // members counted by the timer (-1: batch)
 };

@@ -147,7 +147,6 @@ struct Work {
     double2* sigL = nullptr; // last-contact self-energy  [B][nn]
     int* piv = nullptr;      // 2B * nmax
     int* info = nullptr;     // 2B
-    double2* R = nullptr;    // 2B * 16 * nmax
     double* zre = nullptr;   // 2B energies (the batch twice, for the lead members)
     int* kidx = nullptr;     // 2B
     double* fF = nullptr;    // B  occupation injected at the first contact
@@ -178,7 +177,6 @@ static size_t work_bytes(const qt_ham& h, int B, bool sigma) {
     for (int i = 0; i < NTEMP; ++i) add(sizeof(double2) * (size_t)nn * 2 * B);
     add(sizeof(double2) * (size_t)nn * B * 2);
     add(sizeof(int) * (size_t)2 * B * h.nmax);
-    add(sizeof(double2) * (size_t)2 * B * 16 * h.nmax);
     int64_t ndiag = 0;
     for (int s = 0; s < h.nx; ++s) ndiag += h.nm[s];
     add(sizeof(double2) * (size_t)3 * B * ndiag);
@@ -204,7 +202,6 @@ static void work_carve(Arena& a, Work& w, const qt_ham& h, int B, bool sigma) {
     w.sigL = a.take<double2>((size_t)w.nn * B);
     w.piv = a.take<int>((size_t)2 * B * h.nmax);
     w.info = a.take<int>((size_t)2 * B);
-    w.R = a.take<double2>((size_t)2 * B * 16 * h.nmax);
     w.zre = a.take<double>((size_t)2 * B);
     w.kidx = a.take<int>((size_t)2 * B);
     w.fF = a.take<double>(B);
@@ -237,7 +234,6 @@ static void inv_batch(Ctx& c, Work& w, int n, int batch, double2* A, int64_t sa,
     a.sx = sx;
     a.piv = w.piv;
     a.info = w.info;
-    a.R = w.R;
     a.active = active;
     a.flop_batch = flop_batch;
     zinv(c.stream, a, &c.timer);
@@ -1442,13 +1438,12 @@ int qt_invert_batched(int32_t n, int32_t batch, void* A, int32_t* info) {
     try {
         QT_HIP(hipSetDevice(c.device));
         const int64_t nn = (int64_t)n * n;
-        c.arena.reserve(sizeof(double2) * (size_t)(2 * nn * batch + 16 * (int64_t)n * batch) +
+        c.arena.reserve(sizeof(double2) * (size_t)(2 * nn * batch) +
                         sizeof(int) * (size_t)(n + 1) * batch + 8192);
         c.arena.reset();
         Work w;
         double2* dA = c.arena.take<double2>(nn * batch);
         double2* dX = c.arena.take<double2>(nn * batch);
-        w.R = c.arena.take<double2>(16 * (int64_t)n * batch);
         w.piv = c.arena.take<int>((int64_t)n * batch);
         w.info = c.arena.take<int>(batch);
         QT_HIP(hipMemcpy(dA, A, sizeof(double2) * nn * batch, hipMemcpyHostToDevice));
@@ -1726,7 +1721,7 @@ struct WSolve {
         const size_t nn = (size_t)nm * nm;
         const size_t blk = sizeof(double2) * nn * B;
         const size_t per = (blk + 256) * (size_t)(19 * nx) + (blk + 256) * 5;
-        return per + sizeof(int) * (size_t)B * (nm + 2) + sizeof(double2) * (size_t)B * 16 * nm + 8192;
+        return per + sizeof(int) * (size_t)B * (nm + 2) + 8192;
     }
     void carve(Arena& a) {
         auto slabs = [&](int count) { return a.take<double2>((size_t)count * B * nn); };
@@ -1745,7 +1740,6 @@ struct WSolve {
         w.piv = a.take<int>((size_t)B * nm);
         w.info = a.take<int>((size_t)B);
         w.infoacc = a.take<int>((size_t)B);
-        w.R = a.take<double2>((size_t)B * 16 * nm);
     }
     double2* at(double2* base, int i) const { return base + (int64_t)i * B * nn; }
     ZOp mem(const double2* base, int i) const { return op_member(at(const_cast<double2*>(base), i), nm, nn); }
