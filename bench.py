@@ -32,6 +32,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "RGF energy-points/s @nb=512×100 slabs: FP64 vs mixed+warm-start @≤1e-8 err"
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+# HBM bytes moved by one 512^3 complex product without an addend (A and B read
+# once, C written once), measured with rocprofv3 FETCH_SIZE / WRITE_SIZE and a
+# copy-kernel calibration; a launch moves this per batch member
+GEMM_BYTES_PER_MEMBER_512 = 12 * 2 ** 20
 
 
 def cpu_baseline(nb: int, slabs: int, threads: int, seed: int) -> dict:
@@ -235,7 +239,8 @@ def main() -> int:
                    "nb": nb, "slabs": nx, "kz": 4, "energies": 1024, "points_per_step": B,
                    "precision": "fp64"},
         "roofline": {"bound": "mfma", "achieved": round(gemm_tf, 3), "peak": PEAK_FP64,
-                     "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4),
+                     "traffic": (GEMM_BYTES_PER_MEMBER_512 * B) if nb == 512 else None,
                      "kernel": "zgemm_mfma_kernel (batched complex128 GEMM)",
                      "launches": acc["gemm_launches"]},
         "path": {"lead_iterations_avg": round(K, 2),
