@@ -24,8 +24,6 @@
 // zeroed (linalg.f90:50-58).
 #include <algorithm>
 #include <climits>
-#include <map>
-#include <mutex>
 
 #include "qt_internal.h"
 
@@ -305,23 +303,23 @@ InvArgs members(const InvArgs& a, int b0, int cnt) {
     return h;
 }
 
-// a second stream (default priority) and two events for splitting a batch,
-// one set per calling stream (callers on different streams run concurrently)
+// a second stream (default priority) and two events for splitting a batch
 struct Side {
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    int device = -1;
 };
-Side& side(hipStream_t main) {
-    static std::mutex mu;
-    static std::map<hipStream_t, Side> sides;
-    std::lock_guard<std::mutex> lk(mu);
-    auto it = sides.find(main);
-    if (it != sides.end()) return it->second;
-    Side sd;
-    QT_HIP(hipStreamCreate(&sd.s));
-    QT_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
-    QT_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
-    return sides.emplace(main, sd).first->second;
+Side& side() {
+    static Side sd;
+    int dev = 0;
+    QT_HIP(hipGetDevice(&dev));
+    if (sd.device != dev) {
+        QT_HIP(hipStreamCreate(&sd.s));
+        QT_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
+        QT_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
+        sd.device = dev;
+    }
+    return sd;
 }
 
 }  // namespace
@@ -336,7 +334,7 @@ void zinv(hipStream_t s, const InvArgs& a, Timer* t) {
     if (n > 1024) throw hip_error("zinv: block size above 1024", 6);
     if (t && t->enabled) t->begin(s, 1);
     if (a.batch >= 8 && n >= 64) {
-        Side& sd = side(s);
+        Side& sd = side();
         const int h0 = a.batch / 2;
         QT_HIP(hipEventRecord(sd.fork, s));
         QT_HIP(hipStreamWaitEvent(sd.s, sd.fork, 0));

@@ -19,8 +19,6 @@
 #include <mutex>
 #include <string>
 #include <vector>
-#include <thread>
-#include <chrono>
 
 #include "../../include/qt_rgf.h"
 #include "qt_internal.h"
@@ -83,21 +81,6 @@ struct Ctx {
 };
 
 static Ctx* g_ctx = nullptr;
-static Ctx* g_lane2 = nullptr;  // second lane of g_ctx's device, created on first use
-
-static void release_lane(Ctx* l) {
-    if (!l) return;
-    (void)hipStreamSynchronize(l->stream);
-    l->arena.release();
-    if (l->timer.created)
-        for (int k = 0; k < 2; ++k)
-            for (int i = 0; i < 4096; ++i) (void)hipEventDestroy(l->timer.ev[k][i]);
-    for (int i = 0; i < 4; ++i) (void)hipEventDestroy(l->ev[i]);
-    (void)hipHostFree(l->host_count);
-    (void)hipStreamDestroy(l->stream);
-    delete l;
-}
-
 static std::mutex g_init_mu;
 static thread_local std::string g_last_error;
 
@@ -988,8 +971,6 @@ int qt_finish(void) {
     if (!g_ctx) return QT_OK;
     Ctx* c = g_ctx;
     (void)hipStreamSynchronize(c->stream);
-    release_lane(g_lane2);
-    g_lane2 = nullptr;
     c->arena.release();
     fft_release();
     if (c->timer.created)
@@ -1090,237 +1071,15 @@ int qt_ham_destroy(qt_ham* h) {
     return QT_OK;
 }
 
-static int choose_batch(qt_ham& h, int npts, bool sigma, const Arena& arena, double frac) {
+static int choose_batch(qt_ham& h, int npts, bool sigma) {
     const char* env = getenv("QT_BATCH");
     if (env && atoi(env) > 0) return std::min(npts, atoi(env));
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) freeb = (size_t)8 << 30;
-    const size_t budget = (size_t)(frac * (double)(freeb + arena.cap));
+    const size_t budget = (size_t)(0.80 * (double)(freeb + g_ctx->arena.cap));
     int B = std::min(npts, 128);
     while (B > 1 && work_bytes(h, B, sigma) > budget) B = B * 3 / 4;
     return std::max(B, 1);
-}
-
-// Everything one batch of points needs from the caller's arguments.
-struct SolveCall {
-    qt_ham* h;
-    const qt_rgf_params* p;
-    const double* En;
-    const int32_t* kidx;
-    const void* const* sigma_lesser_ph;
-    const void* const* sigma_r_ph;
-    qt_rgf_outputs* out;
-    bool fwd, fix, sigma, gw, dev_out;
-    std::vector<int64_t> doff;
-    int64_t ndiag;
-};
-
-// Points [pbeg, pend) in device batches of B on the stream of `c` (one lane).
-static int solve_range(Ctx& c, const SolveCall& a, Work& w, int B, int pbeg, int pend, qt_timing& tm) {
-    qt_ham* h = a.h;
-    const qt_rgf_params* p = a.p;
-    const double* En = a.En;
-    const int32_t* kidx = a.kidx;
-    const void* const* sigma_lesser_ph = a.sigma_lesser_ph;
-    const void* const* sigma_r_ph = a.sigma_r_ph;
-    qt_rgf_outputs* out = a.out;
-    const bool fwd = a.fwd, fix = a.fix, sigma = a.sigma, gw = a.gw, dev_out = a.dev_out;
-    const std::vector<int64_t>& doff = a.doff;
-    const int64_t ndiag = a.ndiag;
-    std::vector<double> hz(2 * B), hf(5 * B);
-    std::vector<int> hk(2 * B);
-    std::vector<int> hinfo(B), hit(2 * B);
-    std::vector<double> htr(B), htre(B);
-    int status = QT_OK;
-    for (int p0 = pbeg; p0 < pend; p0 += B) {
-        const int nb = std::min(B, pend - p0);
-        // pad the batch by repeating its last point (results discarded)
-        for (int b = 0; b < B; ++b) {
-            const int ip = p0 + std::min(b, nb - 1);
-            hz[b] = hz[B + b] = En[ip];
-            hk[b] = hk[B + b] = kidx ? kidx[ip] : 0;
-            // occupations: rgf_mod.f90:100,182 (forward) / :321,395 (backward):
-            // the first contact uses (mur, TEMPr), the last one (mul, TEMPl);
-            // gw_mod.f90:1185,1232,1333: left (mul, TEMPl) injected, its
-            // current with (mul, TEMPr), right (mur, TEMPl)
-            double fF, fFc, fL;
-            if (gw) {
-                fF = ferm((En[ip] - p->mul) / (BOLTZ * p->templ));
-                fFc = ferm((En[ip] - p->mul) / (BOLTZ * p->tempr));
-                fL = ferm((En[ip] - p->mur) / (BOLTZ * p->templ));
-            } else {
-                fF = fFc = ferm((En[ip] - p->mur) / (BOLTZ * p->tempr));
-                fL = ferm((En[ip] - p->mul) / (BOLTZ * p->templ));
-            }
-            hf[b] = fF;
-            hf[B + b] = fL;
-            hf[2 * B + b] = -fF;
-            hf[3 * B + b] = -fL;
-            hf[4 * B + b] = fFc;
-        }
-        QT_HIP(hipMemcpyAsync(w.zre, hz.data(), sizeof(double) * 2 * B, hipMemcpyHostToDevice, c.stream));
-        QT_HIP(hipMemcpyAsync(w.kidx, hk.data(), sizeof(int) * 2 * B, hipMemcpyHostToDevice, c.stream));
-        QT_HIP(hipMemcpyAsync(w.fF, hf.data(), sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
-        QT_HIP(hipMemcpyAsync(w.fL, hf.data() + B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
-        QT_HIP(hipMemcpyAsync(w.nfF, hf.data() + 2 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
-        QT_HIP(hipMemcpyAsync(w.nfL, hf.data() + 3 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
-        QT_HIP(hipMemcpyAsync(w.fFc, hf.data() + 4 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
-        fill_int(c.stream, w.infoacc, B, 0);
-        if (sigma) {
-            // H00 = Hii + sigma_r_ph * S,  SLS = sigma_lesser_ph * S  (rgf_mod.f90:120-121, 132)
-            for (int s = 0; s < h->nx; ++s) {
-                const int n = h->nm[s];
-                const size_t bytes = sizeof(double2) * (size_t)n * n;
-                double2* hr = w.H00 + h->hoff[s] * B;
-                double2* hl = w.SLS + h->hoff[s] * B;
-                for (int b = 0; b < B; ++b) {
-                    const int ip = p0 + std::min(b, nb - 1);
-                    const void* sr = sigma_r_ph[(int64_t)ip * h->nx + s];
-                    const void* sl = sigma_lesser_ph[(int64_t)ip * h->nx + s];
-                    QT_HIP(hipMemcpy(w.T[0] + b * w.nn, sr, bytes, hipMemcpyHostToDevice));
-                    QT_HIP(hipMemcpy(w.T[1] + b * w.nn, sl, bytes, hipMemcpyHostToDevice));
-                }
-                Gemm g{c.stream, &c.timer, B, w.kidx};
-                ZOp S = h->S ? op_k(h->S + h->hoff[s], n, h->hset) : ZOp();
-                if (S.p) {
-                    g(OP_N, OP_N, n, n, n, op_member(w.T[0], n, w.nn), S, w.T[2], n, w.nn);
-                    g(OP_N, OP_N, n, n, n, op_member(w.T[1], n, w.nn), S, hl, n, (int64_t)n * n);
-                } else {
-                    copy_blocks(c.stream, n, B, hl, (int64_t)n * n, op_member(w.T[1], n, w.nn),
-                                nullptr, 0, false);
-                }
-                ShiftArgs sh;
-                sh.n = n;
-                sh.batch = B;
-                sh.out = hr;
-                sh.sout = (int64_t)n * n;
-                sh.M1 = op_k(h->H + h->hoff[s], n, h->hset);
-                sh.c1 = 1.0;
-                sh.M2 = op_member(S.p ? w.T[2] : w.T[0], n, w.nn);
-                sh.c2 = 1.0;
-                sh.kidx = w.kidx;
-                shifted(c.stream, sh);
-            }
-        }
-        Solve sv{c, *h, *p, w, B, fwd, fix, sigma, gw};
-        sv.doff = doff;
-        const bool want_diag = out->gr_diag || out->gl_diag || out->gg_diag;
-        if (want_diag) {
-            if (dev_out) {
-                sv.gr = out->gr_diag ? (double2*)out->gr_diag + (int64_t)p0 * ndiag : w.diag;
-                sv.glo = out->gl_diag ? (double2*)out->gl_diag + (int64_t)p0 * ndiag : w.diag + B * ndiag;
-                sv.gg = out->gg_diag ? (double2*)out->gg_diag + (int64_t)p0 * ndiag : w.diag + 2 * B * ndiag;
-                if (nb < B) {  // padded members must not write past the caller's arrays
-                    sv.gr = w.diag;
-                    sv.glo = w.diag + B * ndiag;
-                    sv.gg = w.diag + 2 * B * ndiag;
-                }
-            } else {
-                sv.gr = w.diag;
-                sv.glo = w.diag + B * ndiag;
-                sv.gg = w.diag + 2 * B * ndiag;
-            }
-        }
-        sv.full.any = out->G_r || out->G_lesser || out->G_greater || out->Jdens || out->Gl || out->Gln ||
-                      (gw && (out->cur || out->GRi1 || out->GLi1 || out->GGi1));
-        if (sv.full.any) {
-            if (nb < B) {
-                // full-block outputs are written per real point only
-            }
-            sv.full.G_r = out->G_r;
-            sv.full.G_lesser = out->G_lesser;
-            sv.full.G_greater = out->G_greater;
-            sv.full.Jdens = out->Jdens;
-            sv.full.Gl = out->Gl;
-            sv.full.Gln = out->Gln;
-            if (gw) {
-                sv.full.cur = out->cur;
-                sv.full.GRi1 = out->GRi1;
-                sv.full.GLi1 = out->GLi1;
-                sv.full.GGi1 = out->GGi1;
-            }
-            sv.full.nx = h->nx;
-            sv.full.p0 = p0;
-        }
-        if (sv.full.any && nb < B) {
-            // restrict host block copies to the real points of this batch
-            static thread_local std::vector<void*> bufs[10];
-            void* const* src[10] = {out->G_r, out->G_lesser, out->G_greater, out->Jdens, out->Gl, out->Gln,
-                                    sv.full.cur, sv.full.GRi1, sv.full.GLi1, sv.full.GGi1};
-            void* const** dst[10] = {&sv.full.G_r, &sv.full.G_lesser, &sv.full.G_greater,
-                                     &sv.full.Jdens, &sv.full.Gl, &sv.full.Gln, &sv.full.cur,
-                                     &sv.full.GRi1, &sv.full.GLi1, &sv.full.GGi1};
-            for (int q = 0; q < 10; ++q) {
-                if (!src[q]) continue;
-                bufs[q].assign((size_t)(p0 + B) * h->nx, nullptr);
-                for (int ip = p0; ip < p0 + nb; ++ip)
-                    for (int s = 0; s < h->nx; ++s)
-                        bufs[q][(size_t)ip * h->nx + s] = src[q][(size_t)ip * h->nx + s];
-                *dst[q] = bufs[q].data();
-            }
-        }
-        sv.run(tm);
-        // per-point scalars
-        if (dev_out) {
-            if (out->tr) QT_HIP(hipMemcpyAsync(out->tr + p0, w.tr, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
-            if (out->tre) QT_HIP(hipMemcpyAsync(out->tre + p0, w.tre, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
-            if (out->info) QT_HIP(hipMemcpyAsync(out->info + p0, w.infoacc, sizeof(int) * nb, hipMemcpyDeviceToDevice, c.stream));
-            if (want_diag && nb < B) {
-                if (out->gr_diag) QT_HIP(hipMemcpyAsync((double2*)out->gr_diag + (int64_t)p0 * ndiag, w.diag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToDevice, c.stream));
-                if (out->gl_diag) QT_HIP(hipMemcpyAsync((double2*)out->gl_diag + (int64_t)p0 * ndiag, w.diag + B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToDevice, c.stream));
-                if (out->gg_diag) QT_HIP(hipMemcpyAsync((double2*)out->gg_diag + (int64_t)p0 * ndiag, w.diag + 2 * B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToDevice, c.stream));
-            }
-            QT_HIP(hipMemcpyAsync(hinfo.data(), w.infoacc, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
-        } else {
-            QT_HIP(hipMemcpyAsync(htr.data(), w.tr, sizeof(double) * B, hipMemcpyDeviceToHost, c.stream));
-            QT_HIP(hipMemcpyAsync(htre.data(), w.tre, sizeof(double) * B, hipMemcpyDeviceToHost, c.stream));
-            QT_HIP(hipMemcpyAsync(hinfo.data(), w.infoacc, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
-            if (out->gr_diag) QT_HIP(hipMemcpyAsync((double2*)out->gr_diag + (int64_t)p0 * ndiag, w.diag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToHost, c.stream));
-            if (out->gl_diag) QT_HIP(hipMemcpyAsync((double2*)out->gl_diag + (int64_t)p0 * ndiag, w.diag + B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToHost, c.stream));
-            if (out->gg_diag) QT_HIP(hipMemcpyAsync((double2*)out->gg_diag + (int64_t)p0 * ndiag, w.diag + 2 * B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToHost, c.stream));
-        }
-        QT_HIP(hipMemcpyAsync(hit.data(), w.lead_it, sizeof(int) * 2 * B, hipMemcpyDeviceToHost, c.stream));
-        QT_HIP(hipStreamSynchronize(c.stream));
-        for (int b = 0; b < nb; ++b) {
-            if (!dev_out) {
-                if (out->tr) out->tr[p0 + b] = htr[b];
-                if (out->tre) out->tre[p0 + b] = htre[b];
-                if (out->info) out->info[p0 + b] = hinfo[b];
-            }
-            if (out->lead_iters && !dev_out) {
-                out->lead_iters[2 * (p0 + b)] = hit[b];
-                out->lead_iters[2 * (p0 + b) + 1] = hit[B + b];
-            }
-            if (hinfo[b] & QT_INFO_SINGULAR) status = QT_ERR_SINGULAR;
-            tm.lead_iterations += hit[b] + hit[B + b];  // per lead, as the reference runs them
-        }
-    }
-    return status;
-}
-
-static void lane_timing(Ctx& c, qt_timing& tm) {
-    c.timer.flush();
-    tm.gemm_ms += c.timer.ms[0];
-    tm.gemm_flops += c.timer.flops[0];
-    tm.gemm_launches += c.timer.launches[0];
-    tm.inv_ms += c.timer.ms[1];
-    tm.inv_flops += c.timer.flops[1];
-    tm.inv_launches += c.timer.launches[1];
-}
-
-static Ctx* second_lane(Ctx& c) {
-    Ctx*& lane2 = g_lane2;
-    if (lane2 && lane2->device == c.device) return lane2;
-    release_lane(lane2);
-    lane2 = nullptr;
-    Ctx* l = new Ctx();
-    l->device = c.device;
-    QT_HIP(hipStreamCreate(&l->stream));
-    QT_HIP(hipHostMalloc((void**)&l->host_count, sizeof(int), hipHostMallocDefault));
-    for (int i = 0; i < 4; ++i) QT_HIP(hipEventCreate(&l->ev[i]));
-    l->timer.enabled = c.timer.enabled;
-    lane2 = l;
-    return l;
 }
 
 int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* En,
@@ -1332,110 +1091,216 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
     if (p->precision != QT_PREC_FP64) return QT_ERR_UNSUPPORTED;
     if (p->variant != QT_RGF_FORWARD && p->variant != QT_RGF_BACKWARD && p->variant != QT_RGF_GW)
         return QT_ERR_INVALID;
-    SolveCall a;
-    a.h = h;
-    a.p = p;
-    a.En = En;
-    a.kidx = kidx;
-    a.sigma_lesser_ph = sigma_lesser_ph;
-    a.sigma_r_ph = sigma_r_ph;
-    a.out = out;
-    a.gw = p->variant == QT_RGF_GW;
-    a.fwd = p->variant != QT_RGF_BACKWARD;
-    a.fix = (p->flags & QT_F_COUPLING_FIX) != 0;
-    if (!h->uniform && (!a.fwd || !a.fix || a.gw)) return QT_ERR_UNSUPPORTED;
+    const bool gw = p->variant == QT_RGF_GW;
+    const bool fwd = p->variant != QT_RGF_BACKWARD;
+    const bool fix = (p->flags & QT_F_COUPLING_FIX) != 0;
+    if (!h->uniform && (!fwd || !fix || gw)) return QT_ERR_UNSUPPORTED;
     if ((sigma_lesser_ph == nullptr) != (sigma_r_ph == nullptr)) return QT_ERR_INVALID;
     for (int i = 0; i < npts; ++i)
         if (kidx && (kidx[i] < 0 || kidx[i] >= h->nk)) return QT_ERR_INVALID;
-    a.dev_out = (p->flags & QT_F_DEVICE_OUTPUTS) != 0;
-    a.sigma = sigma_r_ph != nullptr;
-    const bool full_any = out->G_r || out->G_lesser || out->G_greater || out->Jdens || out->Gl || out->Gln ||
-                          (a.gw && (out->cur || out->GRi1 || out->GLi1 || out->GGi1));
-    a.doff.resize(h->nx);
-    int64_t od = 0;
-    for (int s = 0; s < h->nx; ++s) { a.doff[s] = od; od += h->nm[s]; }
-    a.ndiag = od;
+    const bool dev_out = (p->flags & QT_F_DEVICE_OUTPUTS) != 0;
+    const bool sigma = sigma_r_ph != nullptr;
     Ctx& c = *g_ctx;
     std::lock_guard<std::mutex> lk(c.mu);
     try {
         QT_HIP(hipSetDevice(c.device));
-        // Two lanes (streams, workspaces, host threads) split the points so that
-        // one lane's latency-bound steps (inversion panels, lead decisions,
-        // small kernels) overlap the other's GEMMs.  Only for the resident
-        // batched path: the per-point host-block and scattering paths use
-        // synchronous copies that would serialise the lanes.
-        const char* lanes_env = getenv("QT_LANES");
-        const bool two = !a.sigma && !full_any && npts >= 16 && !(lanes_env && atoi(lanes_env) == 1);
+        const int B = choose_batch(*h, npts, sigma);
+        c.arena.reserve(work_bytes(*h, B, sigma));
+        Work w;
+        work_carve(c.arena, w, *h, B, sigma);
         qt_timing tm{};
+        tm.batch = B;
+        c.timer.reset();
         hipEvent_t e0, e1;
         QT_HIP(hipEventCreate(&e0));
         QT_HIP(hipEventCreate(&e1));
+        QT_HIP(hipEventRecord(e0, c.stream));
+        std::vector<int64_t> doff(h->nx);
+        int64_t od = 0;
+        for (int s = 0; s < h->nx; ++s) { doff[s] = od; od += h->nm[s]; }
+        const int64_t ndiag = od;
+        std::vector<double> hz(2 * B), hf(5 * B);
+        std::vector<int> hk(2 * B);
+        std::vector<int> hinfo(B), hit(2 * B);
+        std::vector<double> htr(B), htre(B);
         int status = QT_OK;
-        auto t_start = std::chrono::steady_clock::now();
-        if (!two) {
-            const int B = choose_batch(*h, npts, a.sigma, c.arena, 0.80);
-            c.arena.reserve(work_bytes(*h, B, a.sigma));
-            Work w;
-            work_carve(c.arena, w, *h, B, a.sigma);
-            tm.batch = B;
-            c.timer.reset();
-            QT_HIP(hipEventRecord(e0, c.stream));
-            status = solve_range(c, a, w, B, 0, npts, tm);
-            QT_HIP(hipEventRecord(e1, c.stream));
-            QT_HIP(hipEventSynchronize(e1));
-            float tot = 0;
-            QT_HIP(hipEventElapsedTime(&tot, e0, e1));
-            tm.total_ms = tot;
-            lane_timing(c, tm);
-        } else {
-            Ctx& c2 = *second_lane(c);
-            c2.timer.enabled = c.timer.enabled;
-            const int half = (npts + 1) / 2;
-            const int B1 = choose_batch(*h, half, a.sigma, c.arena, 0.40);
-            c.arena.reserve(work_bytes(*h, B1, a.sigma));
-            Work w1;
-            work_carve(c.arena, w1, *h, B1, a.sigma);
-            const int B2 = choose_batch(*h, npts - half, a.sigma, c2.arena, 0.80);
-            c2.arena.reserve(work_bytes(*h, B2, a.sigma));
-            Work w2;
-            work_carve(c2.arena, w2, *h, B2, a.sigma);
-            tm.batch = B1 + B2;
-            c.timer.reset();
-            c2.timer.reset();
-            qt_timing tm2{};
-            int status2 = QT_OK;
-            std::string err2;
-            int code2 = 0;
-            std::thread t2([&] {
-                try {
-                    QT_HIP(hipSetDevice(c2.device));
-                    status2 = solve_range(c2, a, w2, B2, half, npts, tm2);
-                    QT_HIP(hipStreamSynchronize(c2.stream));
-                } catch (const hip_error& e) {
-                    err2 = e.what();
-                    code2 = e.code;
+        for (int p0 = 0; p0 < npts; p0 += B) {
+            const int nb = std::min(B, npts - p0);
+            // pad the batch by repeating its last point (results discarded)
+            for (int b = 0; b < B; ++b) {
+                const int ip = p0 + std::min(b, nb - 1);
+                hz[b] = hz[B + b] = En[ip];
+                hk[b] = hk[B + b] = kidx ? kidx[ip] : 0;
+                // occupations: rgf_mod.f90:100,182 (forward) / :321,395 (backward):
+                // the first contact uses (mur, TEMPr), the last one (mul, TEMPl);
+                // gw_mod.f90:1185,1232,1333: left (mul, TEMPl) injected, its
+                // current with (mul, TEMPr), right (mur, TEMPl)
+                double fF, fFc, fL;
+                if (gw) {
+                    fF = ferm((En[ip] - p->mul) / (BOLTZ * p->templ));
+                    fFc = ferm((En[ip] - p->mul) / (BOLTZ * p->tempr));
+                    fL = ferm((En[ip] - p->mur) / (BOLTZ * p->templ));
+                } else {
+                    fF = fFc = ferm((En[ip] - p->mur) / (BOLTZ * p->tempr));
+                    fL = ferm((En[ip] - p->mul) / (BOLTZ * p->templ));
                 }
-            });
-            try {
-                status = solve_range(c, a, w1, B1, 0, half, tm);
-                QT_HIP(hipStreamSynchronize(c.stream));
-            } catch (...) {
-                t2.join();
-                throw;
+                hf[b] = fF;
+                hf[B + b] = fL;
+                hf[2 * B + b] = -fF;
+                hf[3 * B + b] = -fL;
+                hf[4 * B + b] = fFc;
             }
-            t2.join();
-            if (code2) throw hip_error(err2, code2);
-            if (status2 != QT_OK) status = status2;
-            tm.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-            tm.lead_ms += tm2.lead_ms;
-            tm.forward_ms += tm2.forward_ms;
-            tm.backward_ms += tm2.backward_ms;
-            tm.lead_iterations += tm2.lead_iterations;
-            lane_timing(c, tm);
-            lane_timing(c2, tm);
+            QT_HIP(hipMemcpyAsync(w.zre, hz.data(), sizeof(double) * 2 * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.kidx, hk.data(), sizeof(int) * 2 * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.fF, hf.data(), sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.fL, hf.data() + B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.nfF, hf.data() + 2 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.nfL, hf.data() + 3 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(w.fFc, hf.data() + 4 * B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+            fill_int(c.stream, w.infoacc, B, 0);
+            if (sigma) {
+                // H00 = Hii + sigma_r_ph * S,  SLS = sigma_lesser_ph * S  (rgf_mod.f90:120-121, 132)
+                for (int s = 0; s < h->nx; ++s) {
+                    const int n = h->nm[s];
+                    const size_t bytes = sizeof(double2) * (size_t)n * n;
+                    double2* hr = w.H00 + h->hoff[s] * B;
+                    double2* hl = w.SLS + h->hoff[s] * B;
+                    for (int b = 0; b < B; ++b) {
+                        const int ip = p0 + std::min(b, nb - 1);
+                        const void* sr = sigma_r_ph[(int64_t)ip * h->nx + s];
+                        const void* sl = sigma_lesser_ph[(int64_t)ip * h->nx + s];
+                        QT_HIP(hipMemcpy(w.T[0] + b * w.nn, sr, bytes, hipMemcpyHostToDevice));
+                        QT_HIP(hipMemcpy(w.T[1] + b * w.nn, sl, bytes, hipMemcpyHostToDevice));
+                    }
+                    Gemm g{c.stream, &c.timer, B, w.kidx};
+                    ZOp S = h->S ? op_k(h->S + h->hoff[s], n, h->hset) : ZOp();
+                    if (S.p) {
+                        g(OP_N, OP_N, n, n, n, op_member(w.T[0], n, w.nn), S, w.T[2], n, w.nn);
+                        g(OP_N, OP_N, n, n, n, op_member(w.T[1], n, w.nn), S, hl, n, (int64_t)n * n);
+                    } else {
+                        copy_blocks(c.stream, n, B, hl, (int64_t)n * n, op_member(w.T[1], n, w.nn),
+                                    nullptr, 0, false);
+                    }
+                    ShiftArgs sh;
+                    sh.n = n;
+                    sh.batch = B;
+                    sh.out = hr;
+                    sh.sout = (int64_t)n * n;
+                    sh.M1 = op_k(h->H + h->hoff[s], n, h->hset);
+                    sh.c1 = 1.0;
+                    sh.M2 = op_member(S.p ? w.T[2] : w.T[0], n, w.nn);
+                    sh.c2 = 1.0;
+                    sh.kidx = w.kidx;
+                    shifted(c.stream, sh);
+                }
+            }
+            Solve sv{c, *h, *p, w, B, fwd, fix, sigma, gw};
+            sv.doff = doff;
+            const bool want_diag = out->gr_diag || out->gl_diag || out->gg_diag;
+            if (want_diag) {
+                if (dev_out) {
+                    sv.gr = out->gr_diag ? (double2*)out->gr_diag + (int64_t)p0 * ndiag : w.diag;
+                    sv.glo = out->gl_diag ? (double2*)out->gl_diag + (int64_t)p0 * ndiag : w.diag + B * ndiag;
+                    sv.gg = out->gg_diag ? (double2*)out->gg_diag + (int64_t)p0 * ndiag : w.diag + 2 * B * ndiag;
+                    if (nb < B) {  // padded members must not write past the caller's arrays
+                        sv.gr = w.diag;
+                        sv.glo = w.diag + B * ndiag;
+                        sv.gg = w.diag + 2 * B * ndiag;
+                    }
+                } else {
+                    sv.gr = w.diag;
+                    sv.glo = w.diag + B * ndiag;
+                    sv.gg = w.diag + 2 * B * ndiag;
+                }
+            }
+            sv.full.any = out->G_r || out->G_lesser || out->G_greater || out->Jdens || out->Gl || out->Gln ||
+                          (gw && (out->cur || out->GRi1 || out->GLi1 || out->GGi1));
+            if (sv.full.any) {
+                if (nb < B) {
+                    // full-block outputs are written per real point only
+                }
+                sv.full.G_r = out->G_r;
+                sv.full.G_lesser = out->G_lesser;
+                sv.full.G_greater = out->G_greater;
+                sv.full.Jdens = out->Jdens;
+                sv.full.Gl = out->Gl;
+                sv.full.Gln = out->Gln;
+                if (gw) {
+                    sv.full.cur = out->cur;
+                    sv.full.GRi1 = out->GRi1;
+                    sv.full.GLi1 = out->GLi1;
+                    sv.full.GGi1 = out->GGi1;
+                }
+                sv.full.nx = h->nx;
+                sv.full.p0 = p0;
+            }
+            if (sv.full.any && nb < B) {
+                // restrict host block copies to the real points of this batch
+                static thread_local std::vector<void*> bufs[10];
+                void* const* src[10] = {out->G_r, out->G_lesser, out->G_greater, out->Jdens, out->Gl, out->Gln,
+                                        sv.full.cur, sv.full.GRi1, sv.full.GLi1, sv.full.GGi1};
+                void* const** dst[10] = {&sv.full.G_r, &sv.full.G_lesser, &sv.full.G_greater,
+                                         &sv.full.Jdens, &sv.full.Gl, &sv.full.Gln, &sv.full.cur,
+                                         &sv.full.GRi1, &sv.full.GLi1, &sv.full.GGi1};
+                for (int q = 0; q < 10; ++q) {
+                    if (!src[q]) continue;
+                    bufs[q].assign((size_t)(p0 + B) * h->nx, nullptr);
+                    for (int ip = p0; ip < p0 + nb; ++ip)
+                        for (int s = 0; s < h->nx; ++s)
+                            bufs[q][(size_t)ip * h->nx + s] = src[q][(size_t)ip * h->nx + s];
+                    *dst[q] = bufs[q].data();
+                }
+            }
+            sv.run(tm);
+            // per-point scalars
+            if (dev_out) {
+                if (out->tr) QT_HIP(hipMemcpyAsync(out->tr + p0, w.tr, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
+                if (out->tre) QT_HIP(hipMemcpyAsync(out->tre + p0, w.tre, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
+                if (out->info) QT_HIP(hipMemcpyAsync(out->info + p0, w.infoacc, sizeof(int) * nb, hipMemcpyDeviceToDevice, c.stream));
+                if (want_diag && nb < B) {
+                    if (out->gr_diag) QT_HIP(hipMemcpyAsync((double2*)out->gr_diag + (int64_t)p0 * ndiag, w.diag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToDevice, c.stream));
+                    if (out->gl_diag) QT_HIP(hipMemcpyAsync((double2*)out->gl_diag + (int64_t)p0 * ndiag, w.diag + B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToDevice, c.stream));
+                    if (out->gg_diag) QT_HIP(hipMemcpyAsync((double2*)out->gg_diag + (int64_t)p0 * ndiag, w.diag + 2 * B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToDevice, c.stream));
+                }
+                QT_HIP(hipMemcpyAsync(hinfo.data(), w.infoacc, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+            } else {
+                QT_HIP(hipMemcpyAsync(htr.data(), w.tr, sizeof(double) * B, hipMemcpyDeviceToHost, c.stream));
+                QT_HIP(hipMemcpyAsync(htre.data(), w.tre, sizeof(double) * B, hipMemcpyDeviceToHost, c.stream));
+                QT_HIP(hipMemcpyAsync(hinfo.data(), w.infoacc, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+                if (out->gr_diag) QT_HIP(hipMemcpyAsync((double2*)out->gr_diag + (int64_t)p0 * ndiag, w.diag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToHost, c.stream));
+                if (out->gl_diag) QT_HIP(hipMemcpyAsync((double2*)out->gl_diag + (int64_t)p0 * ndiag, w.diag + B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToHost, c.stream));
+                if (out->gg_diag) QT_HIP(hipMemcpyAsync((double2*)out->gg_diag + (int64_t)p0 * ndiag, w.diag + 2 * B * ndiag, sizeof(double2) * nb * ndiag, hipMemcpyDeviceToHost, c.stream));
+            }
+            QT_HIP(hipMemcpyAsync(hit.data(), w.lead_it, sizeof(int) * 2 * B, hipMemcpyDeviceToHost, c.stream));
+            QT_HIP(hipStreamSynchronize(c.stream));
+            for (int b = 0; b < nb; ++b) {
+                if (!dev_out) {
+                    if (out->tr) out->tr[p0 + b] = htr[b];
+                    if (out->tre) out->tre[p0 + b] = htre[b];
+                    if (out->info) out->info[p0 + b] = hinfo[b];
+                }
+                if (out->lead_iters && !dev_out) {
+                    out->lead_iters[2 * (p0 + b)] = hit[b];
+                    out->lead_iters[2 * (p0 + b) + 1] = hit[B + b];
+                }
+                if (hinfo[b] & QT_INFO_SINGULAR) status = QT_ERR_SINGULAR;
+                tm.lead_iterations += hit[b] + hit[B + b];  // per lead, as the reference runs them
+            }
         }
+        QT_HIP(hipEventRecord(e1, c.stream));
+        QT_HIP(hipEventSynchronize(e1));
+        float tot = 0;
+        QT_HIP(hipEventElapsedTime(&tot, e0, e1));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        c.timer.flush();
+        tm.total_ms = tot;
+        tm.gemm_ms = c.timer.ms[0];
+        tm.gemm_flops = c.timer.flops[0];
+        tm.gemm_launches = c.timer.launches[0];
+        tm.inv_ms = c.timer.ms[1];
+        tm.inv_flops = c.timer.flops[1];
+        tm.inv_launches = c.timer.launches[1];
         c.last = tm;
         return status;
     } catch (const hip_error& e) {
