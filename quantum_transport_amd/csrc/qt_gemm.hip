@@ -220,19 +220,21 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
     const int m0 = (tile % tiles_m) * WBM;
     const int n0 = (tile / tiles_m) * WBN;
 
-    __shared__ double2 As[2][WBK][WBM];
-    __shared__ double2 Bs[2][WBK][WBN];
+    // LDS rows padded (A: 130, B: 66 complex) so that the stores of every
+    // load pattern below hit distinct banks within a 16-lane pass
+    __shared__ double2 As[2][WBK][WBM + 2];
+    __shared__ double2 Bs[2][WBK][WBN + 2];
 
     const int tid = threadIdx.x;
     double2 ra[4], rb[2];
 
     auto loadA = [&](int k0) {
         if (OPA == OP_N) {
-            const int kk = tid >> 5, ii = (tid & 31) * 4;
-            const int gk = k0 + kk;
+            // element e = tid + 256 q: rows contiguous across lanes
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int gi = m0 + ii + q;
+                const int e = tid + 256 * q, kk = e >> 7, ii = e & 127;
+                const int gi = m0 + ii, gk = k0 + kk;
                 ra[q] = (gi < M && gk < K) ? Ap[gi + gk * lda] : make_double2(0.0, 0.0);
             }
         } else {
@@ -247,9 +249,11 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
     };
     auto storeA = [&](int buf) {
         if (OPA == OP_N) {
-            const int kk = tid >> 5, ii = (tid & 31) * 4;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) As[buf][kk][ii + q] = ra[q];
+            for (int q = 0; q < 4; ++q) {
+                const int e = tid + 256 * q;
+                As[buf][e >> 7][e & 127] = ra[q];
+            }
         } else {
             const int ii = tid >> 1, kk = (tid & 1) * 4;
 #pragma unroll
@@ -258,32 +262,30 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
     };
     auto loadB = [&](int k0) {
         if (OPB == OP_N) {
-            const int jj = tid >> 2, kk = (tid & 3) * 2;
-            const int gj = n0 + jj;
+            // 8 lanes cover the 8 k of one column (128 B contiguous)
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                const int gk = k0 + kk + q;
+                const int kk = tid & 7, jj = (tid >> 3) + 32 * q;
+                const int gj = n0 + jj, gk = k0 + kk;
                 rb[q] = (gj < N && gk < K) ? Bp[gk + gj * ldb] : make_double2(0.0, 0.0);
             }
         } else {
-            const int kk = tid >> 5, jj = (tid & 31) * 2;
-            const int gk = k0 + kk;
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                const int gj = n0 + jj + q;
+                const int jj = tid & 63, kk = (tid >> 6) + 4 * q;
+                const int gj = n0 + jj, gk = k0 + kk;
                 rb[q] = (gj < N && gk < K) ? Bp[gj + gk * ldb] : make_double2(0.0, 0.0);
             }
         }
     };
     auto storeB = [&](int buf) {
         if (OPB == OP_N) {
-            const int jj = tid >> 2, kk = (tid & 3) * 2;
 #pragma unroll
-            for (int q = 0; q < 2; ++q) Bs[buf][kk + q][jj] = rb[q];
+            for (int q = 0; q < 2; ++q) Bs[buf][tid & 7][(tid >> 3) + 32 * q] = rb[q];
         } else {
-            const int kk = tid >> 5, jj = (tid & 31) * 2;
 #pragma unroll
-            for (int q = 0; q < 2; ++q) Bs[buf][kk][jj + q] = OPB == OP_C ? cconj(rb[q]) : rb[q];
+            for (int q = 0; q < 2; ++q)
+                Bs[buf][(tid >> 6) + 4 * q][tid & 63] = OPB == OP_C ? cconj(rb[q]) : rb[q];
         }
     };
 
