@@ -220,10 +220,12 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
     const int m0 = (tile % tiles_m) * WBM;
     const int n0 = (tile / tiles_m) * WBN;
 
-    // LDS rows padded (A: 130, B: 66 complex) so that the stores of every
-    // load pattern below hit distinct banks within a 16-lane pass
-    __shared__ double2 As[2][WBK][WBM + 2];
-    __shared__ double2 Bs[2][WBK][WBN + 2];
+    // Every load pattern below has consecutive lanes either along the LDS row
+    // (contiguous stores) or along k (8 lanes = the 8 rows of a stage).  The
+    // rows are padded to 137 and 73 complex (row stride = 144 mod 256 bytes),
+    // so 8 lanes writing 8 different rows hit 8 distinct 16-byte bank groups.
+    __shared__ double2 As[2][WBK][WBM + 9];
+    __shared__ double2 Bs[2][WBK][WBN + 9];
 
     const int tid = threadIdx.x;
     double2 ra[4], rb[2];
@@ -238,11 +240,11 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
                 ra[q] = (gi < M && gk < K) ? Ap[gi + gk * lda] : make_double2(0.0, 0.0);
             }
         } else {
-            const int ii = tid >> 1, kk = (tid & 1) * 4;
-            const int gi = m0 + ii;
+            // 8 lanes cover the 8 k of one row of op(A) (128 B contiguous)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int gk = k0 + kk + q;
+                const int kk = tid & 7, ii = (tid >> 3) + 32 * q;
+                const int gi = m0 + ii, gk = k0 + kk;
                 ra[q] = (gi < M && gk < K) ? Ap[gk + gi * lda] : make_double2(0.0, 0.0);
             }
         }
@@ -255,9 +257,9 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
                 As[buf][e >> 7][e & 127] = ra[q];
             }
         } else {
-            const int ii = tid >> 1, kk = (tid & 1) * 4;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) As[buf][kk + q][ii] = OPA == OP_C ? cconj(ra[q]) : ra[q];
+            for (int q = 0; q < 4; ++q)
+                As[buf][tid & 7][(tid >> 3) + 32 * q] = OPA == OP_C ? cconj(ra[q]) : ra[q];
         }
     };
     auto loadB = [&](int k0) {
