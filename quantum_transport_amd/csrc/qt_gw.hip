@@ -172,7 +172,52 @@ __device__ __forceinline__ void finish(double2* s, const double2* tw, const Spec
     }
 }
 
+// ---------------------------------------------------------------- packed transforms
+// Every G operand enters as i Im G, i.e. i times a real series y.  Two such
+// series share one complex transform: with Z = F(y1 + i y2),
+//   F(i y1)[k] = i (Z[k] + conj Z[-k]) / 2,   F(i y2)[k] = (Z[k] - conj Z[-k]) / 2,
+// and the flipped operand of a correlation needs no transform of its own,
+// since F(Yf)[k] = F(Y)[-k].
+template <int L, int NT>
+__device__ __forceinline__ void load_pack(double2* s, const double2* __restrict__ y1,
+                                          const double2* __restrict__ y2, int n) {
+    for (int i = threadIdx.x; i < L; i += NT) {
+        double2 v = make_double2(0.0, 0.0);
+        if (i < n) v = make_double2(y1[i].y, y2 ? y2[i].y : 0.0);
+        s[pad(i)] = v;
+    }
+}
+// s <- F(Im y1 + i Im y2)
+template <int L, int NT>
+__device__ __forceinline__ void fwd_pack(double2* s, const double2* tw, const double2* y1, const double2* y2,
+                                         int n) {
+    __syncthreads();
+    load_pack<L, NT>(s, y1, y2, n);
+    __syncthreads();
+    fft::transform<L, NT, false, RADIX>(s, tw);
+}
+// Z[k] and Z[-k] of the thread's sample q
+template <int L, int NT>
+__device__ __forceinline__ void zpair(const double2* s, int q, double2& zk, double2& zm) {
+    const int i = threadIdx.x + q * NT;
+    zk = s[pad(i)];
+    zm = s[pad((L - i) & (L - 1))];
+}
+// F(i y1) and F(i y2) at k from zk = Z[k], zm = Z[-k] (swap them for -k)
+__device__ __forceinline__ double2 spec1(double2 zk, double2 zm) {
+    return make_double2(0.5 * (zm.y - zk.y), 0.5 * (zk.x + zm.x));
+}
+__device__ __forceinline__ double2 spec2(double2 zk, double2 zm) {
+    return make_double2(0.5 * (zk.x - zm.x), 0.5 * (zk.y + zm.y));
+}
+
 // ---------------------------------------------------------------- P
+// Two passes over k per q, each with two packed transforms per k (the
+// reference takes eight forward transforms per k):
+//   1. (G^<_ji, G^>_ji) at k' and (G^<_ij, G^>_ij) at k give P^< and P^>,
+//      correlations of real series that share one inverse transform (real
+//      and imaginary parts);
+//   2. (G^<_ij at k, G^<_ji at k') and (G^R_ij at k, k') give P^R.
 template <int L, int NT>
 __global__ __launch_bounds__(NT) void gw_pol_kernel(GwDims d, const int* __restrict__ elems, int nelem,
                                                     const double2* __restrict__ Gr,
@@ -180,6 +225,8 @@ __global__ __launch_bounds__(NT) void gw_pol_kernel(GwDims d, const int* __restr
                                                     const double2* __restrict__ Gg, double2* Pr,
                                                     double2* Pl, double2* Pg, double2 dE,
                                                     const double2* __restrict__ tw) {
+    using S = Spec<L, NT>;
+    constexpr int PER = S::PER;
     extern __shared__ double2 s[];
     const int w = blockIdx.x;
     if (w >= nelem * d.nx) return;
@@ -188,60 +235,112 @@ __global__ __launch_bounds__(NT) void gw_pol_kernel(GwDims d, const int* __restr
     const int row = ij % d.nm, col = ij / d.nm;
     const int ji = col + row * d.nm;
     const int n = d.nen;
-    Spec<L, NT> acc, hold;
     for (int iq = 0; iq < d.nk; ++iq) {
-        // P^< = sum_k corr(G^<_ij(k), G^>_ji(k'))
-        acc.zero();
+        S accl, accg;
+        accl.zero();
+        accg.zero();
         for (int ik = 0; ik < d.nk; ++ik) {
             const int kd = kd_index(ik, iq, d.nky, d.nkz);
-            corr_term<L, NT>(s, tw, acc, hold, series(Gl, d, ij, ix, ik), L_IMAG, series(Gg, d, ji, ix, kd),
-                             L_IMAG, n);
+            // F(i Im G^<_ji(k'))(-k), F(i Im G^>_ji(k'))(-k)
+            fwd_pack<L, NT>(s, tw, series(Gl, d, ji, ix, kd), series(Gg, d, ji, ix, kd), n);
+            double2 bl[PER], bg[PER];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                if (!S::ok(threadIdx.x + q * NT)) continue;
+                double2 zk, zm;
+                zpair<L, NT>(s, q, zk, zm);
+                bl[q] = spec1(zm, zk);
+                bg[q] = spec2(zm, zk);
+            }
+            // P^< += F(i Im G^<_ij(k)) F(i Im G^>_ji(k'))(-k), P^> likewise
+            fwd_pack<L, NT>(s, tw, series(Gl, d, ij, ix, ik), series(Gg, d, ij, ix, ik), n);
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                if (!S::ok(threadIdx.x + q * NT)) continue;
+                double2 zk, zm;
+                zpair<L, NT>(s, q, zk, zm);
+                accl.v[q] = cadd(accl.v[q], cmul(spec1(zk, zm), bg[q]));
+                accg.v[q] = cadd(accg.v[q], cmul(spec2(zk, zm), bl[q]));
+            }
         }
-        finish<L, NT>(s, tw, acc, dE, n, true, series(Pl, d, ij, ix, iq));
-        // P^> = sum_k corr(G^>_ij(k), G^<_ji(k'))
-        acc.zero();
+        // P^< (real part) and P^> (imaginary part) from one inverse transform
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * NT;
+            if (S::ok(i)) s[pad(i)] = make_double2(accl.v[q].x - accg.v[q].y, accl.v[q].y + accg.v[q].x);
+        }
+        __syncthreads();
+        fft::transform<L, NT, true, RADIX>(s, tw);
+        const double inv = 1.0 / (double)L;
+        const int h = n / 2;
+        double2* zl = series(Pl, d, ij, ix, iq);
+        double2* zg = series(Pg, d, ij, ix, iq);
+        for (int i = threadIdx.x; i < n; i += NT) {
+            const int l = i - h;
+            const double2 v = s[pad(l >= 0 ? l : l + L)];
+            zl[i] = make_double2(dE.x * v.x * inv, dE.y * v.x * inv);
+            zg[i] = make_double2(dE.x * v.y * inv, dE.y * v.y * inv);
+        }
+        // P^R += F(i Im G^<_ij(k)) F(-i Im G^R_ij(k'))(-k) + F(i Im G^R_ij(k)) F(i Im G^<_ji(k'))(-k)
+        S accr;
+        accr.zero();
         for (int ik = 0; ik < d.nk; ++ik) {
             const int kd = kd_index(ik, iq, d.nky, d.nkz);
-            corr_term<L, NT>(s, tw, acc, hold, series(Gg, d, ij, ix, ik), L_IMAG, series(Gl, d, ji, ix, kd),
-                             L_IMAG, n);
+            fwd_pack<L, NT>(s, tw, series(Gl, d, ij, ix, ik), series(Gl, d, ji, ix, kd), n);
+            double2 al[PER], bl[PER];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                if (!S::ok(threadIdx.x + q * NT)) continue;
+                double2 zk, zm;
+                zpair<L, NT>(s, q, zk, zm);
+                al[q] = spec1(zk, zm);
+                bl[q] = spec2(zm, zk);
+            }
+            const bool same = kd == ik;
+            fwd_pack<L, NT>(s, tw, series(Gr, d, ij, ix, ik), same ? nullptr : series(Gr, d, ij, ix, kd), n);
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                if (!S::ok(threadIdx.x + q * NT)) continue;
+                double2 zk, zm;
+                zpair<L, NT>(s, q, zk, zm);
+                const double2 rk = spec1(zk, zm);
+                const double2 rd = same ? spec1(zm, zk) : spec2(zm, zk);
+                const double2 t1 = cmul(al[q], rd), t2 = cmul(rk, bl[q]);
+                accr.v[q] = make_double2(accr.v[q].x - t1.x + t2.x, accr.v[q].y - t1.y + t2.y);
+            }
         }
-        finish<L, NT>(s, tw, acc, dE, n, true, series(Pg, d, ij, ix, iq));
-        // P^R = sum_k corr(G^<_ij(k), conj G^R_ij(k')) + corr(G^R_ij(k), G^<_ji(k'))
-        acc.zero();
-        for (int ik = 0; ik < d.nk; ++ik) {
-            const int kd = kd_index(ik, iq, d.nky, d.nkz);
-            corr_term<L, NT>(s, tw, acc, hold, series(Gl, d, ij, ix, ik), L_IMAG, series(Gr, d, ij, ix, kd),
-                             L_IMAG_CONJ, n);
-            corr_term<L, NT>(s, tw, acc, hold, series(Gr, d, ij, ix, ik), L_IMAG, series(Gl, d, ji, ix, kd),
-                             L_IMAG, n);
-        }
-        finish<L, NT>(s, tw, acc, dE, n, true, series(Pr, d, ij, ix, iq));
+        finish<L, NT>(s, tw, accr, dE, n, true, series(Pr, d, ij, ix, iq));
     }
 }
 
 // ---------------------------------------------------------------- Sigma
-// One workgroup per element pair (ij, ji) with row <= col and slab ix.  The
-// Sigma of ij is kept in registers (EXT samples per thread, the same sample
-// mapping for every output), the Sigma of ji is folded into it as
-// (S + S^T)/2 as soon as each output is ready, so no thread reads back
-// global memory.
+// Sigma of element e at slab ix and point k, the q-sums in two passes:
+//   1. i Im G^< and i Im G^R at k' share one packed transform; with the
+//      transforms of W^< and W^R they give Sigma^< and the three terms of
+//      Sigma^R (W^< + W^R enters by linearity);
+//   2. i Im G^> at k' and W^> give Sigma^>.
+// The reference takes eight forward transforms per q, these take five.
+// Every thread owns the same samples i = tid + j NT of every output, so the
+// (S + S^T)/2 fold reads back what the same thread wrote for the transposed
+// element eo (second == true) and writes the mean to both.
 template <int L, int NT>
-__device__ __forceinline__ void sigma_elem(double2* s, const double2* tw, const GwDims& d, int ij, int ix,
-                                           int ik, const double2* Gr, const double2* Gl, const double2* Gg,
-                                           const double2* Wr, const double2* Wl, const double2* Wg,
-                                           double2 dE, Spec<L, NT>& acc, Spec<L, NT>& hold, bool second,
-                                           double2* aR, double2* aL, double2* aG) {
+__device__ __forceinline__ void sigma_elem(double2* s, const double2* tw, const GwDims& d, int e, int eo,
+                                           int ix, int ik, const double2* Gr, const double2* Gl,
+                                           const double2* Gg, const double2* Wr, const double2* Wl,
+                                           const double2* Wg, double2 dE, double2* Sr, double2* Sl,
+                                           double2* Sg, bool second) {
+    using Sp = Spec<L, NT>;
+    constexpr int PER = Sp::PER;
     constexpr int EXT = (L / 2 + NT - 1) / NT;
     const int n = d.nen;
-    double2 t[EXT];
-    double dim[EXT];
-    // t = dE/L * F^-1[acc] at the samples this thread owns
-    auto extract = [&]() {
+    const double inv = 1.0 / (double)L;
+    // dE/L * F^-1[acc] at the samples this thread owns
+    auto extract = [&](const Sp& acc, double2* t) {
         __syncthreads();
         acc.put(s);
         __syncthreads();
         fft::transform<L, NT, true, RADIX>(s, tw);
-        const double inv = 1.0 / (double)L;
 #pragma unroll
         for (int q = 0; q < EXT; ++q) {
             const int i = threadIdx.x + q * NT;
@@ -249,58 +348,108 @@ __device__ __forceinline__ void sigma_elem(double2* s, const double2* tw, const 
             t[q] = make_double2((dE.x * v.x - dE.y * v.y) * inv, (dE.x * v.y + dE.y * v.x) * inv);
         }
     };
-    auto fold = [&](double2* a) {
+    auto store = [&](double2* S, const double2* t) {
+        double2* own = series(S, d, e, ix, ik);
+        double2* other = series(S, d, eo, ix, ik);
 #pragma unroll
-        for (int q = 0; q < EXT; ++q)
-            a[q] = second ? make_double2(0.5 * (a[q].x + t[q].x), 0.5 * (a[q].y + t[q].y)) : t[q];
+        for (int q = 0; q < EXT; ++q) {
+            const int i = threadIdx.x + q * NT;
+            if (i >= n) continue;
+            if (second) {
+                const double2 o = other[i];
+                const double2 m = make_double2(0.5 * (o.x + t[q].x), 0.5 * (o.y + t[q].y));
+                own[i] = m;
+                other[i] = m;
+            } else {
+                own[i] = t[q];
+            }
+        }
     };
-    // conv term: acc += F(X) F(Yr); X = i Im G, Yr the rotated W (+ W2)
-    auto conv_term = [&](const double2* g, const double2* w, const double2* w2) {
+    auto fwd_w = [&](const double2* w) {
         __syncthreads();
-        load_rot<L, NT>(s, w, w2, n);
+        load_rot<L, NT>(s, w, nullptr, n);
         __syncthreads();
         fft::transform<L, NT, false, RADIX>(s, tw);
-        hold.take(s);
-        __syncthreads();
-        load_x<L, NT>(s, g, n, L_IMAG);
-        __syncthreads();
-        fft::transform<L, NT, false, RADIX>(s, tw);
-        acc.fma_lds(s, hold);
     };
-    // Sigma^< = sum_q conv(G^<(k'), W^<(q))
-    acc.zero();
-    for (int iq = 0; iq < d.nk; ++iq) {
-        const int kd = kd_index(ik, iq, d.nky, d.nkz);
-        conv_term(series(Gl, d, ij, ix, kd), series(Wl, d, ij, ix, iq), nullptr);
-    }
-    extract();
+    // the first element's outputs are read back below by the same threads
+    if (second) __threadfence_block();
+    double2 t[EXT], rre[EXT];
+    double dim[EXT];
+    {
+        Sp accl, accr;
+        accl.zero();
+        accr.zero();
+        for (int iq = 0; iq < d.nk; ++iq) {
+            const int kd = kd_index(ik, iq, d.nky, d.nkz);
+            fwd_pack<L, NT>(s, tw, series(Gl, d, e, ix, kd), series(Gr, d, e, ix, kd), n);
+            double2 gl[PER], gr[PER];
 #pragma unroll
-    for (int q = 0; q < EXT; ++q) dim[q] = -t[q].y;
-    fold(aL);
-    // Sigma^> = sum_q conv(G^>(k'), W^>(q))
-    acc.zero();
-    for (int iq = 0; iq < d.nk; ++iq) {
-        const int kd = kd_index(ik, iq, d.nky, d.nkz);
-        conv_term(series(Gg, d, ij, ix, kd), series(Wg, d, ij, ix, iq), nullptr);
-    }
-    extract();
+            for (int q = 0; q < PER; ++q) {
+                if (!Sp::ok(threadIdx.x + q * NT)) continue;
+                double2 zk, zm;
+                zpair<L, NT>(s, q, zk, zm);
+                gl[q] = spec1(zk, zm);
+                gr[q] = spec2(zk, zm);
+            }
+            // Sigma^< += F(G^<) F(W^<r), Sigma^R += F(G^R) F(W^<r)
+            fwd_w(series(Wl, d, e, ix, iq));
 #pragma unroll
-    for (int q = 0; q < EXT; ++q) dim[q] += t[q].y;
-    fold(aG);
-    // Sigma^R = sum_q conv(G^<, W^R) + conv(G^R, W^< + W^R), then
-    // Re Sigma^R + i Im(Sigma^> - Sigma^<) / 2  (gw_mod.f90:694-698)
-    acc.zero();
-    for (int iq = 0; iq < d.nk; ++iq) {
-        const int kd = kd_index(ik, iq, d.nky, d.nkz);
-        conv_term(series(Gl, d, ij, ix, kd), series(Wr, d, ij, ix, iq), nullptr);
-        conv_term(series(Gr, d, ij, ix, kd), series(Wl, d, ij, ix, iq), series(Wr, d, ij, ix, iq));
-    }
-    extract();
+            for (int q = 0; q < PER; ++q) {
+                const int i = threadIdx.x + q * NT;
+                if (!Sp::ok(i)) continue;
+                const double2 wv = s[pad(i)];
+                accl.v[q] = cadd(accl.v[q], cmul(gl[q], wv));
+                accr.v[q] = cadd(accr.v[q], cmul(gr[q], wv));
+            }
+            // Sigma^R += (F(G^<) + F(G^R)) F(W^Rr)
+            fwd_w(series(Wr, d, e, ix, iq));
 #pragma unroll
-    for (int q = 0; q < EXT; ++q) t[q].y = 0.5 * dim[q];
-    fold(aR);
+            for (int q = 0; q < PER; ++q) {
+                const int i = threadIdx.x + q * NT;
+                if (!Sp::ok(i)) continue;
+                accr.v[q] = cadd(accr.v[q], cmul(cadd(gl[q], gr[q]), s[pad(i)]));
+            }
+        }
+        extract(accl, t);
+#pragma unroll
+        for (int q = 0; q < EXT; ++q) dim[q] = -t[q].y;
+        store(Sl, t);
+        extract(accr, t);
+#pragma unroll
+        for (int q = 0; q < EXT; ++q) rre[q] = t[q];
+    }
+    {
+        Sp accg;
+        accg.zero();
+        for (int iq = 0; iq < d.nk; ++iq) {
+            const int kd = kd_index(ik, iq, d.nky, d.nkz);
+            fwd_pack<L, NT>(s, tw, series(Gg, d, e, ix, kd), nullptr, n);
+            double2 gg[PER];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                if (!Sp::ok(threadIdx.x + q * NT)) continue;
+                double2 zk, zm;
+                zpair<L, NT>(s, q, zk, zm);
+                gg[q] = spec1(zk, zm);
+            }
+            fwd_w(series(Wg, d, e, ix, iq));
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int i = threadIdx.x + q * NT;
+                if (!Sp::ok(i)) continue;
+                accg.v[q] = cadd(accg.v[q], cmul(gg[q], s[pad(i)]));
+            }
+        }
+        extract(accg, t);
+    }
+    store(Sg, t);
+    // Sigma^R = Re Sigma^R + i Im(Sigma^> - Sigma^<) / 2  (gw_mod.f90:694-698)
+#pragma unroll
+    for (int q = 0; q < EXT; ++q) t[q] = make_double2(rre[q].x, 0.5 * (dim[q] + t[q].y));
+    store(Sr, t);
 }
 
+// One workgroup per element pair (ij, ji) with row <= col and slab ix.
 template <int L, int NT>
 __global__ __launch_bounds__(NT) void gw_sigma_kernel(GwDims d, const int* __restrict__ pairs, int npair,
                                                       const double2* __restrict__ Gr,
@@ -311,7 +460,6 @@ __global__ __launch_bounds__(NT) void gw_sigma_kernel(GwDims d, const int* __res
                                                       const double2* __restrict__ Wg, double2* Sr,
                                                       double2* Sl, double2* Sg, double2 dE,
                                                       const double2* __restrict__ tw) {
-    constexpr int EXT = (L / 2 + NT - 1) / NT;
     extern __shared__ double2 s[];
     const int w = blockIdx.x;
     if (w >= npair * d.nx) return;
@@ -319,27 +467,10 @@ __global__ __launch_bounds__(NT) void gw_sigma_kernel(GwDims d, const int* __res
     const int ij = pairs[p];
     const int row = ij % d.nm, col = ij / d.nm;
     const int ji = col + row * d.nm;
-    const int n = d.nen;
-    Spec<L, NT> acc, hold;
     for (int ik = 0; ik < d.nk; ++ik) {
-        double2 aR[EXT], aL[EXT], aG[EXT];
-        sigma_elem<L, NT>(s, tw, d, ij, ix, ik, Gr, Gl, Gg, Wr, Wl, Wg, dE, acc, hold, false, aR, aL, aG);
+        sigma_elem<L, NT>(s, tw, d, ij, ij, ix, ik, Gr, Gl, Gg, Wr, Wl, Wg, dE, Sr, Sl, Sg, false);
         if (ji != ij)
-            sigma_elem<L, NT>(s, tw, d, ji, ix, ik, Gr, Gl, Gg, Wr, Wl, Wg, dE, acc, hold, true, aR, aL, aG);
-        double2 *r1 = series(Sr, d, ij, ix, ik), *l1 = series(Sl, d, ij, ix, ik), *g1 = series(Sg, d, ij, ix, ik);
-        double2 *r2 = series(Sr, d, ji, ix, ik), *l2 = series(Sl, d, ji, ix, ik), *g2 = series(Sg, d, ji, ix, ik);
-#pragma unroll
-        for (int q = 0; q < EXT; ++q) {
-            const int i = threadIdx.x + q * NT;
-            if (i < n) {
-                r1[i] = aR[q];
-                l1[i] = aL[q];
-                g1[i] = aG[q];
-                r2[i] = aR[q];
-                l2[i] = aL[q];
-                g2[i] = aG[q];
-            }
-        }
+            sigma_elem<L, NT>(s, tw, d, ji, ij, ix, ik, Gr, Gl, Gg, Wr, Wl, Wg, dE, Sr, Sl, Sg, true);
     }
 }
 
