@@ -564,9 +564,12 @@ InvArgs members(const InvArgs& a, int b0, int cnt) {
 }
 
 // a second stream (default priority) and two events for splitting a batch
+// up to three more streams (default priority) and their events for splitting
+// a batch; QT_INV_PARTS (2..4, default 2) sets the number of parts
+constexpr int kMaxParts = 4;
 struct Side {
-    hipStream_t s = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t s[kMaxParts - 1] = {};
+    hipEvent_t fork = nullptr, join[kMaxParts - 1] = {};
     int device = -1;
 };
 Side& side() {
@@ -574,20 +577,30 @@ Side& side() {
     int dev = 0;
     QT_HIP(hipGetDevice(&dev));
     if (sd.device != dev) {
-        QT_HIP(hipStreamCreate(&sd.s));
         QT_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
-        QT_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
+        for (int i = 0; i < kMaxParts - 1; ++i) {
+            QT_HIP(hipStreamCreate(&sd.s[i]));
+            QT_HIP(hipEventCreateWithFlags(&sd.join[i], hipEventDisableTiming));
+        }
         sd.device = dev;
     }
     return sd;
+}
+int inv_parts() {
+    static const int v = [] {
+        const char* e = std::getenv("QT_INV_PARTS");
+        const int p = e ? std::atoi(e) : 2;
+        return p < 2 ? 2 : (p > kMaxParts ? kMaxParts : p);
+    }();
+    return v;
 }
 
 }  // namespace
 
 // The panel kernel keeps one workgroup per member busy for a whole latency-
 // bound column sweep, leaving most CUs idle; the batch is therefore split in
-// two halves on two streams so that the panels of one half run while the
-// MFMA updates of the other half fill the remaining CUs.
+// parts on as many streams so that the panels of one part run while the
+// MFMA updates of the others fill the remaining CUs.
 void zinv(hipStream_t s, const InvArgs& a, Timer* t) {
     const int n = a.n;
     if (n <= 0 || a.batch <= 0) return;
@@ -595,13 +608,19 @@ void zinv(hipStream_t s, const InvArgs& a, Timer* t) {
     if (t && t->enabled) t->begin(s, 1);
     if (a.batch >= 8 && n >= 64) {
         Side& sd = side();
-        const int h0 = a.batch / 2;
+        const int parts = std::min(inv_parts(), a.batch / 4);
         QT_HIP(hipEventRecord(sd.fork, s));
-        QT_HIP(hipStreamWaitEvent(sd.s, sd.fork, 0));
-        zinv_part(sd.s, members(a, h0, a.batch - h0));
-        zinv_part(s, members(a, 0, h0));
-        QT_HIP(hipEventRecord(sd.join, sd.s));
-        QT_HIP(hipStreamWaitEvent(s, sd.join, 0));
+        // part p holds members [p*batch/parts, (p+1)*batch/parts); part 0 stays on s
+        for (int p = 1; p < parts; ++p) {
+            const int b0 = p * a.batch / parts, b1 = (p + 1) * a.batch / parts;
+            QT_HIP(hipStreamWaitEvent(sd.s[p - 1], sd.fork, 0));
+            zinv_part(sd.s[p - 1], members(a, b0, b1 - b0));
+        }
+        zinv_part(s, members(a, 0, a.batch / parts));
+        for (int p = 1; p < parts; ++p) {
+            QT_HIP(hipEventRecord(sd.join[p - 1], sd.s[p - 1]));
+            QT_HIP(hipStreamWaitEvent(s, sd.join[p - 1], 0));
+        }
     } else {
         zinv_part(s, a);
     }
