@@ -144,6 +144,40 @@ def run_gw(args) -> dict:
     }
 
 
+def run_vendor_zgemm(args) -> dict:
+    """The vendor library's batched complex128 GEMM (torch.bmm, which calls
+    rocBLAS) at the sweep's product shape, for comparison with the
+    ``roofline.achieved`` of the default RGF run (our batched GEMM kernel).
+    Not a path of this library: a reference point only."""
+    import torch
+    dev = torch.device("cuda:0")
+    B, n = args.batch, args.nb
+    g = torch.Generator(device=dev).manual_seed(args.seed)
+    A = torch.randn(B, n, n, dtype=torch.complex128, device=dev, generator=g)
+    X = torch.randn(B, n, n, dtype=torch.complex128, device=dev, generator=g)
+    C = torch.empty_like(A)
+    for _ in range(max(args.warmup, 1)):
+        torch.bmm(A, X, out=C)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(args.steps, 1) * 4
+    e0.record()
+    for _ in range(reps):
+        torch.bmm(A, X, out=C)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    tf = 8.0 * n ** 3 * B / (ms * 1e-3) / 1e12
+    return {"metric": f"vendor batched ZGEMM TFLOP/s @{B}x{n}^3 (torch.bmm -> rocBLAS)",
+            "value": round(tf, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": reps,
+            "warmup": max(args.warmup, 1), "ms_per_step": round(ms, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (torch normal samples)",
+            "config": {"workload": f"complex128 batched GEMM, {B} members of {n}^3", "batch": B, "n": n},
+            "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
+                         "frac": round(tf / 78.6, 4), "traffic": None, "kernel": "rocBLAS zgemm (torch.bmm)"},
+            "cpu_baseline": None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,8 +189,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1002)
-    ap.add_argument("--workload", choices=("rgf", "gw"), default="rgf",
-                    help="rgf: the headline RGF sweep; gw: the GW energy-axis stage")
+    ap.add_argument("--workload", choices=("rgf", "gw", "vendor-zgemm"), default="rgf",
+                    help="rgf: the headline RGF sweep; gw: the GW energy-axis stage; "
+                         "vendor-zgemm: rocBLAS batched ZGEMM at the sweep's shape, for comparison")
     ap.add_argument("--gw-slabs", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -167,6 +202,9 @@ def main() -> int:
 
     if args.workload == "gw":
         print(json.dumps(run_gw(args)), flush=True)
+        return 0
+    if args.workload == "vendor-zgemm":
+        print(json.dumps(run_vendor_zgemm(args)), flush=True)
         return 0
 
     from quantum_transport_amd import hamiltonian as hm
