@@ -178,6 +178,92 @@ def run_vendor_zgemm(args) -> dict:
             "cpu_baseline": None}
 
 
+def time_batches(rgf, H, Eall, Kall, B: int, nd: int, steps: int, warmup: int,
+                 coupling_fix: bool = False):
+    """Run ``warmup`` untimed and ``steps`` timed device batches of B points
+    (outputs stay on the device); returns the wall time of the timed batches
+    and the summed kernel timings."""
+    bufs = {"tr": rgf.DeviceBuffer(8 * B), "tre": rgf.DeviceBuffer(8 * B)}
+    for key in ("gr_diag", "gl_diag", "gg_diag"):
+        bufs[key] = rgf.DeviceBuffer(16 * B * nd)
+    cursor = [0]
+
+    def step():
+        i0 = cursor[0] % len(Eall)
+        idx = (np.arange(B) + i0) % len(Eall)
+        cursor[0] += B
+        rgf.solve_batch_device(H, Eall[idx], Kall[idx], bufs, mul=0.0, mur=-0.1,
+                               coupling_fix=coupling_fix)
+
+    for _ in range(warmup):
+        step()
+    rgf.lib().qt_synchronize()
+    rgf.set_kernel_timing(True)
+    acc = {"gemm_ms": 0.0, "gemm_flops": 0.0, "inv_ms": 0.0, "inv_flops": 0.0, "gemm_launches": 0,
+           "lead_iterations": 0, "lead_ms": 0.0, "forward_ms": 0.0, "backward_ms": 0.0}
+    rgf.lib().qt_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        t = rgf.last_timing()
+        for k in acc:
+            acc[k] += t[k]
+    rgf.lib().qt_synchronize()
+    elapsed = time.perf_counter() - t0
+    rgf.set_kernel_timing(False)
+    for b in bufs.values():
+        b.free()
+    return elapsed, acc
+
+
+def run_uneven(args, hm, rgf) -> dict:
+    """Uneven slabs: 80 slabs with sizes drawn uniformly from {128, 192, ..., 768}
+    (direct block construction), 2048 energies in [-2, 2] eV x 8 kz on one
+    k-point set (the 8 kz replicate the same blocks), plain FP64."""
+    rng = np.random.default_rng(args.seed)
+    sizes = [int(v) for v in rng.choice(np.arange(128, 769, 64), size=80)]
+    dev = hm.uneven_device(sizes, args.seed)
+    grid = hm.energy_grid(-2.0, 2.0, 2048)
+    Eall = np.repeat(grid, 8)
+    Kall = np.zeros(len(Eall), np.int32)
+    rgf.init(0)
+    H = rgf.Hamiltonian.from_device(dev)
+    nd = int(sum(sizes))
+    B = args.batch
+    # uneven slabs need the coupling of the slab sizes (the reference indexes
+    # H_{i,i+1} one slab off in its second pass, which only works for equal sizes)
+    elapsed, acc = time_batches(rgf, H, Eall, Kall, B, nd, args.steps, args.warmup, coupling_fix=True)
+    H.close()
+    points = B * args.steps
+    gemm_tf = acc["gemm_flops"] / (acc["gemm_ms"] * 1e-3) / 1e12 if acc["gemm_ms"] > 0 else 0.0
+    inv_tf = acc["inv_flops"] / (acc["inv_ms"] * 1e-3) / 1e12 if acc["inv_ms"] > 0 else 0.0
+    K = acc["lead_iterations"] / max(1, 2 * points)
+    # 120 n^3 per slab with n^3 -> (n_i n_{i-1} terms); leads at the end slab sizes
+    sweep = sum(120.0 * float(n) ** 3 for n in sizes)
+    lead = (K * 56 + 24) * (float(sizes[0]) ** 3 + float(sizes[-1]) ** 3)
+    return {
+        "metric": "RGF energy-points/s, uneven slabs {128..768} x 80, FP64",
+        "value": round(points / elapsed, 4), "unit": "energy points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 blocks, numpy-drawn slab sizes)",
+        "config": {"workload": f"ballistic RGF, 80 uneven slabs (sum n = {nd}), 8 kz x 2048 E in "
+                               f"[-2,2] eV, G^R/G^</G^> diagonals + contact currents, plain FP64",
+                   "slabs": 80, "sum_n": nd, "points_per_step": B, "precision": "fp64"},
+        "roofline": {"bound": "mfma", "achieved": round(gemm_tf, 3), "peak": PEAK_FP64,
+                     "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4), "traffic": None,
+                     "kernel": "zgemm_mfma_kernel (batched complex128 GEMM)",
+                     "launches": acc["gemm_launches"]},
+        "path": {"lead_iterations_avg": round(K, 2),
+                 "algorithmic_tflops_approx": round(points * (sweep + lead) / elapsed / 1e12, 3),
+                 "inversion_tflops": round(inv_tf, 3),
+                 "gemm_share_of_wall": round(acc["gemm_ms"] / (1e3 * elapsed), 3),
+                 "inversion_share_of_wall": round(acc["inv_ms"] / (1e3 * elapsed), 3)},
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,8 +275,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1002)
-    ap.add_argument("--workload", choices=("rgf", "gw", "vendor-zgemm"), default="rgf",
-                    help="rgf: the headline RGF sweep; gw: the GW energy-axis stage; "
+    ap.add_argument("--workload", choices=("rgf", "uneven", "gw", "vendor-zgemm"), default="rgf",
+                    help="rgf: the headline RGF sweep; uneven: 80 slabs of 128..768 orbitals; "
+                         "gw: the GW energy-axis stage; "
                          "vendor-zgemm: rocBLAS batched ZGEMM at the sweep's shape, for comparison")
     ap.add_argument("--gw-slabs", type=int, default=2)
     args = ap.parse_args()
@@ -210,6 +297,10 @@ def main() -> int:
     from quantum_transport_amd import hamiltonian as hm
     from quantum_transport_amd import rgf
 
+    if args.workload == "uneven":
+        print(json.dumps(run_uneven(args, hm, rgf)), flush=True)
+        return 0
+
     nb, nx, B = args.nb, args.slabs, args.batch
     kz = [0.0, np.pi / 4, np.pi / 2, 3 * np.pi / 4]
     w = hm.synthetic_wannier(nb, args.seed, tz_density=0.02)
@@ -221,35 +312,7 @@ def main() -> int:
     rgf.init(0)
     H = rgf.Hamiltonian.from_device(dev)
     del dev
-    nd = nb * nx
-    bufs = {"tr": rgf.DeviceBuffer(8 * B), "tre": rgf.DeviceBuffer(8 * B)}
-    for key in ("gr_diag", "gl_diag", "gg_diag"):
-        bufs[key] = rgf.DeviceBuffer(16 * B * nd)
-
-    cursor = [0]
-
-    def step():
-        i0 = cursor[0] % len(Eall)
-        idx = (np.arange(B) + i0) % len(Eall)
-        cursor[0] += B
-        rgf.solve_batch_device(H, Eall[idx], Kall[idx], bufs, mul=0.0, mur=-0.1)
-
-    for _ in range(args.warmup):
-        step()
-    rgf.lib().qt_synchronize()
-    rgf.set_kernel_timing(True)
-    acc = {"gemm_ms": 0.0, "gemm_flops": 0.0, "inv_ms": 0.0, "inv_flops": 0.0, "gemm_launches": 0,
-           "lead_iterations": 0, "lead_ms": 0.0, "forward_ms": 0.0, "backward_ms": 0.0}
-    rgf.lib().qt_synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        t = rgf.last_timing()
-        for k in acc:
-            acc[k] += t[k]
-    rgf.lib().qt_synchronize()
-    elapsed = time.perf_counter() - t0
-    rgf.set_kernel_timing(False)
+    elapsed, acc = time_batches(rgf, H, Eall, Kall, B, nb * nx, args.steps, args.warmup)
     points = B * args.steps
     value = points / elapsed
 
@@ -290,8 +353,6 @@ def main() -> int:
                  "backward_ms": round(acc["backward_ms"], 1)},
         "cpu_baseline": None,
     }
-    for b in bufs.values():
-        b.free()
     H.close()
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
