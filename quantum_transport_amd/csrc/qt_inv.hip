@@ -228,21 +228,13 @@ __global__ __launch_bounds__(NT) void gj_panel_loop_kernel(double2* A, int64_t s
             red_i[buf][wave] = wave * 64 + (__ffsll((long long)hit) - 1);
         }
         __syncthreads();
-        double rv[NW];
-        int ri[NW];
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            rv[w] = red_v[buf][w];
-            ri[w] = red_i[buf][w];
-        }
-        double bv = rv[0];
-        int p = ri[0];
-#pragma unroll
-        for (int w = 1; w < NW; ++w) {
-            const bool take = rv[w] > bv || (rv[w] == bv && ri[w] < p);
-            bv = take ? rv[w] : bv;
-            p = take ? ri[w] : p;
-        }
+        // lane w < NW takes wave w's candidate; the first lane holding the
+        // maximum has the smallest row (wave w's rows are 64w..64w+63)
+        const double rvv = lane < NW ? red_v[buf][lane] : -2.0;
+        const int riv = lane < NW ? red_i[buf][lane] : 0;
+        const double bv = wave_max(rvv);
+        const unsigned long long hitw = __ballot(rvv == bv);
+        int p = __builtin_amdgcn_readlane(riv, __ffsll((long long)hitw) - 1);
         const int j = kb + c;
         if (p < 0 || p >= n) p = 0;  // unreachable: n - j rows are always unpivoted
         if (r == p) {
@@ -545,6 +537,8 @@ void zinv_part(hipStream_t s, const InvArgs& a) {
         run_panels<32, 256>(s, a);
     else if (n <= 512)
         run_panels<32, 512>(s, a);
+    else if (n <= 768)
+        run_panels<32, 768>(s, a);  // 3 waves per SIMD leave room for the 32-column panel
     else
         run_panels<16, 1024>(s, a);
     hipLaunchKernelGGL(gj_finish_kernel, dim3((n + 7) / 8, a.batch), dim3(256), 0, s, a.A, a.sa, a.X,
