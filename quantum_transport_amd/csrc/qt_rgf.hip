@@ -78,10 +78,6 @@ struct Ctx {
     std::mutex mu;
     int* host_count = nullptr;  // pinned
     hipEvent_t ev[4];
-    // second stream (default priority) for the G^< products of pass 1, which
-    // overlap the next slab's Schur complement and inversion
-    hipStream_t lane = nullptr;
-    hipEvent_t ev_inv = nullptr, ev_lane = nullptr;
 };
 
 static Ctx* g_ctx = nullptr;
@@ -102,15 +98,6 @@ static ZOp op_k(const double2* p, int64_t ld, int64_t sk) {
     o.ld = ld;
     o.sk = sk;
     return o;
-}
-
-// QT_OVERLAP=0 keeps pass 1 on one stream; read once
-static bool lane_overlap() {
-    static const bool v = [] {
-        const char* e = std::getenv("QT_OVERLAP");
-        return !(e && e[0] == '0');
-    }();
-    return v;
 }
 
 struct Gemm {
@@ -710,9 +697,6 @@ struct Solve {
             g(OP_N, OP_C, n, n, n, op_member(T[2], n, nn), opGl(sf), gln(sf), n, sq(sf));
         }
         // ---------------- pass 1, interior slabs (rgf_mod.f90:113-136 / 330-353)
-        const bool overlap = lane_overlap();
-        Gemm g2{c.lane, &c.timer, B, w.kidx};
-        bool lane_used = false;
         int sp = sf;
         for (int sc = sf + step; sc != sl; sc += step) {
             const int n = h.nm[sc], np = h.nm[sp];
@@ -741,27 +725,13 @@ struct Solve {
                 g(OP_N, OP_C, n, n, n, op_member(T[4], n, nn), opGl(sc), gln(sc), n, sq(sc));
             } else {
                 // no interior injection: Gln = Gl U Gln' U^H Gl^H = V Gln' V^H with
-                // V = Gl U, three products instead of four.  They run on the
-                // second stream: they read Gl(sc), Gln(sp) and write only T2, T3
-                // and Gln(sc), none of which the next slab's Schur complement and
-                // inversion (T0, T1, Gl(sc+1)) touch, so they fill the GPU while
-                // that inversion's latency-bound panels run.
-                const Gemm& gg = overlap ? g2 : g;
-                if (overlap) {
-                    QT_HIP(hipEventRecord(c.ev_inv, s));
-                    QT_HIP(hipStreamWaitEvent(c.lane, c.ev_inv, 0));
-                    lane_used = true;
-                }
-                gg(OP_N, OP_N, n, np, n, opGl(sc), U, T[2], n, nn);      // V = Gl U
-                gg(OP_N, OP_N, n, np, np, op_member(T[2], n, nn), opGln(sp), T[3], n, nn);
-                gg(OP_N, OP_C, n, n, np, op_member(T[3], n, nn), op_member(T[2], n, nn), gln(sc), n,
-                   sq(sc));                                              // (V Gln') V^H
+                // V = Gl U, three products instead of four
+                g(OP_N, OP_N, n, np, n, opGl(sc), U, T[2], n, nn);       // V = Gl U
+                g(OP_N, OP_N, n, np, np, op_member(T[2], n, nn), opGln(sp), T[3], n, nn);
+                g(OP_N, OP_C, n, n, np, op_member(T[3], n, nn), op_member(T[2], n, nn), gln(sc), n,
+                  sq(sc));                                               // (V Gln') V^H
             }
             sp = sc;
-        }
-        if (lane_used) {  // the last slab reads Gln(sp) and reuses T2, T3
-            QT_HIP(hipEventRecord(c.ev_lane, c.lane));
-            QT_HIP(hipStreamWaitEvent(s, c.ev_lane, 0));
         }
         // ---------------- last slab of pass 1 (rgf_mod.f90:172-202 / 385-415)
         double2* Gc = T[6];   // G of the slab just solved (n x n per point, stride nn)
@@ -987,9 +957,6 @@ int qt_init(int device) {
         Ctx* c = new Ctx();
         c->device = device;
         QT_HIP(hipStreamCreate(&c->stream));
-        QT_HIP(hipStreamCreate(&c->lane));
-        QT_HIP(hipEventCreateWithFlags(&c->ev_inv, hipEventDisableTiming));
-        QT_HIP(hipEventCreateWithFlags(&c->ev_lane, hipEventDisableTiming));
         QT_HIP(hipHostMalloc((void**)&c->host_count, sizeof(int), hipHostMallocDefault));
         for (int i = 0; i < 4; ++i) QT_HIP(hipEventCreate(&c->ev[i]));
         g_ctx = c;
@@ -1010,11 +977,7 @@ int qt_finish(void) {
         for (int k = 0; k < 2; ++k)
             for (int i = 0; i < 4096; ++i) (void)hipEventDestroy(c->timer.ev[k][i]);
     for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c->ev[i]);
-    (void)hipEventDestroy(c->ev_inv);
-    (void)hipEventDestroy(c->ev_lane);
     (void)hipHostFree(c->host_count);
-    (void)hipStreamSynchronize(c->lane);
-    (void)hipStreamDestroy(c->lane);
     (void)hipStreamDestroy(c->stream);
     delete c;
     g_ctx = nullptr;
