@@ -94,6 +94,28 @@ struct Spec {
     }
 };
 
+// The twiddle table is copied into LDS behind the transform buffer (L
+// entries; 64 KB at L = 4096, which the one-workgroup-per-CU occupancy of
+// these kernels leaves free), so the passes read it with LDS latency.
+#ifndef QT_GW_TW_LDS
+#define QT_GW_TW_LDS 1
+#endif
+template <int L>
+constexpr size_t gw_lds_elems() {
+    return (size_t)fft::padded_len(L) + (QT_GW_TW_LDS ? (size_t)L : 0);
+}
+template <int L, int NT>
+__device__ __forceinline__ const double2* stage_twiddles(double2* s, const double2* __restrict__ tw) {
+#if QT_GW_TW_LDS
+    double2* t = s + fft::padded_len(L);
+    for (int i = threadIdx.x; i < L; i += NT) t[i] = tw[i];
+    __syncthreads();
+    return t;
+#else
+    return tw;
+#endif
+}
+
 // radix of the FFT passes (8: register budget of the assembly kernels)
 #ifndef QT_GW_RADIX
 #define QT_GW_RADIX 8
@@ -179,6 +201,7 @@ __global__ __launch_bounds__(NT) void gw_pol_kernel(GwDims d, const int* __restr
     extern __shared__ double2 s[];
     const int w = blockIdx.x;
     if (w >= nelem * d.nx) return;
+    tw = stage_twiddles<L, NT>(s, tw);
     const int e = w / d.nx, ix = w % d.nx;
     const int ij = elems[e];
     const int row = ij % d.nm, col = ij / d.nm;
@@ -412,6 +435,7 @@ __global__ __launch_bounds__(NT) void gw_sigma_kernel(GwDims d, const int* __res
     extern __shared__ double2 s[];
     const int w = blockIdx.x;
     if (w >= npair * d.nx) return;
+    tw = stage_twiddles<L, NT>(s, tw);
     const int p = w / d.nx, ix = w % d.nx;
     const int ij = pairs[p];
     const int row = ij % d.nm, col = ij / d.nm;
@@ -440,7 +464,7 @@ template <int L>
 void launch_pol(hipStream_t st, const GwDims& d, const int* elems, int nelem, const double2* Gr,
                 const double2* Gl, const double2* Gg, double2* Pr, double2* Pl, double2* Pg, double2 dE) {
     constexpr int NT = GwCfg<L>::NT;
-    const size_t lds = sizeof(double2) * fft::padded_len(L);
+    const size_t lds = sizeof(double2) * gw_lds_elems<L>();
     static bool attr = false;
     if (!attr) {
         QT_HIP(hipFuncSetAttribute((const void*)gw_pol_kernel<L, NT>,
@@ -459,7 +483,7 @@ void launch_sigma(hipStream_t st, const GwDims& d, const int* pairs, int npair, 
                   const double2* Gl, const double2* Gg, const double2* Wr, const double2* Wl,
                   const double2* Wg, double2* Sr, double2* Sl, double2* Sg, double2 dE) {
     constexpr int NT = GwCfg<L>::NT;
-    const size_t lds = sizeof(double2) * fft::padded_len(L);
+    const size_t lds = sizeof(double2) * gw_lds_elems<L>();
     static bool attr = false;
     if (!attr) {
         QT_HIP(hipFuncSetAttribute((const void*)gw_sigma_kernel<L, NT>,
