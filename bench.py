@@ -144,6 +144,53 @@ def run_gw(args) -> dict:
     }
 
 
+def run_corrconv(args) -> dict:
+    """Batched corr1d2 and conv1d2 (fft_mod.f90:58-91, 129-168) on device-resident
+    series: 65536 series pairs of 2048 energies (FFT length 4096).  Algorithmic
+    HBM bytes per series: two inputs and one output of n complex128 samples."""
+    import torch
+    from quantum_transport_amd import rgf
+
+    n, ns = 2048, 65536
+    rgf.init(0)
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    X = torch.randn(ns * n, dtype=torch.complex128, device=dev, generator=g)
+    Y = torch.randn(ns * n, dtype=torch.complex128, device=dev, generator=g)
+    Z = torch.empty(ns * n, dtype=torch.complex128, device=dev)
+    torch.cuda.synchronize()
+    L = rgf.lib()
+    res = {}
+    for mode, fn in (("corr", L.qt_corr1d2_batched), ("conv", L.qt_conv1d2_batched)):
+        def call():
+            rgf.check(fn(n, ns, X.data_ptr(), Y.data_ptr(), Z.data_ptr(), rgf.QT_F_DEVICE_OUTPUTS))
+        for _ in range(max(args.warmup, 1)):
+            call()
+        L.qt_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            call()
+        L.qt_synchronize()
+        res[mode] = (time.perf_counter() - t0) / args.steps
+    bytes_call = 3.0 * ns * n * 16
+    bw = {m: bytes_call / t / 1e9 for m, t in res.items()}
+    return {"metric": "batched corr1d2 series/s @n=2048 (FFT length 4096)",
+            "value": round(ns / res["corr"], 1), "unit": "series/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": max(args.warmup, 1),
+            "ms_per_step": round(1e3 * res["corr"], 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (torch normal samples)",
+            "config": {"workload": f"corr1d2 / conv1d2, {ns} series pairs of {n} samples", "n": n,
+                       "series": ns},
+            "roofline": {"bound": "hbm", "achieved": round(bw["corr"], 1), "peak": 8000.0, "unit": "GB/s",
+                         "frac": round(bw["corr"] / 8000.0, 4), "traffic": None,
+                         "kernel": "corrconv_kernel (corr1d2)"},
+            "stages": {"corr_ms": round(1e3 * res["corr"], 3), "conv_ms": round(1e3 * res["conv"], 3),
+                       "conv_GBps": round(bw["conv"], 1)},
+            "cpu_baseline": None}
+
+
 def run_vendor_zgemm(args) -> dict:
     """The vendor library's batched complex128 GEMM (torch.bmm, which calls
     rocBLAS) at the sweep's product shape, for comparison with the
@@ -275,9 +322,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1002)
-    ap.add_argument("--workload", choices=("rgf", "uneven", "gw", "vendor-zgemm"), default="rgf",
+    ap.add_argument("--workload", choices=("rgf", "uneven", "gw", "corrconv", "vendor-zgemm"),
+                    default="rgf",
                     help="rgf: the headline RGF sweep; uneven: 80 slabs of 128..768 orbitals; "
-                         "gw: the GW energy-axis stage; "
+                         "gw: the GW energy-axis stage; corrconv: batched corr1d2/conv1d2; "
                          "vendor-zgemm: rocBLAS batched ZGEMM at the sweep's shape, for comparison")
     ap.add_argument("--gw-slabs", type=int, default=2)
     args = ap.parse_args()
@@ -289,6 +337,9 @@ def main() -> int:
 
     if args.workload == "gw":
         print(json.dumps(run_gw(args)), flush=True)
+        return 0
+    if args.workload == "corrconv":
+        print(json.dumps(run_corrconv(args)), flush=True)
         return 0
     if args.workload == "vendor-zgemm":
         print(json.dumps(run_vendor_zgemm(args)), flush=True)
