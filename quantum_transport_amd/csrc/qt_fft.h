@@ -183,6 +183,155 @@ __device__ __forceinline__ void transform(double2* s, const double2* __restrict_
     passes<L, NT, 1, INV, RMAX>(s, tw);
 }
 
+// ---- transforms with fused input / output
+// A thread's own samples are i = tid + q NT, q < L / NT.  When every pass
+// has at least NT butterflies (L / R >= NT for its radix R), the first pass
+// reads exactly the thread's samples (j = tid + b NT, inputs j + r L/R,
+// q = b + r PER) and the last pass (NS = L / R) produces them, so the
+// inputs can come straight from registers or global memory and the outputs
+// go straight to registers or global memory, saving an LDS round trip and a
+// barrier at each end.  Loaders are called as ld(i, q), stores as
+// st(i, q, value); i is the natural sample index.
+
+// first pass (NS = 1: no twiddles), inputs from ld
+template <int L, int NT, int R, bool INV, class Ld>
+__device__ __forceinline__ void pass_first(double2* s, Ld&& ld) {
+    constexpr int NBF = L / R;
+    constexpr int PER = NBF >= NT ? NBF / NT : 1;
+    double2 v[PER][R];
+    const int tid = opaque_tid();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = tid + b * NT;
+        if (NBF >= NT || j < NBF) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = ld(j + r * NBF, b + r * PER);
+        }
+    }
+    __syncthreads();  // earlier readers of s are done
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = tid + b * NT;
+        if (NBF >= NT || j < NBF) {
+            dft<R, INV>(v[b]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[pad(j * R + r)] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// last pass (NS R = L, so k = j and the outputs are j + r NS), outputs to st;
+// s is only read
+template <int L, int NT, int R, int NS, bool INV, class St>
+__device__ __forceinline__ void pass_last(const double2* s, const double2* __restrict__ tw, St&& st) {
+    static_assert(NS * R == L, "last pass");
+    constexpr int NBF = L / R;
+    constexpr int PER = NBF >= NT ? NBF / NT : 1;
+    double2 v[PER][R];
+    const int tid = opaque_tid();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = tid + b * NT;
+        if (NBF >= NT || j < NBF) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = s[pad(j + r * NBF)];
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = tid + b * NT;
+        if (NBF >= NT || j < NBF) {
+            if constexpr (NS > 1) {
+                constexpr int step = L / (NS * R);
+                double2 wp[4];
+#pragma unroll
+                for (int bit = 0; (1 << bit) < R; ++bit) wp[bit] = tw[(j << bit) * step];
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    double2 w = make_double2(1.0, 0.0);
+                    bool first = true;
+#pragma unroll
+                    for (int bit = 0; (1 << bit) < R; ++bit)
+                        if (r & (1 << bit)) {
+                            w = first ? wp[bit] : cmul(w, wp[bit]);
+                            first = false;
+                        }
+                    v[b][r] = INV ? cmulc(v[b][r], w) : cmul(v[b][r], w);
+                }
+            }
+            dft<R, INV>(v[b]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) st(j + r * NS, b + r * PER, v[b][r]);
+        }
+    }
+}
+
+template <int L, int NT, int NS, bool INV, int RMAX, class St>
+__device__ __forceinline__ void passes_to(double2* s, const double2* __restrict__ tw, St&& st) {
+    constexpr int rem = L / NS;
+    constexpr int R = rem >= RMAX ? RMAX : rem;
+    if constexpr (NS * R == L) {
+        pass_last<L, NT, R, NS, INV>(s, tw, st);
+    } else {
+        pass<L, NT, R, NS, INV>(s, tw);
+        passes_to<L, NT, NS * R, INV, RMAX>(s, tw, st);
+    }
+}
+
+// radix of the last pass of transform<L, NT, INV, RMAX>
+template <int L, int RMAX, int NS = 1>
+constexpr int last_radix() {
+    if constexpr (L / NS <= RMAX) return L / NS;
+    else return last_radix<L, RMAX, NS * RMAX>();
+}
+// true when the first and last passes cover exactly the threads' own samples
+template <int L, int NT, int RMAX>
+constexpr bool fused_io() {
+    return L > RMAX && L / RMAX >= NT && L / last_radix<L, RMAX>() >= NT;
+}
+
+// transform of the samples ld(i, q) left in s (natural order); ld is only
+// asked for global-memory style values (its q may be ignored)
+template <int L, int NT, bool INV, int RMAX, class Ld>
+__device__ __forceinline__ void transform_load(double2* s, const double2* __restrict__ tw, Ld&& ld) {
+    constexpr int R0 = L >= RMAX ? RMAX : L;
+    if constexpr (R0 < L) {
+        pass_first<L, NT, R0, INV>(s, ld);
+        passes<L, NT, R0, INV, RMAX>(s, tw);
+    } else {
+        __syncthreads();
+        for (int i = threadIdx.x; i < L; i += NT) s[pad(i)] = ld(i, -1);
+        __syncthreads();
+        transform<L, NT, INV, RMAX>(s, tw);
+    }
+}
+
+// transform of the thread's samples ld(i, q) with the results handed to
+// st(i, q, value) for the thread's own samples; s is scratch
+template <int L, int NT, bool INV, int RMAX, class Ld, class St>
+__device__ __forceinline__ void transform_io(double2* s, const double2* __restrict__ tw, Ld&& ld, St&& st) {
+    if constexpr (fused_io<L, NT, RMAX>()) {
+        pass_first<L, NT, RMAX, INV>(s, ld);
+        passes_to<L, NT, RMAX, INV, RMAX>(s, tw, st);
+    } else {
+        constexpr int PER = L >= NT ? L / NT : 1;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * NT;
+            if (L >= NT || i < L) s[pad(i)] = ld(i, q);
+        }
+        __syncthreads();
+        transform<L, NT, INV, RMAX>(s, tw);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int i = threadIdx.x + q * NT;
+            if (L >= NT || i < L) st(i, q, s[pad(i)]);
+        }
+    }
+}
+
 // threads per workgroup used for length L
 __host__ __device__ constexpr int threads_for(int L) { return L >= 4096 ? 256 : (L >= 32 ? (L / 16 > 64 ? L / 16 : 64) : 64); }
 

@@ -57,24 +57,6 @@ __device__ __forceinline__ double2* series(double2* A, const GwDims& d, int ij, 
     return A + (((int64_t)ij * d.nx + ix) * d.nk + ik) * (int64_t)d.nen;
 }
 
-// s <- Yr (+ Yr of y2 when given), Yr[l mod L] = Y[l + n/2]  (conv)
-template <int L, int NT>
-__device__ __forceinline__ void load_rot(double2* s, const double2* __restrict__ y,
-                                         const double2* __restrict__ y2, int n) {
-    const int h = n / 2;
-    for (int i = threadIdx.x; i < L; i += NT) {
-        int src = -1;
-        if (i < n - h) src = i + h;
-        else if (i >= L - h) src = i - (L - h);
-        double2 v = make_double2(0.0, 0.0);
-        if (src >= 0) {
-            v = y[src];
-            if (y2) v = cadd(v, y2[src]);
-        }
-        s[pad(i)] = v;
-    }
-}
-
 // the spectrum samples i = tid + q*NT (i < L) of one thread
 template <int L, int NT>
 struct Spec {
@@ -84,13 +66,6 @@ struct Spec {
     __device__ __forceinline__ void zero() {
 #pragma unroll
         for (int q = 0; q < PER; ++q) v[q] = make_double2(0.0, 0.0);
-    }
-    __device__ __forceinline__ void put(double2* s) const {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int i = threadIdx.x + q * NT;
-            if (ok(i)) s[pad(i)] = v[q];
-        }
     }
 };
 
@@ -123,24 +98,20 @@ __device__ __forceinline__ const double2* stage_twiddles(double2* s, const doubl
 constexpr int RADIX = QT_GW_RADIX;
 
 // inverse of acc, scaled by c / L, lags written to z (corr) or samples (conv)
+// (the inverse reads acc from registers and writes z directly where the
+// transform allows it)
 template <int L, int NT>
 __device__ __forceinline__ void finish(double2* s, const double2* tw, const Spec<L, NT>& acc,
                                        double2 c, int n, bool corr, double2* z) {
-    __syncthreads();
-    acc.put(s);
-    __syncthreads();
-    fft::transform<L, NT, true, RADIX>(s, tw);
     const double inv = 1.0 / (double)L;
     const int h = n / 2;
-    for (int i = threadIdx.x; i < n; i += NT) {
-        int src = i;
-        if (corr) {
-            const int l = i - h;
-            src = l >= 0 ? l : l + L;
-        }
-        const double2 v = s[pad(src)];
-        z[i] = make_double2((c.x * v.x - c.y * v.y) * inv, (c.x * v.y + c.y * v.x) * inv);
-    }
+    fft::transform_io<L, NT, true, RADIX>(
+        s, tw, [&](int, int q) { return acc.v[q]; },
+        [&](int m, int, double2 v) {
+            // sample m is lag m (corr: output i = m + n/2 mod L) or output m (conv)
+            const int i = corr ? (m + h) & (L - 1) : m;
+            if (i < n) z[i] = make_double2((c.x * v.x - c.y * v.y) * inv, (c.x * v.y + c.y * v.x) * inv);
+        });
 }
 
 // ---------------------------------------------------------------- packed transforms
@@ -149,23 +120,13 @@ __device__ __forceinline__ void finish(double2* s, const double2* tw, const Spec
 //   F(i y1)[k] = i (Z[k] + conj Z[-k]) / 2,   F(i y2)[k] = (Z[k] - conj Z[-k]) / 2,
 // and the flipped operand of a correlation needs no transform of its own,
 // since F(Yf)[k] = F(Y)[-k].
-template <int L, int NT>
-__device__ __forceinline__ void load_pack(double2* s, const double2* __restrict__ y1,
-                                          const double2* __restrict__ y2, int n) {
-    for (int i = threadIdx.x; i < L; i += NT) {
-        double2 v = make_double2(0.0, 0.0);
-        if (i < n) v = make_double2(y1[i].y, y2 ? y2[i].y : 0.0);
-        s[pad(i)] = v;
-    }
-}
 // s <- F(Im y1 + i Im y2)
 template <int L, int NT>
 __device__ __forceinline__ void fwd_pack(double2* s, const double2* tw, const double2* y1, const double2* y2,
                                          int n) {
-    __syncthreads();
-    load_pack<L, NT>(s, y1, y2, n);
-    __syncthreads();
-    fft::transform<L, NT, false, RADIX>(s, tw);
+    fft::transform_load<L, NT, false, RADIX>(s, tw, [&](int i, int) {
+        return i < n ? make_double2(y1[i].y, y2 ? y2[i].y : 0.0) : make_double2(0.0, 0.0);
+    });
 }
 // Z[k] and Z[-k] of the thread's sample q
 template <int L, int NT>
@@ -235,24 +196,25 @@ __global__ __launch_bounds__(NT) void gw_pol_kernel(GwDims d, const int* __restr
                 accg.v[q] = cadd(accg.v[q], cmul(spec2(zk, zm), bl[q]));
             }
         }
-        // P^< (real part) and P^> (imaginary part) from one inverse transform
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int i = threadIdx.x + q * NT;
-            if (S::ok(i)) s[pad(i)] = make_double2(accl.v[q].x - accg.v[q].y, accl.v[q].y + accg.v[q].x);
-        }
-        __syncthreads();
-        fft::transform<L, NT, true, RADIX>(s, tw);
-        const double inv = 1.0 / (double)L;
-        const int h = n / 2;
-        double2* zl = series(Pl, d, ij, ix, iq);
-        double2* zg = series(Pg, d, ij, ix, iq);
-        for (int i = threadIdx.x; i < n; i += NT) {
-            const int l = i - h;
-            const double2 v = s[pad(l >= 0 ? l : l + L)];
-            zl[i] = make_double2(dE.x * v.x * inv, dE.y * v.x * inv);
-            zg[i] = make_double2(dE.x * v.y * inv, dE.y * v.y * inv);
+        // P^< (real part) and P^> (imaginary part) from one inverse transform;
+        // sample m is lag m, i.e. output i = m + n/2 (mod L)
+        {
+            const double inv = 1.0 / (double)L;
+            const int h = n / 2;
+            double2* zl = series(Pl, d, ij, ix, iq);
+            double2* zg = series(Pg, d, ij, ix, iq);
+            fft::transform_io<L, NT, true, RADIX>(
+                s, tw,
+                [&](int, int q) {
+                    return make_double2(accl.v[q].x - accg.v[q].y, accl.v[q].y + accg.v[q].x);
+                },
+                [&](int m, int, double2 v) {
+                    const int i = (m + h) & (L - 1);
+                    if (i < n) {
+                        zl[i] = make_double2(dE.x * v.x * inv, dE.y * v.x * inv);
+                        zg[i] = make_double2(dE.x * v.y * inv, dE.y * v.y * inv);
+                    }
+                });
         }
         // P^R += F(i Im G^<_ij(k)) F(-i Im G^R_ij(k'))(-k) + F(i Im G^R_ij(k)) F(i Im G^<_ji(k'))(-k)
         S accr;
@@ -308,17 +270,16 @@ __device__ __forceinline__ void sigma_elem(double2* s, const double2* tw, const 
     const int n = d.nen;
     const double inv = 1.0 / (double)L;
     // dE/L * F^-1[acc] at the samples this thread owns
+    // (the inverse takes acc from registers and hands back the thread's own
+    // samples, where the transform allows it)
     auto extract = [&](const Sp& acc, double2* t) {
-        __syncthreads();
-        acc.put(s);
-        __syncthreads();
-        fft::transform<L, NT, true, RADIX>(s, tw);
-#pragma unroll
-        for (int q = 0; q < EXT; ++q) {
-            const int i = threadIdx.x + q * NT;
-            const double2 v = i < n ? s[pad(i)] : make_double2(0.0, 0.0);
-            t[q] = make_double2((dE.x * v.x - dE.y * v.y) * inv, (dE.x * v.y + dE.y * v.x) * inv);
-        }
+        fft::transform_io<L, NT, true, RADIX>(
+            s, tw, [&](int, int q) { return acc.v[q]; },
+            [&](int i, int q, double2 v) {
+                if (q >= EXT) return;
+                if (i >= n) v = make_double2(0.0, 0.0);
+                t[q] = make_double2((dE.x * v.x - dE.y * v.y) * inv, (dE.x * v.y + dE.y * v.x) * inv);
+            });
     };
     auto store = [&](double2* S, const double2* t) {
         double2* own = series(S, d, e, ix, ik);
@@ -337,11 +298,15 @@ __device__ __forceinline__ void sigma_elem(double2* s, const double2* tw, const 
             }
         }
     };
+    // F(Yr), Yr[l mod L] = Y[l + n/2]  (conv)
     auto fwd_w = [&](const double2* w) {
-        __syncthreads();
-        load_rot<L, NT>(s, w, nullptr, n);
-        __syncthreads();
-        fft::transform<L, NT, false, RADIX>(s, tw);
+        const int h = n / 2;
+        fft::transform_load<L, NT, false, RADIX>(s, tw, [&](int i, int) {
+            int src = -1;
+            if (i < n - h) src = i + h;
+            else if (i >= L - h) src = i - (L - h);
+            return src >= 0 ? w[src] : make_double2(0.0, 0.0);
+        });
     };
     // the first element's outputs are read back below by the same threads
     if (second) __threadfence_block();
