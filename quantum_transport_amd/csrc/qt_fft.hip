@@ -39,62 +39,45 @@ __global__ __launch_bounds__(NT) void corrconv_kernel(int n, int mode, const dou
     const double2* x = X + sidx * n;
     const double2* y = Y + sidx * n;
     const int h = n / 2;
-    const int tid = threadIdx.x;
-    // ---- transform of the first operand: X (corr) or the rotated Y (conv)
-    for (int i = tid; i < L; i += NT) {
-        double2 v = make_double2(0.0, 0.0);
-        if (mode == 0) {
-            if (i < n) v = x[i];
-        } else {
-            // Yr[l mod L] = Y[l + n/2], l in [-n/2, n-1-n/2]
-            if (i < n - h) v = y[i + h];
-            else if (i >= L - h) v = y[i - (L - h)];
-        }
-        s[pad(i)] = v;
-    }
-    __syncthreads();
-    fft::transform<L, NT, false>(s, tw);
+    // ---- transform of the first operand, X (corr) or the rotated Y (conv),
+    // kept in registers: Yr[l mod L] = Y[l + n/2], l in [-n/2, n-1-n/2]
     double2 fx[PER];
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int i = tid + q * NT;
-        fx[q] = i < L ? s[pad(i)] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    // ---- second operand: the flipped Y (corr) or X (conv)
-    for (int i = tid; i < L; i += NT) {
-        double2 v = make_double2(0.0, 0.0);
-        if (mode == 0) {
-            // Yf[(-q) mod L] = Y[q]
-            if (i == 0) v = y[0];
-            else if (i > L - n) v = y[L - i];
-        } else {
-            if (i < n) v = x[i];
-        }
-        s[pad(i)] = v;
-    }
-    __syncthreads();
-    fft::transform<L, NT, false>(s, tw);
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int i = tid + q * NT;
-        if (i < L) s[pad(i)] = fft::cmul(s[pad(i)], fx[q]);
-    }
-    __syncthreads();
-    fft::transform<L, NT, true>(s, tw);
+    fft::transform_io<L, NT, false, 16>(
+        s, tw,
+        [&](int i, int) {
+            double2 v = make_double2(0.0, 0.0);
+            if (mode == 0) {
+                if (i < n) v = x[i];
+            } else {
+                if (i < n - h) v = y[i + h];
+                else if (i >= L - h) v = y[i - (L - h)];
+            }
+            return v;
+        },
+        [&](int, int q, double2 v) { fx[q] = v; });
+    // ---- times the transform of the flipped Y (corr, Yf[(-q) mod L] = Y[q]) or X (conv)
+    fft::transform_io<L, NT, false, 16>(
+        s, tw,
+        [&](int i, int) {
+            double2 v = make_double2(0.0, 0.0);
+            if (mode == 0) {
+                if (i == 0) v = y[0];
+                else if (i > L - n) v = y[L - i];
+            } else {
+                if (i < n) v = x[i];
+            }
+            return v;
+        },
+        [&](int, int q, double2 v) { fx[q] = fft::cmul(v, fx[q]); });
+    // ---- inverse; sample m is lag m (corr: output i = m + n/2 mod L) or output m (conv)
     const double inv = 1.0 / (double)L;
     double2* z = Z + sidx * n;
-    for (int i = tid; i < n; i += NT) {
-        int src;
-        if (mode == 0) {
-            const int l = i - h;  // lag
-            src = l >= 0 ? l : l + L;
-        } else {
-            src = i;
-        }
-        const double2 v = s[pad(src)];
-        z[i] = make_double2(v.x * inv, v.y * inv);
-    }
+    fft::transform_io<L, NT, true, 16>(
+        s, tw, [&](int, int q) { return fx[q]; },
+        [&](int m, int, double2 v) {
+            const int i = mode == 0 ? (m + h) & (L - 1) : m;
+            if (i < n) z[i] = make_double2(v.x * inv, v.y * inv);
+        });
 }
 
 std::mutex tw_mu;
