@@ -226,7 +226,7 @@ def run_vendor_zgemm(args) -> dict:
 
 
 def time_batches(rgf, H, Eall, Kall, B: int, nd: int, steps: int, warmup: int,
-                 coupling_fix: bool = False):
+                 coupling_fix: bool = False, precision: int = 0):
     """Run ``warmup`` untimed and ``steps`` timed device batches of B points
     (outputs stay on the device); returns the wall time of the timed batches
     and the summed kernel timings."""
@@ -240,7 +240,7 @@ def time_batches(rgf, H, Eall, Kall, B: int, nd: int, steps: int, warmup: int,
         idx = (np.arange(B) + i0) % len(Eall)
         cursor[0] += B
         rgf.solve_batch_device(H, Eall[idx], Kall[idx], bufs, mul=0.0, mur=-0.1,
-                               coupling_fix=coupling_fix)
+                               coupling_fix=coupling_fix, precision=precision)
 
     for _ in range(warmup):
         step()
@@ -261,6 +261,43 @@ def time_batches(rgf, H, Eall, Kall, B: int, nd: int, steps: int, warmup: int,
     for b in bufs.values():
         b.free()
     return elapsed, acc
+
+
+def spectra_errors(mix: dict, ref: dict, nm) -> dict:
+    """max |mixed - fp64| / max |fp64| of the per-slab LDOS, ndos, pdos
+    (-2 Im Tr G^R, Im Tr G^<, -Im Tr G^>, negf_mod.f90:143-149) and of the
+    contact currents tr, tre."""
+    off = np.cumsum([0] + list(nm))
+    out = {}
+    for name, key, coef in (("ldos", "gr_diag", -2.0), ("ndos", "gl_diag", 1.0), ("pdos", "gg_diag", -1.0)):
+        a = np.stack([mix[key][:, off[i]:off[i + 1]].sum(axis=1).imag for i in range(len(nm))], 1) * coef
+        b = np.stack([ref[key][:, off[i]:off[i + 1]].sum(axis=1).imag for i in range(len(nm))], 1) * coef
+        out[name] = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    for key in ("tr", "tre"):
+        out[key] = float(np.abs(mix[key] - ref[key]).max() / max(np.abs(ref[key]).max(), 1e-300))
+    return out
+
+
+def run_mixed_leg(args, rgf, H, Eall, Kall, nb: int, nx: int, fp64_value: float) -> dict:
+    """The same workload with QT_PREC_MIXED (sweep products on the int8 matrix
+    cores, operands rounded to 40 bits; leads and inversions FP64): timed like
+    the FP64 leg, and its error measured on 32 points spread over the grid."""
+    B = args.batch
+    elapsed, acc = time_batches(rgf, H, Eall, Kall, B, nb * nx, args.steps, args.warmup,
+                                precision=rgf.QT_PREC_MIXED)
+    value = B * args.steps / elapsed
+    idx = np.linspace(0, len(Eall) - 1, 32).astype(int)
+    ref = rgf.solve_batch(H, Eall[idx], Kall[idx], mul=0.0, mur=-0.1)
+    mix = rgf.solve_batch(H, Eall[idx], Kall[idx], mul=0.0, mur=-0.1, precision=rgf.QT_PREC_MIXED)
+    errs = spectra_errors(mix, ref, [nb] * nx)
+    return {"value": round(value, 4), "unit": "energy points/s",
+            "speedup_vs_fp64": round(value / fp64_value, 3),
+            "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+            "gemm_ms_per_launch": round(acc["gemm_ms"] / max(1, acc["gemm_launches"]), 4),
+            "max_rel_err": max(errs.values()), "errors": errs, "bar": 1e-8,
+            "meets_bar": bool(max(errs.values()) <= 1e-8),
+            "error_sample": "32 points evenly spread over the 4096-point (E, kz) list, against the FP64 run",
+            "products": "int8 MFMA, 6 base-128 digits per value (40-bit operands), Karatsuba complex product"}
 
 
 def run_uneven(args, hm, rgf) -> dict:
@@ -328,6 +365,7 @@ def main() -> int:
                          "gw: the GW energy-axis stage; corrconv: batched corr1d2/conv1d2; "
                          "vendor-zgemm: rocBLAS batched ZGEMM at the sweep's shape, for comparison")
     ap.add_argument("--gw-slabs", type=int, default=2)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the RGF path is measured per GPU "
@@ -404,6 +442,8 @@ def main() -> int:
                  "backward_ms": round(acc["backward_ms"], 1)},
         "cpu_baseline": None,
     }
+    if not args.no_mixed:
+        res["mixed"] = run_mixed_leg(args, rgf, H, Eall, Kall, nb, nx, value)
     H.close()
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -428,6 +428,12 @@ void zgemm(hipStream_t s, int opa, int opb, const ZGemmArgs& a, Timer* t) {
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return;
     if (t && t->enabled) t->begin(s, 0);
     const int key = opa * 3 + opb;
+    if (a.prec == 1 && zgemm_i8_fits(a)) {
+        zgemm_i8(s, opa, opb, a);
+        if (t && t->enabled)
+            t->end(s, 0, 8.0 * a.M * a.N * (double)a.K * (a.flop_batch >= 0 ? a.flop_batch : a.batch));
+        return;
+    }
     switch (key) {
         case 0: launch<OP_N, OP_N>(s, a); break;
         case 2: launch<OP_N, OP_C>(s, a); break;

@@ -107,6 +107,7 @@ struct Gemm {
     const int* kidx;
     const int* active = nullptr;
     int flop_batch = -1;
+    int prec = 0;  // 1: int8 digit-slice products (QT_PREC_MIXED)
     // C = alpha op(A) op(B) + beta D
     void operator()(int opa, int opb, int M, int N, int K, ZOp A, ZOp B, double2* C, int64_t ldc,
                     int64_t sbc, double alpha = 1.0, ZOp D = ZOp(), double beta = 0.0) const {
@@ -126,6 +127,7 @@ struct Gemm {
         g.kidx = kidx;
         g.active = active;
         g.flop_batch = flop_batch;
+        g.prec = prec;
         zgemm(s, opa, opb, g, t);
     }
 };
@@ -664,6 +666,9 @@ struct Solve {
             g(OP_N, OP_N, nL, nL, nL, op_member(T[0], nL, nn), opC(cL), w.sigL, nL, nn);
         }
         QT_HIP(hipEventRecord(c.ev[1], s));
+        // mixed mode: the sweep's block products on the int8 matrix cores (the
+        // decimation, its self-energies and every inversion stay in FP64)
+        g.prec = p.precision == QT_PREC_MIXED ? 1 : 0;
 
         // ---------------- pass 1, first slab (rgf_mod.f90:91-104 / 312-325)
         {
@@ -1088,7 +1093,7 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
     if (!g_ctx) return QT_ERR_NOT_INIT;
     if (!h || !p || npts < 0 || (npts > 0 && !En) || !out) return QT_ERR_INVALID;
     if (npts == 0) return QT_OK;
-    if (p->precision != QT_PREC_FP64) return QT_ERR_UNSUPPORTED;
+    if (p->precision != QT_PREC_FP64 && p->precision != QT_PREC_MIXED) return QT_ERR_UNSUPPORTED;
     if (p->variant != QT_RGF_FORWARD && p->variant != QT_RGF_BACKWARD && p->variant != QT_RGF_GW)
         return QT_ERR_INVALID;
     const bool gw = p->variant == QT_RGF_GW;
@@ -1386,7 +1391,16 @@ int qt_zgemm_batched(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_t k, 
                      double alpha, const void* A, int64_t lda, int64_t strideA, const void* B,
                      int64_t ldb, int64_t strideB, double beta, void* R, int64_t ldr,
                      int64_t strideR) {
+    return qt_zgemm_batched_prec(opa, opb, m, n, k, batch, alpha, A, lda, strideA, B, ldb, strideB, beta, R,
+                                 ldr, strideR, QT_PREC_FP64);
+}
+
+int qt_zgemm_batched_prec(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_t k, int32_t batch,
+                          double alpha, const void* A, int64_t lda, int64_t strideA, const void* B,
+                          int64_t ldb, int64_t strideB, double beta, void* R, int64_t ldr,
+                          int64_t strideR, int32_t precision) {
     if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (precision != QT_PREC_FP64 && precision != QT_PREC_MIXED) return QT_ERR_INVALID;
     if (m <= 0 || n <= 0 || k < 0 || batch <= 0 || !A || !B || !R) return QT_ERR_INVALID;
     if (opa < 0 || opa > 2 || opb < 0 || opb > 2) return QT_ERR_INVALID;
     const int64_t ar = opa == OP_N ? m : k, ac = opa == OP_N ? k : m;
@@ -1421,6 +1435,7 @@ int qt_zgemm_batched(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_t k, 
         if (beta != 0.0) g.D = op_member(dR, ldr, strideR);
         g.alpha = alpha;
         g.beta = beta;
+        g.prec = precision == QT_PREC_MIXED ? 1 : 0;
         zgemm(c.stream, opa, opb, g, &c.timer);
         QT_HIP(hipStreamSynchronize(c.stream));
         QT_HIP(hipMemcpy(R, dR, sizeof(double2) * er, hipMemcpyDeviceToHost));

@@ -26,6 +26,7 @@ QT_RGF_FORWARD = 0
 QT_RGF_BACKWARD = 1
 QT_RGF_GW = 2
 QT_PREC_FP64 = 0
+QT_PREC_MIXED = 1
 QT_F_COUPLING_FIX = 0x1
 QT_F_DEVICE_OUTPUTS = 0x2
 QT_F_LEAD_RESET = 0x4
@@ -36,7 +37,7 @@ QT_ERR_SINGULAR = 5
 EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham_destroy",
             "qt_rgf_solve", "qt_sancho", "qt_device_alloc", "qt_device_free", "qt_memcpy_h2d",
             "qt_memcpy_d2h", "qt_synchronize", "qt_set_kernel_timing", "qt_last_timing",
-            "qt_zgemm_batched", "qt_invert_batched", "qt_corr1d2_batched",
+            "qt_zgemm_batched", "qt_zgemm_batched_prec", "qt_invert_batched", "qt_corr1d2_batched",
             "qt_conv1d2_batched", "qt_gw_polarization", "qt_gw_selfenergy", "qt_gw_mix",
             "qt_gw_wsolve_batched"]
 
@@ -178,18 +179,20 @@ class Hamiltonian:
 def make_params(variant: str = "forward", mul: float = 0.0, mur: float = 0.0,
                 templ: float = 300.0, tempr: float = 300.0, coupling_fix: bool = False,
                 lead_eta: float = 1e-5, lead_nmax: int = 50, lead_tol: float = 1e-100,
-                lead_reset: bool = False, device_outputs: bool = False) -> Params:
+                lead_reset: bool = False, device_outputs: bool = False,
+                precision: int = QT_PREC_FP64) -> Params:
     flags = (QT_F_COUPLING_FIX if coupling_fix else 0) | (QT_F_DEVICE_OUTPUTS if device_outputs else 0) \
         | (QT_F_LEAD_RESET if lead_reset else 0)
     code = {"forward": QT_RGF_FORWARD, "backward": QT_RGF_BACKWARD, "gw": QT_RGF_GW}[variant]
-    return Params(code, QT_PREC_FP64, flags, lead_nmax, mul, mur, templ, tempr, lead_eta, lead_tol)
+    return Params(code, precision, flags, lead_nmax, mul, mur, templ, tempr, lead_eta, lead_tol)
 
 
 def solve_batch(H: Hamiltonian, En, kidx=None, *, variant: str = "forward", mul: float = 0.0,
                 mur: float = 0.0, templ: float = 300.0, tempr: float = 300.0,
                 coupling_fix: bool = False, sigma_lesser_ph=None, sigma_r_ph=None,
                 diag: bool = True, full: bool = False, lead_eta: float = 1e-5,
-                lead_nmax: int = 50, raise_singular: bool = False) -> dict:
+                lead_nmax: int = 50, raise_singular: bool = False,
+                precision: int = QT_PREC_FP64) -> dict:
     """Batched RGF over points (En[ip], kidx[ip]).  Returns tr, tre,
     lead_iters, info and (diag=True) the diagonals of G^R, G^<, G^> per
     slab, [npts, sum(nm)]; with full=True also the full block lists
@@ -223,7 +226,8 @@ def solve_batch(H: Hamiltonian, En, kidx=None, *, variant: str = "forward", mul:
         srs = [_f(b) for pt in sigma_r_ph for b in pt]
         keep += [sls, srs]
         sl, sr = _ptrs(sls), _ptrs(srs)
-    p = make_params(variant, mul, mur, templ, tempr, coupling_fix, lead_eta, lead_nmax)
+    p = make_params(variant, mul, mur, templ, tempr, coupling_fix, lead_eta, lead_nmax,
+                    precision=precision)
     st = lib().qt_rgf_solve(H.handle, C.byref(p), C.c_int32(npts), En.ctypes.data_as(C.c_void_p),
                             kidx.ctypes.data_as(C.c_void_p), sl, sr, C.byref(o))
     if st == QT_ERR_SINGULAR and not raise_singular:
@@ -323,7 +327,8 @@ class DeviceBuffer:
 
 def solve_batch_device(H: Hamiltonian, En, kidx, bufs: dict, *, variant: str = "forward",
                        mul: float = 0.0, mur: float = 0.0, templ: float = 300.0,
-                       tempr: float = 300.0, coupling_fix: bool = False) -> None:
+                       tempr: float = 300.0, coupling_fix: bool = False,
+                       precision: int = QT_PREC_FP64) -> None:
     """Batched solve with outputs left in device buffers (bufs: tr, tre,
     gr_diag, gl_diag, gg_diag -> DeviceBuffer)."""
     En = np.ascontiguousarray(En, dtype=np.float64)
@@ -337,7 +342,8 @@ def solve_batch_device(H: Hamiltonian, En, kidx, bufs: dict, *, variant: str = "
     for k in ("gr_diag", "gl_diag", "gg_diag"):
         if k in bufs:
             setattr(o, k, bufs[k].ptr)
-    p = make_params(variant, mul, mur, templ, tempr, coupling_fix, device_outputs=True)
+    p = make_params(variant, mul, mur, templ, tempr, coupling_fix, device_outputs=True,
+                    precision=precision)
     st = lib().qt_rgf_solve(H.handle, C.byref(p), C.c_int32(npts), En.ctypes.data_as(C.c_void_p),
                             kidx.ctypes.data_as(C.c_void_p), None, None, C.byref(o))
     if st != QT_ERR_SINGULAR:
@@ -345,10 +351,12 @@ def solve_batch_device(H: Hamiltonian, En, kidx, bufs: dict, *, variant: str = "
 
 
 def zgemm_batched(A: np.ndarray, B: np.ndarray, opa: int = 0, opb: int = 0, alpha: float = 1.0,
-                  beta: float = 0.0, R: Optional[np.ndarray] = None) -> np.ndarray:
+                  beta: float = 0.0, R: Optional[np.ndarray] = None,
+                  precision: int = QT_PREC_FP64) -> np.ndarray:
     """R[b] = alpha op(A[b]) op(B[b]) + beta R[b] on the device (MUL_C,
     matrix_c.f90:615-660).  A, B: (batch, rows, cols) stacks of column-major
-    blocks (each block stored Fortran-ordered)."""
+    blocks (each block stored Fortran-ordered).  precision=QT_PREC_MIXED runs
+    the product on the int8 matrix cores (operands rounded to 40 bits)."""
     init()
     A = np.asarray(A, dtype=np.complex128)
     B = np.asarray(B, dtype=np.complex128)
@@ -364,12 +372,12 @@ def zgemm_batched(A: np.ndarray, B: np.ndarray, opa: int = 0, opb: int = 0, alph
     else:
         # always a copy: the library writes the result into this buffer
         Rf = np.array(np.transpose(np.asarray(R, np.complex128), (0, 2, 1)), order="C", copy=True)
-    check(lib().qt_zgemm_batched(C.c_int32(opa), C.c_int32(opb), C.c_int32(m), C.c_int32(n),
-                                 C.c_int32(k), C.c_int32(batch), C.c_double(alpha),
-                                 Af.ctypes.data_as(C.c_void_p), C.c_int64(ar), C.c_int64(ar * ac),
-                                 Bf.ctypes.data_as(C.c_void_p), C.c_int64(br), C.c_int64(br * bc),
-                                 C.c_double(beta), Rf.ctypes.data_as(C.c_void_p), C.c_int64(m),
-                                 C.c_int64(m * n)))
+    check(lib().qt_zgemm_batched_prec(C.c_int32(opa), C.c_int32(opb), C.c_int32(m), C.c_int32(n),
+                                      C.c_int32(k), C.c_int32(batch), C.c_double(alpha),
+                                      Af.ctypes.data_as(C.c_void_p), C.c_int64(ar), C.c_int64(ar * ac),
+                                      Bf.ctypes.data_as(C.c_void_p), C.c_int64(br), C.c_int64(br * bc),
+                                      C.c_double(beta), Rf.ctypes.data_as(C.c_void_p), C.c_int64(m),
+                                      C.c_int64(m * n), C.c_int32(precision)))
     return np.transpose(Rf, (0, 2, 1))
 
 
