@@ -1,13 +1,16 @@
 ! qt_negf.f90 -- NEGF driver program: reads the namelist /input/ from the
-! file 'input' and the Wannier Hamiltonian from 'ham_dat', solves the
-! ballistic transport problem on the MI355X and writes ldos/ndos/pdos0000.dat.
-! Namelist variables and defaults follow the reference driver (main.f90:49,
-! 75-86 there); k-points are all zero, one per nk.
+! file 'input' and the Hamiltonian from 'ham_dat' (calc_type = 'w90') or from
+! the COO file 'h_dat' (any other calc_type: 0-based real entries, slabs of
+! nb orbitals, as main.f90:109-115 there), solves the ballistic transport
+! problem on the MI355X and writes ldos/ndos/pdos0000.dat.  Namelist
+! variables and defaults follow the reference driver (main.f90:49,75-86
+! there); k-points are all zero, one per nk (one for the COO route, whose
+! blocks carry no k dependence; each slab is then one output cell, Lx = 1).
 program qt_negf
     use omp_lib
     use negf_mod, only: negf_solve
     use matrix_c, only: type_matrix_complex, free
-    use deviceHam_mod, only: devH_build_fromWannierFile
+    use deviceHam_mod, only: devH_build_fromWannierFile, devH_build_fromCOOfile
     implicit none
     type(type_matrix_complex), allocatable, dimension(:, :) :: Hii, H1i, Sii
     integer :: nx, ns, nen, nk, nb, rc, fu, nomp
@@ -27,14 +30,21 @@ program qt_negf
     else
         write (*, *) 'Warn: input file input does not exist'
     end if
-    if (calc_type /= 'w90') error stop 'qt_negf: only calc_type = w90 is provided'
     comm_size = 1
     comm_rank = 0
     local_NE = nen/comm_size
     first_local_energy = local_NE*comm_rank + 1
-    allocate (k(2, nk))
-    k = 0d0
-    call devH_build_fromWannierFile('ham_dat', Hii, H1i, Sii, nx, ns, nb, nk, k, Lx)
+    if (calc_type == 'w90') then
+        allocate (k(2, nk))
+        k = 0d0
+        call devH_build_fromWannierFile('ham_dat', Hii, H1i, Sii, nx, ns, nb, nk, k, Lx)
+    else
+        call devH_build_fromCOOfile('h_dat', Hii, H1i, Sii, (/0, 0/), (/nb, nb/), nx, &
+                                    use0index=.true., iscomplex=.false., threshold=1d-10, blocksize=nb)
+        nk = 1
+        ns = 1
+        Lx = 1d0
+    end if
     t0 = omp_get_wtime()
     call negf_solve(nx, nen, nk, emin, emax, Hii, H1i, Sii, temp, mu, comm_size, comm_rank, &
                     local_NE, first_local_energy, nb, ns, Lx)

@@ -240,6 +240,113 @@ def device_from_wannier(w: Wannier, nx: int, ns: int, kpts: Sequence[Tuple[float
     return Device([w.nb * ns] * nx, Hii, H1i, None, w.Lx, w.nb, ns)
 
 
+# ---------------------------------------------------------------------------
+# devH_build_fromCOOfile equivalent
+# ---------------------------------------------------------------------------
+
+
+def read_coo(path: str, use0index: bool = True, iscomplex: bool = False):
+    """load_COOmatrix (deviceHam_mod.f90:40-89): one entry per line,
+    ``i j re`` or (iscomplex) ``i j re im``; with use0index the indices are
+    0-based.  Returns 1-based row and column arrays, the complex values and
+    the matrix order.  The reference sets the order to the largest index read
+    before the 0-based shift (one less than the order for 0-based files);
+    here it is the largest 1-based index."""
+    rows, cols, vals = [], [], []
+    with open(path) as fh:
+        for line in fh:
+            tok = line.split()
+            if len(tok) < 3:
+                continue
+            i, j = int(tok[0]), int(tok[1])
+            re = float(tok[2])
+            im = float(tok[3]) if iscomplex and len(tok) > 3 else 0.0
+            if use0index:
+                i, j = i + 1, j + 1
+            rows.append(i)
+            cols.append(j)
+            vals.append(complex(re, im))
+    r = np.array(rows, dtype=np.int64)
+    c = np.array(cols, dtype=np.int64)
+    norb = int(max(r.max(), c.max())) if len(r) else 0
+    return r, c, np.array(vals, dtype=np.complex128), norb
+
+
+def write_coo(path: str, H: np.ndarray, use0index: bool = True, iscomplex: bool = False,
+              upper_only: bool = True) -> None:
+    """Writes the nonzeros of H in the format read_coo reads (for tests and
+    for converting devices built elsewhere)."""
+    off = 0 if use0index else 1
+    with open(path, "w") as fh:
+        for i, j in zip(*np.nonzero(H)):
+            if upper_only and j < i:
+                continue
+            v = H[i, j]
+            if iscomplex:
+                fh.write(f"{i + off} {j + off} {v.real:.17g} {v.imag:.17g}\n")
+            else:
+                fh.write(f"{i + off} {j + off} {v.real:.17g}\n")
+
+
+def device_from_coo(hpath: str, blocksize: int, use0index: bool = True, iscomplex: bool = False,
+                    spath: Optional[str] = None) -> Device:
+    """devH_build_fromCOOfile with a fixed block size (deviceHam_mod.f90:91-219,
+    the route main.f90:113-114 takes for calc_type /= 'w90'): nx = order /
+    blocksize uniform slabs of one Hermitian matrix given by the entries of
+    its upper triangle (an entry inside a diagonal block is mirrored, as the
+    reference does; an entry of block row ix and column ix+1 is H_{ix,ix+1}).
+
+    Kept from the reference: the file format, the mirroring, S = identity
+    without an overlap file, the lead couplings repeating the adjacent
+    interior coupling.  Different, because the reference's block does not run
+    as written: the slab of 1-based row r is (r-1)/blocksize (the reference's
+    r/blocksize puts the last row of each slab into the next one, at local
+    row 0); H1i[s] holds H_{s,s-1} = H_{s-1,s}^H, the convention the solver
+    reads (the reference stores H_{s,s+1} there and leaves the last coupling
+    zero); entries below the diagonal are accepted (a lower block entry is
+    H_{s,s-1} itself); the order counts 0-based indices correctly.  The
+    reference's threshold filter and graph slicing are commented out there
+    and not provided."""
+    r, c, v, norb = read_coo(hpath, use0index, iscomplex)
+    if blocksize <= 0 or norb % blocksize:
+        raise ValueError(f"matrix order {norb} is not a multiple of blocksize {blocksize}")
+    nx = norb // blocksize
+    if nx < 2:
+        raise ValueError("the COO route needs at least two slabs")
+    bs = blocksize
+    Hii = [np.zeros((bs, bs), np.complex128, order="F") for _ in range(nx)]
+    H1i = [np.zeros((bs, bs), np.complex128, order="F") for _ in range(nx + 1)]
+    for i, j, x in zip(r - 1, c - 1, v):
+        si, sj = i // bs, j // bs
+        mi, mj = i - si * bs, j - sj * bs
+        if si == sj:
+            Hii[si][mi, mj] = x
+            Hii[si][mj, mi] = np.conj(x)
+        elif sj == si + 1:        # H_{si, si+1}: H1i[si+1] = its conjugate transpose
+            H1i[sj][mj, mi] = np.conj(x)
+        elif sj == si - 1:        # H_{si, si-1} itself
+            H1i[si][mi, mj] = x
+        else:
+            raise ValueError(f"entry ({i + 1}, {j + 1}) couples slabs {si + 1} and {sj + 1}: "
+                             "the device is not block-tridiagonal at this block size")
+    H1i[0] = H1i[1].copy(order="F")
+    H1i[nx] = H1i[nx - 1].copy(order="F")
+    Sii = None
+    if spath is not None:
+        sr, sc, sv, sn = read_coo(spath, use0index, iscomplex)
+        if sn > norb:
+            raise ValueError("overlap matrix larger than the Hamiltonian")
+        Sii = [np.zeros((bs, bs), np.complex128, order="F") for _ in range(nx)]
+        for i, j, x in zip(sr - 1, sc - 1, sv):
+            si, sj = i // bs, j // bs
+            if si != sj:
+                raise ValueError("the overlap must be block diagonal")
+            Sii[si][i - si * bs, j - sj * bs] = x
+            Sii[si][j - sj * bs, i - si * bs] = np.conj(x)
+        Sii = [Sii]
+    return Device([bs] * nx, [Hii], [H1i], Sii, 1.0, bs, 1)
+
+
 def energy_grid(emin: float, emax: float, nen: int) -> np.ndarray:
     """negf_mod.f90:100-101."""
     if nen == 1:

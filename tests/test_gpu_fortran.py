@@ -119,3 +119,45 @@ def test_fortran_gw_stages(oracle_mod):
         ref = oracle_mod.gw_wsolve(Vii, V1i, PR[..., w], PL[..., w], PG[..., w])
         for got, r in zip((WR[..., w], WL[..., w], WG[..., w]), ref):
             assert np.abs(got - r).max() <= 1e-10 * np.abs(r).max()
+
+
+@pytest.mark.gpu
+def test_qt_negf_coo_route(tmp_path, oracle_mod, flavour):
+    """calc_type /= 'w90': the driver reads 'h_dat' through
+    devH_build_fromCOOfile (0-based real entries of the upper triangle, slabs
+    of nb orbitals, as main.f90:113-114) and writes the same spectra as the
+    oracle on the blocks hamiltonian.device_from_coo builds from that file:
+    every printed value is the oracle's rounded to the 4 printed digits."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("qt_negf not built (make -C quantum_transport_amd/fortran)")
+    from quantum_transport_amd import hamiltonian as hm
+
+    nb, nx = 8, 6
+    _, dev = hm.synthetic_device(nb, nx, seed=21, coupling_density=0.4)
+    n = nb * nx
+    H = np.zeros((n, n))
+    for s in range(nx):
+        H[s * nb:(s + 1) * nb, s * nb:(s + 1) * nb] = dev.Hii[0][s].real
+        if s > 0:
+            H[s * nb:(s + 1) * nb, (s - 1) * nb:s * nb] = dev.H1i[0][s].real
+            H[(s - 1) * nb:s * nb, s * nb:(s + 1) * nb] = dev.H1i[0][s].real.T
+    hm.write_coo(str(tmp_path / "h_dat"), H, use0index=True, iscomplex=False)
+    (tmp_path / "input").write_text("&input\nnb=8\nnen=40\nemin=-3.0\nemax=3.0\n"
+                                    "mu(:) = 0.0, 0.0\ntemp(:) = 300.0, 300.0\ncalc_type='coo'\n/\n")
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    coo = hm.device_from_coo(str(tmp_path / "h_dat"), nb, use0index=True, iscomplex=False)
+    E = hm.energy_grid(-3.0, 3.0, 40)
+    ref = oracle_mod.rgf_batch(coo, E, None, 0.0, 0.0, flavour=flavour)
+    for name, key, coef in (("ldos", "gr_diag", -2.0), ("ndos", "gl_diag", 1.0), ("pdos", "gg_diag", -1.0)):
+        vals = np.array([[float(t) for t in ln.split()] for ln in _records(tmp_path / f"{name}0000.dat")
+                         if ln.strip()])
+        assert vals.shape == (40 * nx, 4)
+        want = np.asarray(ref[key]).reshape(40, nx, nb).sum(-1).imag.ravel() * coef
+        scale = np.abs(want).max()
+        for (x, e, _, got), w in zip(vals, want):
+            if abs(w) < 1e-6 * scale:
+                continue
+            ulp = 10.0 ** (np.floor(np.log10(abs(got))) + 1 - 4)  # 0.dddd E+xx: 4 digits
+            assert abs(got - w) <= 0.5 * ulp * (1 + 1e-6), (name, e, got, w)

@@ -2,6 +2,7 @@
 w90_MAT_DEF block convention (wannierHam3d.f90:273-301), energy grid
 (negf_mod.f90:100-101)."""
 import numpy as np
+import pytest
 
 from quantum_transport_amd import hamiltonian as hm
 
@@ -62,3 +63,62 @@ def test_mat_def_blocks_of_ns_cells(golden_dir):
 def test_energy_grid():
     E = hm.energy_grid(-6.0, -1.0, 200)
     assert E[0] == -6.0 and abs(E[-1] + 1.0) < 1e-14 and len(E) == 200
+
+
+def _dense_from_device(dev):
+    n = sum(dev.nm)
+    H = np.zeros((n, n), np.complex128)
+    off = np.cumsum([0] + list(dev.nm))
+    for s in range(dev.nx):
+        H[off[s]:off[s + 1], off[s]:off[s + 1]] = dev.Hii[0][s]
+        if s > 0:  # H1i[s] = H_{s,s-1}
+            H[off[s]:off[s + 1], off[s - 1]:off[s]] = dev.H1i[0][s]
+            H[off[s - 1]:off[s], off[s]:off[s + 1]] = dev.H1i[0][s].conj().T
+    return H
+
+
+def test_coo_route_rebuilds_the_device(tmp_path):
+    """devH_build_fromCOOfile (deviceHam_mod.f90:91-219) with blocksize: the
+    upper triangle of a block-tridiagonal Hermitian matrix written as a COO
+    file (0-based, complex) comes back as the same blocks, H1i[s] = H_{s,s-1},
+    the lead couplings repeating the outer interior couplings"""
+    _, dev = hm.synthetic_device(6, 5, seed=11, coupling_density=0.4)
+    H = _dense_from_device(dev)
+    assert np.allclose(H, H.conj().T)
+    path = tmp_path / "h_dat"
+    hm.write_coo(str(path), H, use0index=True, iscomplex=True)
+    got = hm.device_from_coo(str(path), 6, use0index=True, iscomplex=True)
+    assert got.nm == [6] * 5 and got.nk == 1 and got.Sii is None
+    for s in range(5):
+        assert np.array_equal(got.Hii[0][s], dev.Hii[0][s])
+    for s in range(1, 5):
+        assert np.array_equal(got.H1i[0][s], dev.H1i[0][s])
+    assert np.array_equal(got.H1i[0][0], got.H1i[0][1])
+    assert np.array_equal(got.H1i[0][5], got.H1i[0][4])
+    assert np.array_equal(_dense_from_device(got), H)
+
+
+def test_coo_route_real_one_based_and_errors(tmp_path):
+    """1-based real files (iscomplex=False keeps the real parts, as
+    main.f90:113-114 reads h_dat); the last row of each slab stays in its slab;
+    orders that are not a multiple of the block size and couplings beyond the
+    next slab are rejected"""
+    rng = np.random.default_rng(2)
+    n, bs = 12, 4
+    H = np.zeros((n, n))
+    for i in range(n):
+        for j in range(i, min(n, (i // bs + 2) * bs)):
+            H[i, j] = H[j, i] = rng.standard_normal()
+    path = tmp_path / "h1"
+    hm.write_coo(str(path), H, use0index=False, iscomplex=False)
+    r, c, v, norb = hm.read_coo(str(path), use0index=False)
+    assert norb == n and r.min() == 1
+    dev = hm.device_from_coo(str(path), bs, use0index=False)
+    assert np.array_equal(_dense_from_device(dev).real, H)
+    assert dev.Hii[0][0][bs - 1, bs - 1] == H[bs - 1, bs - 1]  # row bs is in slab 1
+    with pytest.raises(ValueError):
+        hm.device_from_coo(str(path), 5, use0index=False)
+    H[0, n - 1] = H[n - 1, 0] = 1.0
+    hm.write_coo(str(path), H, use0index=False)
+    with pytest.raises(ValueError):
+        hm.device_from_coo(str(path), bs, use0index=False)
