@@ -5,13 +5,17 @@
 ! points to the device in batched calls of qt_rgf_solve (forward variant,
 ! the arithmetic of rgf_mod's rgf_variableblock_forward) and keeps only the
 ! diagonals it writes: ldos / ndos / pdos with coefficients (1,-2), (1,1),
-! (1,-1) (negf_mod.f90:141-149 there).
+! (1,-1) (negf_mod.f90:141-149 there).  It also writes what the reference's
+! ballistic driver computes but drops: the spectral currents at both
+! contacts per energy ('trL0000.dat', 'trR0000.dat') and their energy
+! integrals in A (appended to 'Id.dat', spin degeneracy 1, single-rank runs),
+! with the writers the GW driver calls (gw_mod.f90:409-428 there).
 module negf_mod
     use iso_c_binding
     use qt_capi
     use matrix_c, only: type_matrix_complex
     use qt_rgf_host, only: lists_identity
-    use Output, only: write_spectrum_from_diagonals
+    use Output, only: write_spectrum_from_diagonals, write_transmission_spectrum_k, write_total_current
     implicit none
     private
     integer, parameter :: dp = 8
@@ -38,6 +42,7 @@ contains
         type(qt_rgf_outputs) :: o
         complex(dp), allocatable, target :: gr(:, :), gl(:, :), gg(:, :)
         complex(dp), allocatable :: DR(:, :, :), DL(:, :, :), DG(:, :, :)
+        real(dp), allocatable :: TL(:, :), TR_(:, :)
         real(c_double), allocatable, target :: tr(:), tre(:), pe(:)
         integer(c_int32_t), allocatable, target :: pk(:), info(:)
         integer :: ie, ik, s, nd, npts, p0, nb, ip, i
@@ -84,6 +89,7 @@ contains
             end do
         end do
         allocate (DR(nd, nk, local_NE), DL(nd, nk, local_NE), DG(nd, nk, local_NE))
+        allocate (TL(nk, local_NE), TR_(nk, local_NE))
         allocate (gr(nd, CHUNK), gl(nd, CHUNK), gg(nd, CHUNK), tr(CHUNK), tre(CHUNK), info(CHUNK))
         p%variant = QT_RGF_FORWARD
         p%mul = mu(1)
@@ -111,6 +117,8 @@ contains
                 DR(:, ik, ie) = gr(:, i)
                 DL(:, ik, ie) = gl(:, i)
                 DG(:, ik, ie) = gg(:, i)
+                TL(ik, ie) = tr(i)
+                TR_(ik, ie) = tre(i)
                 if (iand(info(i), QT_INFO_LEAD_NMAX) /= 0) &
                     write (*, *) 'SEVERE warning: nmax reached in sancho!!! E =', pe(ip)
             end do
@@ -125,8 +133,14 @@ contains
                                                    nslab, Lx, [1.0d0, 1.0d0], comm_rank /= 0)
                 call write_spectrum_from_diagonals('pdos', 0, DG, local_NE, local_energies, nk, nx, nbnd, &
                                                    nslab, Lx, [1.0d0, -1.0d0], comm_rank /= 0)
+                ! 'trL' holds tr and 'trR' tre, the mapping of gw_mod.f90:409-412
+                ! there (tr is evaluated at the last contact, tre at the first)
+                call write_transmission_spectrum_k('trL', 0, TL, local_NE, local_energies, nk, comm_rank /= 0)
+                call write_transmission_spectrum_k('trR', 0, TR_, local_NE, local_energies, nk, comm_rank /= 0)
             end if
         end do
+        ! the integral needs every energy: written when one rank holds them all
+        if (comm_size == 1) call write_total_current('Id.dat', 0, sum(TL), sum(TR_), dE, 1d0, nk)
     end subroutine negf_solve
 
 end module negf_mod

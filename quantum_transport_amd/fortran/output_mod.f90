@@ -9,7 +9,10 @@ module Output
     implicit none
     private
     integer, parameter :: dp = 8
-    public :: write_spectrum_summed_over_k, write_spectrum_from_diagonals
+    public :: write_spectrum_summed_over_k, write_spectrum_from_diagonals, write_transmission_spectrum_k, &
+              write_total_current
+    ! constants of the reference's current formula (static.f90:10,14, fft_mod.f90:19 there)
+    real(dp), parameter :: hbar = 1.0546d-34, e0 = 1.6022d-19, tpi = 3.14159265359d0*2.0d0
 
 contains
 
@@ -87,5 +90,37 @@ contains
         end do
         close (11)
     end subroutine write_spectrum_from_diagonals
+
+    ! Spectral current per energy, the file the GW driver names 'gw_trL' /
+    ! 'gw_trR' and never defines (gw_mod.f90:409-412 there): one record
+    ! '(2E18.4)' per energy, E and the k-average of T(E, k).
+    subroutine write_transmission_spectrum_k(dataset, i, T, nen, en, nk, append)
+        character(len=*), intent(in) :: dataset
+        integer, intent(in) :: i, nen, nk
+        real(dp), intent(in) :: T(nk, nen), en(nen)
+        logical, intent(in), optional :: append
+        logical :: app
+        integer :: ie
+        app = .false.
+        if (present(append)) app = append
+        call open_spectrum(dataset, i, app)
+        do ie = 1, nen
+            write (11, '(2E18.4)') en(ie), sum(T(:, ie))/dble(nk)
+        end do
+        close (11)
+    end subroutine write_transmission_spectrum_k
+
+    ! Energy-integrated terminal currents in A, the record gw_mod.f90:424-428
+    ! there appends to 'Id_iteration.dat': '(I4,2E16.6)' iteration,
+    ! sum(trL) dE e0 / (2 pi hbar) e0 spindeg / nk, the same for trR.
+    subroutine write_total_current(fname, iter, sum_tr, sum_tre, dE, spindeg, nk)
+        character(len=*), intent(in) :: fname
+        integer, intent(in) :: iter, nk
+        real(dp), intent(in) :: sum_tr, sum_tre, dE, spindeg
+        open (unit=101, file=trim(fname), status='unknown', position='append')
+        write (101, '(I4,2E16.6)') iter, sum_tr*dE*e0/tpi/hbar*e0*spindeg/dble(nk), &
+            sum_tre*dE*e0/tpi/hbar*e0*spindeg/dble(nk)
+        close (101)
+    end subroutine write_total_current
 
 end module Output

@@ -143,13 +143,13 @@ def test_qt_negf_coo_route(tmp_path, oracle_mod, flavour):
             H[(s - 1) * nb:s * nb, s * nb:(s + 1) * nb] = dev.H1i[0][s].real.T
     hm.write_coo(str(tmp_path / "h_dat"), H, use0index=True, iscomplex=False)
     (tmp_path / "input").write_text("&input\nnb=8\nnen=40\nemin=-3.0\nemax=3.0\n"
-                                    "mu(:) = 0.0, 0.0\ntemp(:) = 300.0, 300.0\ncalc_type='coo'\n/\n")
+                                    "mu(:) = 0.0, -0.1\ntemp(:) = 300.0, 300.0\ncalc_type='coo'\n/\n")
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     coo = hm.device_from_coo(str(tmp_path / "h_dat"), nb, use0index=True, iscomplex=False)
     E = hm.energy_grid(-3.0, 3.0, 40)
-    ref = oracle_mod.rgf_batch(coo, E, None, 0.0, 0.0, flavour=flavour)
+    ref = oracle_mod.rgf_batch(coo, E, None, 0.0, -0.1, flavour=flavour)
     for name, key, coef in (("ldos", "gr_diag", -2.0), ("ndos", "gl_diag", 1.0), ("pdos", "gg_diag", -1.0)):
         vals = np.array([[float(t) for t in ln.split()] for ln in _records(tmp_path / f"{name}0000.dat")
                          if ln.strip()])
@@ -161,3 +161,23 @@ def test_qt_negf_coo_route(tmp_path, oracle_mod, flavour):
                 continue
             ulp = 10.0 ** (np.floor(np.log10(abs(got))) + 1 - 4)  # 0.dddd E+xx: 4 digits
             assert abs(got - w) <= 0.5 * ulp * (1 + 1e-6), (name, e, got, w)
+    # spectral currents per energy ('(2E18.4)': E, tr) and their integrals
+    # ('(I4,2E16.6)', gw_mod.f90:427 there, spin degeneracy 1, nk = 1)
+    for name, key in (("trL", "tr"), ("trR", "tre")):
+        vals = np.array([[float(t) for t in ln.split()] for ln in _records(tmp_path / f"{name}0000.dat")
+                         if ln.strip()])
+        assert vals.shape == (40, 2) and np.allclose(vals[:, 0], E, rtol=1e-3)
+        want = np.asarray(ref[key])
+        scale = np.abs(want).max()
+        assert scale > 0
+        for got, w in zip(vals[:, 1], want):
+            if abs(w) < 1e-6 * scale:
+                continue
+            ulp = 10.0 ** (np.floor(np.log10(abs(got))) + 1 - 4)
+            assert abs(got - w) <= 0.5 * ulp * (1 + 1e-6), (name, got, w)
+    rec = _records(tmp_path / "Id.dat")[0].split()
+    hbar, e0, tpi = 1.0546e-34, 1.6022e-19, 3.14159265359 * 2.0
+    dE = E[1] - E[0]
+    for got, key in zip(rec[1:], ("tr", "tre")):
+        want = np.sum(ref[key]) * dE * e0 / tpi / hbar * e0
+        assert abs(float(got) - want) <= 1e-5 * abs(want), (key, got, want)
