@@ -483,8 +483,8 @@ int kernel_variant() {
 #endif
 }
 
-// per-stream workspace for the digit planes (grown on demand; growth
-// synchronises the device through hipFree/hipMalloc)
+// per-stream workspace for the digit planes, grown on demand: growth waits
+// for the stream's earlier products before the old planes are freed
 struct Ws {
     int8_t* planes = nullptr;
     size_t pcap = 0;
@@ -501,6 +501,7 @@ Ws& workspace(hipStream_t s, size_t pbytes, size_t sints) {
     QT_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(ws_mu);
     Ws& w = ws_map()[{dev, s}];
+    if ((w.pcap < pbytes && w.planes) || (w.scap < sints && w.scales)) QT_HIP(hipStreamSynchronize(s));
     if (w.pcap < pbytes) {
         if (w.planes) QT_HIP(hipFree(w.planes));
         w.planes = nullptr;
