@@ -159,3 +159,40 @@ def test_mixed_solve_vs_fp64_kz(qt):
     print("mixed vs fp64", e)
     for key, v in e.items():
         assert v <= (1e-3 if key in ("tr", "tre") else 1e-6), (key, v)
+
+
+@pytest.mark.parametrize("variant", ["backward", "uneven"])
+def test_mixed_other_variants(qt, variant):
+    """The mixed mode in the backward variant (nb = 64 x 8 slabs) and on
+    uneven slabs of 64-192 orbitals (coupling fix): against the FP64 device
+    path, with the bars of the nb = 256 case (spectra 1e-6, currents 1e-3)."""
+    if variant == "backward":
+        _, dev = hm.synthetic_device(64, 8, seed=71, coupling_density=0.1)
+        kw = dict(variant="backward", mul=0.05, mur=-0.15)
+    else:
+        dev = hm.uneven_device([64, 128, 96, 192, 64], seed=72, coupling_density=0.1)
+        kw = dict(mul=0.0, mur=-0.1, coupling_fix=True)
+    E = hm.energy_grid(-2.5, 2.5, 12)
+    H = qt.Hamiltonian.from_device(dev)
+    fp = qt.solve_batch(H, E, None, **kw)
+    mix = qt.solve_batch(H, E, None, precision=qt.QT_PREC_MIXED, **kw)
+    H.close()
+    assert (mix["info"] == 0).all()
+    e = _mixed_errors(mix, fp, dev.nm)
+    print(variant, "mixed vs fp64", e)
+    assert max(e.values()) > 0.0  # the int8 products ran
+    for key, v in e.items():
+        assert v <= (1e-3 if key in ("tr", "tre") else 1e-6), (key, v)
+
+
+def test_mixed_small_blocks_stay_fp64(qt):
+    """Blocks below 64 orbitals take the FP64 product in the mixed mode, so
+    the results equal the FP64 run bit for bit"""
+    _, dev = hm.synthetic_device(32, 6, seed=73)
+    E = hm.energy_grid(-2.0, 2.0, 6)
+    H = qt.Hamiltonian.from_device(dev)
+    fp = qt.solve_batch(H, E, None, mul=0.0, mur=-0.1)
+    mix = qt.solve_batch(H, E, None, mul=0.0, mur=-0.1, precision=qt.QT_PREC_MIXED)
+    H.close()
+    for key in ("gr_diag", "gl_diag", "gg_diag", "tr", "tre"):
+        assert np.array_equal(mix[key], fp[key]), key
