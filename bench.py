@@ -20,6 +20,7 @@ thread).
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -300,6 +301,98 @@ def run_mixed_leg(args, rgf, H, Eall, Kall, nb: int, nx: int, fp64_value: float)
             "products": "int8 MFMA, 6 base-128 digits per value (40-bit operands), Karatsuba complex product"}
 
 
+def run_leads(args, hm, rgf) -> dict:
+    """Contact surface Green's functions alone (the lead stage of every point;
+    the shape on which a learned warm start would be judged): lead block
+    nb = 256 from the same generator, 4096 energies in [-3, 3] eV x 16 kz in
+    [0, pi), eta = 1e-6, plain Sancho-Rubio (rgf_mod.f90:482-568, stop test
+    sum |H01| < 1e-100, at most 50 steps) through qt_sancho.  A step is one
+    batch of 4 x --batch points spread over the (E, kz) list.  qt_sancho takes
+    host blocks, so the copies of H00/H10 in and G00 out are inside the timed
+    region; path.kernel_share_of_wall is the GEMM + inversion kernel time over
+    the wall time."""
+    nb, B = 256, args.batch * 4
+    kz = np.arange(16) * np.pi / 16
+    w = hm.synthetic_wannier(nb, 1003, tz_density=0.02)
+    H00s, H10s = zip(*[w.mat_def(0.0, 0.0, q, 1) for q in kz])
+    E = hm.energy_grid(-3.0, 3.0, 4096)
+    npts = 4096 * 16
+    rgf.init(0)
+    cursor = [0]
+
+    def batch():
+        idx = (np.arange(B) * 257 + cursor[0] * 7919) % npts
+        cursor[0] += 1
+        ie, ik = idx // 16, idx % 16
+        return E[ie], [H00s[k] for k in ik], [H10s[k] for k in ik]
+
+    for _ in range(args.warmup):
+        En, h0, h1 = batch()
+        rgf.sancho(h0, h1, En, eta=1e-6, want_gbb=False)
+    rgf.set_kernel_timing(True)
+    acc = {"gemm_ms": 0.0, "gemm_flops": 0.0, "inv_ms": 0.0, "inv_flops": 0.0}
+    iters, conv = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        En, h0, h1 = batch()
+        _, _, it, cv = rgf.sancho(h0, h1, En, eta=1e-6, want_gbb=False)
+        t = rgf.last_timing()
+        for k in acc:
+            acc[k] += t[k]
+        iters.append(it)
+        conv.append(cv)
+    elapsed = time.perf_counter() - t0
+    rgf.set_kernel_timing(False)
+    it = np.concatenate(iters)
+    cv = np.concatenate(conv)
+    points = B * args.steps
+    n3 = float(nb) ** 3
+    flops = float(np.sum(it * 56 + 16)) * n3  # K (INV + 6 GEMM) + INV + 1 GEMM (G00 and the final product)
+    gemm_tf = acc["gemm_flops"] / (acc["gemm_ms"] * 1e-3) / 1e12 if acc["gemm_ms"] > 0 else 0.0
+    res = {
+        "metric": "lead surface Green's functions per second (plain Sancho-Rubio, nb=256, eta=1e-6)",
+        "value": round(points / elapsed, 2), "unit": "lead points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 generator)",
+        "config": {"workload": f"lead decimation, nb={nb}, 4096 E in [-3,3] eV x 16 kz, eta=1e-6, "
+                               f"{B} points per step", "nb": nb, "points_per_step": B},
+        "roofline": {"bound": "mfma", "achieved": round(gemm_tf, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": round(gemm_tf / PEAK_FP64, 4), "traffic": None,
+                     "kernel": "zgemm_mfma_kernel (batched complex128 GEMM, decimation products)"},
+        "path": {"iterations_avg": round(float(it.mean()), 2), "iterations_max": int(it.max()),
+                 "not_converged": int((~cv).sum()),
+                 "algorithmic_tflops": round(flops / elapsed / 1e12, 3),
+                 "inversion_tflops": round(acc["inv_flops"] / (acc["inv_ms"] * 1e-3) / 1e12, 3)
+                 if acc["inv_ms"] > 0 else 0.0,
+                 "kernel_share_of_wall": round((acc["gemm_ms"] + acc["inv_ms"]) / (1e3 * elapsed), 3)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        import concurrent.futures as cf
+
+        import oracle
+        flavour = "mkl" if oracle.available("mkl") else "plain"
+        if flavour == "mkl":  # one MKL thread per call; the points run in parallel threads
+            for name in ("MKL_Set_Num_Threads", "mkl_set_num_threads_"):
+                fn = getattr(oracle.lib("mkl"), name, None)
+                if fn is not None:
+                    fn(C.c_int(1)) if name[0] == "M" else fn(C.byref(C.c_int(1)))
+                    break
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = [(float(E[i * 509 % 4096]), (i * 5) % 16) for i in range(threads)]
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda ek: oracle.sancho(None, H00s[ek[1]], H10s[ek[1]], ek[0], eta=1e-6,
+                                                 want_gbb=False, flavour=flavour), sample))
+        tc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(len(sample) / tc, 4), "unit": "lead points/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{len(sample)} points, one per thread ({flavour} LAPACK/BLAS, "
+                                         f"one BLAS thread each), {tc:.1f}s"}
+    return res
+
+
 def run_uneven(args, hm, rgf) -> dict:
     """Uneven slabs: 80 slabs with sizes drawn uniformly from {128, 192, ..., 768}
     (direct block construction), 2048 energies in [-2, 2] eV x 8 kz on one
@@ -359,9 +452,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1002)
-    ap.add_argument("--workload", choices=("rgf", "uneven", "gw", "corrconv", "vendor-zgemm"),
+    ap.add_argument("--workload", choices=("rgf", "uneven", "leads", "gw", "corrconv", "vendor-zgemm"),
                     default="rgf",
                     help="rgf: the headline RGF sweep; uneven: 80 slabs of 128..768 orbitals; "
+                         "leads: Sancho-Rubio surface Green's functions at nb=256, eta=1e-6; "
                          "gw: the GW energy-axis stage; corrconv: batched corr1d2/conv1d2; "
                          "vendor-zgemm: rocBLAS batched ZGEMM at the sweep's shape, for comparison")
     ap.add_argument("--gw-slabs", type=int, default=2)
@@ -388,6 +482,9 @@ def main() -> int:
 
     if args.workload == "uneven":
         print(json.dumps(run_uneven(args, hm, rgf)), flush=True)
+        return 0
+    if args.workload == "leads":
+        print(json.dumps(run_leads(args, hm, rgf)), flush=True)
         return 0
 
     nb, nx, B = args.nb, args.slabs, args.batch

@@ -274,7 +274,8 @@ typedef struct {
 } qt_timing;
 
 /* When enabled, every GEMM / inversion launch is bracketed by HIP events on
- * the library stream and summed into qt_timing. */
+ * the library stream and summed into qt_timing; qt_last_timing returns the
+ * sums of the last qt_rgf_solve or qt_sancho call. */
 int qt_set_kernel_timing(int enable);
 int qt_last_timing(qt_timing* t);
 

@@ -1345,6 +1345,9 @@ int qt_sancho(int32_t nm, int32_t npts, const double* En, double eta, int32_t nm
         const int64_t nn = (int64_t)nm * nm;
         std::vector<double> hz(2 * half);
         std::vector<int> hit(2 * half), hact(2 * half);
+        c.timer.reset();
+        qt_timing tm{};
+        tm.batch = B;
         for (int p0 = 0; p0 < npts; p0 += B) {
             const int nb = std::min(B, npts - p0);
             for (int m = 0; m < nb; ++m) {
@@ -1379,8 +1382,18 @@ int qt_sancho(int32_t nm, int32_t npts, const double* En, double eta, int32_t nm
                     QT_HIP(hipMemcpy(GBB[p0 + m], GBBdev + m * nn, sizeof(double2) * nn, hipMemcpyDeviceToHost));
                 if (iters) iters[p0 + m] = hit[m];
                 if (converged) converged[p0 + m] = hact[m] ? 0 : 1;
+                tm.lead_iterations += hit[m];
             }
         }
+        // kernel timings of this call for qt_last_timing (GEMM and inversion launches)
+        c.timer.flush();
+        tm.gemm_ms = c.timer.ms[0];
+        tm.gemm_flops = c.timer.flops[0];
+        tm.gemm_launches = c.timer.launches[0];
+        tm.inv_ms = c.timer.ms[1];
+        tm.inv_flops = c.timer.flops[1];
+        tm.inv_launches = c.timer.launches[1];
+        c.last = tm;
         return QT_OK;
     } catch (const hip_error& e) {
         return status_of(e);
