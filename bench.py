@@ -380,7 +380,7 @@ def run_leads(args, hm, rgf) -> dict:
                     fn(C.c_int(1)) if name[0] == "M" else fn(C.byref(C.c_int(1)))
                     break
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        sample = [(float(E[i * 509 % 4096]), (i * 5) % 16) for i in range(threads)]
+        sample = [(float(E[i * 509 % 4096]), (i * 5) % 16) for i in range(16 * threads)]
         t0 = time.perf_counter()
         with cf.ThreadPoolExecutor(threads) as ex:
             list(ex.map(lambda ek: oracle.sancho(None, H00s[ek[1]], H10s[ek[1]], ek[0], eta=1e-6,
@@ -388,7 +388,7 @@ def run_leads(args, hm, rgf) -> dict:
         tc = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": round(len(sample) / tc, 4), "unit": "lead points/s",
                                "cores": threads, "kind": "port",
-                               "sample": f"{len(sample)} points, one per thread ({flavour} LAPACK/BLAS, "
+                               "sample": f"{len(sample)} points on {threads} threads ({flavour} LAPACK/BLAS, "
                                          f"one BLAS thread each), {tc:.1f}s"}
     return res
 

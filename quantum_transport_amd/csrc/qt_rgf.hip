@@ -18,6 +18,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/qt_rgf.h"
@@ -1336,11 +1337,19 @@ int qt_sancho(int32_t nm, int32_t npts, const double* En, double eta, int32_t nm
         (void)hipMemGetInfo(&freeb, &total);
         while (B > 1 && work_bytes(h, (B + 1) / 2, false) > 0.8 * (freeb + c.arena.cap)) B = B * 3 / 4;
         const int half = (B + 1) / 2;  // the workspace holds 2*half members
-        c.arena.reserve(work_bytes(h, half, false) + sizeof(double2) * (size_t)nm * nm * (2 * half + 2));
+        // distinct H00 / H10 blocks of a batch (points usually share the
+        // blocks of a few k-points) are uploaded once into UCAP staging slots
+        // each and spread over the members on the device
+        const int UCAP = std::min(B, 64);
+        c.arena.reserve(work_bytes(h, half, false) + sizeof(double2) * (size_t)nm * nm * (2 * half + 2) +
+                        sizeof(double2) * (size_t)nm * nm * 2 * UCAP + sizeof(int) * 2 * (size_t)B + 1024);
         Work w;
         work_carve(c.arena, w, h, half, false);
         double2* Sdev = S00 ? c.arena.take<double2>((size_t)nm * nm) : nullptr;
         double2* GBBdev = GBB ? c.arena.take<double2>((size_t)nm * nm * 2 * half) : nullptr;
+        double2* U00 = c.arena.take<double2>((size_t)nm * nm * UCAP);
+        double2* U10 = c.arena.take<double2>((size_t)nm * nm * UCAP);
+        int* ukx = c.arena.take<int>(2 * (size_t)B);
         if (Sdev) QT_HIP(hipMemcpy(Sdev, S00, sizeof(double2) * nm * nm, hipMemcpyHostToDevice));
         const int64_t nn = (int64_t)nm * nm;
         std::vector<double> hz(2 * half);
@@ -1350,10 +1359,37 @@ int qt_sancho(int32_t nm, int32_t npts, const double* En, double eta, int32_t nm
         tm.batch = B;
         for (int p0 = 0; p0 < npts; p0 += B) {
             const int nb = std::min(B, npts - p0);
+            std::unordered_map<const void*, int> u0, u1;
+            std::vector<int> k01(2 * (size_t)nb);
+            std::vector<const void*> l0, l1;
             for (int m = 0; m < nb; ++m) {
                 hz[m] = En[p0 + m];
-                QT_HIP(hipMemcpy(w.T[13] + m * nn, H00[p0 + m], sizeof(double2) * nn, hipMemcpyHostToDevice));
-                QT_HIP(hipMemcpy(w.T[11] + m * nn, H10[p0 + m], sizeof(double2) * nn, hipMemcpyHostToDevice));
+                auto i0 = u0.emplace(H00[p0 + m], (int)l0.size());
+                if (i0.second) l0.push_back(H00[p0 + m]);
+                auto i1 = u1.emplace(H10[p0 + m], (int)l1.size());
+                if (i1.second) l1.push_back(H10[p0 + m]);
+                k01[m] = i0.first->second;
+                k01[nb + m] = i1.first->second;
+            }
+            if ((int)l0.size() <= UCAP && (int)l1.size() <= UCAP && ((int)l0.size() < nb || (int)l1.size() < nb)) {
+                for (size_t i = 0; i < l0.size(); ++i)
+                    QT_HIP(hipMemcpy(U00 + i * nn, l0[i], sizeof(double2) * nn, hipMemcpyHostToDevice));
+                for (size_t i = 0; i < l1.size(); ++i)
+                    QT_HIP(hipMemcpy(U10 + i * nn, l1[i], sizeof(double2) * nn, hipMemcpyHostToDevice));
+                QT_HIP(hipMemcpy(ukx, k01.data(), sizeof(int) * 2 * nb, hipMemcpyHostToDevice));
+                ZOp s0, s1;
+                s0.p = U00;
+                s0.ld = nm;
+                s0.sk = nn;
+                s1 = s0;
+                s1.p = U10;
+                copy_blocks(c.stream, nm, nb, w.T[13], nn, s0, ukx, 0, false);
+                copy_blocks(c.stream, nm, nb, w.T[11], nn, s1, ukx + nb, 0, false);
+            } else {
+                for (int m = 0; m < nb; ++m) {
+                    QT_HIP(hipMemcpy(w.T[13] + m * nn, H00[p0 + m], sizeof(double2) * nn, hipMemcpyHostToDevice));
+                    QT_HIP(hipMemcpy(w.T[11] + m * nn, H10[p0 + m], sizeof(double2) * nn, hipMemcpyHostToDevice));
+                }
             }
             QT_HIP(hipMemcpy(w.zre, hz.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
             SanchoIO io;
