@@ -79,6 +79,18 @@ struct Ctx {
     std::mutex mu;
     int* host_count = nullptr;  // pinned
     hipEvent_t ev[4];
+    char* staging = nullptr;    // pinned host staging for block outputs (grown on demand)
+    size_t staging_cap = 0;
+    char* host_staging(size_t bytes) {
+        if (bytes > staging_cap) {
+            if (staging) QT_HIP(hipHostFree(staging));
+            staging = nullptr;
+            staging_cap = 0;
+            QT_HIP(hipHostMalloc((void**)&staging, bytes));
+            staging_cap = bytes;
+        }
+        return staging;
+    }
 };
 
 static Ctx* g_ctx = nullptr;
@@ -984,6 +996,7 @@ int qt_finish(void) {
             for (int i = 0; i < 4096; ++i) (void)hipEventDestroy(c->timer.ev[k][i]);
     for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c->ev[i]);
     (void)hipHostFree(c->host_count);
+    if (c->staging) (void)hipHostFree(c->staging);
     (void)hipStreamDestroy(c->stream);
     delete c;
     g_ctx = nullptr;
@@ -1412,10 +1425,15 @@ int qt_sancho(int32_t nm, int32_t npts, const double* En, double eta, int32_t nm
             QT_HIP(hipStreamSynchronize(c.stream));
             QT_HIP(hipMemcpy(hit.data(), w.lead_it, sizeof(int) * nb, hipMemcpyDeviceToHost));
             QT_HIP(hipMemcpy(hact.data(), w.active, sizeof(int) * nb, hipMemcpyDeviceToHost));
+            // one copy of the batch's blocks into pinned memory, then to the
+            // caller's blocks (one pageable copy per block is several times slower)
+            const size_t bbytes = sizeof(double2) * (size_t)nn;
+            char* stg = c.host_staging(bbytes * nb * (GBB ? 2 : 1));
+            QT_HIP(hipMemcpy(stg, w.T[12], bbytes * nb, hipMemcpyDeviceToHost));
+            if (GBB) QT_HIP(hipMemcpy(stg + bbytes * nb, GBBdev, bbytes * nb, hipMemcpyDeviceToHost));
             for (int m = 0; m < nb; ++m) {
-                QT_HIP(hipMemcpy(G00[p0 + m], w.T[12] + m * nn, sizeof(double2) * nn, hipMemcpyDeviceToHost));
-                if (GBB && GBB[p0 + m])
-                    QT_HIP(hipMemcpy(GBB[p0 + m], GBBdev + m * nn, sizeof(double2) * nn, hipMemcpyDeviceToHost));
+                std::memcpy(G00[p0 + m], stg + bbytes * m, bbytes);
+                if (GBB && GBB[p0 + m]) std::memcpy(GBB[p0 + m], stg + bbytes * (nb + m), bbytes);
                 if (iters) iters[p0 + m] = hit[m];
                 if (converged) converged[p0 + m] = hact[m] ? 0 : 1;
                 tm.lead_iterations += hit[m];
