@@ -303,7 +303,7 @@ def run_mixed_leg(args, rgf, H, Eall, Kall, nb: int, nx: int, fp64_value: float)
 
 def run_leads(args, hm, rgf) -> dict:
     """Contact surface Green's functions alone (the lead stage of every point;
-    the shape on which a learned warm start would be judged): lead block
+    the shape a learned warm start would be measured against): lead block
     nb = 256 from the same generator, 4096 energies in [-3, 3] eV x 16 kz in
     [0, pi), eta = 1e-6, plain Sancho-Rubio (rgf_mod.f90:482-568, stop test
     sum |H01| < 1e-100, at most 50 steps) through qt_sancho.  A step is one
