@@ -174,6 +174,12 @@ __device__ __forceinline__ double wave_max(double x) {
 // the instruction cache once per panel): the panel column is picked from the
 // registers by selects, the pivot search is a DPP maximum plus a ballot for
 // the first row holding it, and row updates are four FMAs per element.
+// The pivot row is not divided by its pivot when chosen: its thread keeps the
+// row w unscaled with the factor s = 1/pivot (true row = s w, w[c] = 1), and
+// multiplies once at the write-back.  Later eliminations keep their form on
+// w (the multiplier g = w[c'] d' and w[c'] <- -g carry the factor), so the
+// thread owning the pivot does no extra work per column; the panel is FP64
+// vector-bound and every wave waits at the barrier for the slowest one.
 template <int NB, int NT>
 __global__ __launch_bounds__(NT) void gj_panel_loop_kernel(double2* A, int64_t sa, int n, int kb,
                                                            int* pivp, int* info, const int* active) {
@@ -197,6 +203,7 @@ __global__ __launch_bounds__(NT) void gj_panel_loop_kernel(double2* A, int64_t s
     __syncthreads();
     bool mine = r < n && !((used[r >> 5] >> (r & 31)) & 1u);
     int zero_col = 0;                      // first zero pivot (1-based), tid 0
+    double2 srow = make_double2(1.0, 0.0);  // this row's pending factor (true row = srow * v)
 
     double2 v[NB];
 #pragma unroll
@@ -253,9 +260,8 @@ __global__ __launch_bounds__(NT) void gj_panel_loop_kernel(double2* A, int64_t s
         const double2 d = (pivv.x == 0.0 && pivv.y == 0.0) ? make_double2(0.0, 0.0) : crecip(pivv);
         double2 vnew;
         if (r == p) {
-#pragma unroll
-            for (int cc = 0; cc < NB; ++cc) v[cc] = cmul(v[cc], d);
-            vnew = d;
+            srow = d;  // the row stays unscaled; true row = d * v, true v[c] = d
+            vnew = make_double2(1.0, 0.0);
         } else {
             const double2 gf = cmul(vc, d);  // f / pivot
 #pragma unroll
@@ -276,9 +282,10 @@ __global__ __launch_bounds__(NT) void gj_panel_loop_kernel(double2* A, int64_t s
     }
     }
     if (r < n) {
+        const bool scaled = srow.x != 1.0 || srow.y != 0.0;
 #pragma unroll
         for (int c = 0; c < NB; ++c)
-            if (c < ncols) Ab[r + (int64_t)(kb + c) * n] = v[c];
+            if (c < ncols) Ab[r + (int64_t)(kb + c) * n] = scaled ? cmul(v[c], srow) : v[c];
     }
     if (tid < ncols) pv[kb + tid] = pcol[tid];
     if (tid == 0 && zero_col != 0 && info[b] == 0) info[b] = zero_col;
