@@ -245,6 +245,18 @@ int qt_gw_wsolve_batched(int32_t nm, int32_t nx, int32_t nw, const void* Vii, co
                          const void* PR, const void* PL, const void* PG, void* WR, void* WL, void* WG,
                          int32_t flags);
 
+/* e-photon self-energies in the monochromatic assumption, replaces
+ * calc_sigma_ephoton_monochromatic_ext (gw_mod.f90:1075-1122):
+ *   Sigma^<>(E) = f M_ix [G^<>(E - hw) + G^<>(E + hw)] M_ix,  f = pre_fact intensity / hw^2,
+ *   Sigma^R = i Im(Sigma^> - Sigma^<) / 2 (real part zero).
+ * Fortran layouts: Mii (nm, nm, length); Gl, Gg (nm, nm, length, nen + 2 nop),
+ * the energy axis extended by nop points (hw = nop dE) on each side, so
+ * output energy ie reads G at ie and ie + 2 nop; Sr, Sl, Sg (nm, nm, length, nen).
+ * hw = 0 is QT_ERR_INVALID.  flags: QT_F_DEVICE_OUTPUTS -> device pointers. */
+int qt_gw_sigma_ephoton(int32_t nm, int32_t length, int32_t nen, int32_t nop, double pre_fact,
+                        double intensity, double hw, const void* Mii, const void* Gl, const void* Gg,
+                        void* Sr, void* Sl, void* Sg, int32_t flags);
+
 /* S += alpha (Snew - S) over n complex128 samples (gw_mod.f90:747-749). */
 int qt_gw_mix(int64_t n, double alpha, void* S, const void* Snew, int32_t flags);
 

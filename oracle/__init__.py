@@ -12,7 +12,7 @@ Functions mirror the reference routines they restate:
 * :func:`rgf_batch`    the E x k loop of negf_mod.f90:115-127
 * :func:`corr1d2`, :func:`conv1d2`  fft_mod.f90:58-91, 129-168
 * :func:`green_rgf_rs` gw_mod.f90:1131-1345
-* :func:`gw_polarization`, :func:`gw_selfenergy`, :func:`gw_mix`
+* :func:`gw_polarization`, :func:`gw_selfenergy`, :func:`gw_mix`, :func:`gw_sigma_ephoton`
                        gw_mod.f90:444-519, 641-743, 747-749
 """
 from __future__ import annotations
@@ -299,6 +299,22 @@ def gw_mix(S, Snew, alpha: float, flavour: str = "plain"):
     lib(flavour).qto_gw_mix(C.c_int64(S.size), C.c_double(alpha), S.ctypes.data_as(C.c_void_p),
                             Snew.ctypes.data_as(C.c_void_p))
     return S
+
+
+def gw_sigma_ephoton(Mii, Gl, Gg, nop: int, pre_fact: float, intensity: float, hw: float,
+                     flavour: str = "plain"):
+    """e-photon Sigma^R, Sigma^<, Sigma^> (gw_mod.f90:1075-1122) on Fortran-ordered
+    Mii (nm, nm, length) and G (nm, nm, length, nen + 2 nop)."""
+    M = np.asfortranarray(Mii, dtype=np.complex128)
+    G = [np.asfortranarray(a, dtype=np.complex128) for a in (Gl, Gg)]
+    nm, _, length = M.shape
+    nen = G[0].shape[3] - 2 * nop
+    S = [np.zeros((nm, nm, length, nen), np.complex128, order="F") for _ in range(3)]
+    lib(flavour).qto_gw_sigma_ephoton(C.c_int(nm), C.c_int(length), C.c_int(nen), C.c_int(nop),
+                                      C.c_double(pre_fact), C.c_double(intensity), C.c_double(hw),
+                                      *(a.ctypes.data_as(C.c_void_p) for a in [M] + G),
+                                      *(x.ctypes.data_as(C.c_void_p) for x in S))
+    return tuple(S)
 
 
 def gw_wsolve(Vii, V1i, PR, PL, PG, flavour: str = "plain"):

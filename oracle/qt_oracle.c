@@ -1152,6 +1152,43 @@ void qto_gw_mix(int64_t n, double alpha, qto_c* S, const qto_c* Snew) {
 }
 #undef SER
 
+/* e-photon self-energies, monochromatic field: calc_sigma_ephoton_monochromatic_ext
+ * (gw_mod.f90:1075-1122).  Per energy ie and slab ix (loop order of
+ * gw_mod.f90:1092-1116): A = G(ie2 - nop) + G(ie2 + nop) with ie2 = ie + nop
+ * (0-based: G at ie and ie + 2 nop), B = M A, A = B M (two ZGEMMs); then the
+ * scaling by pre_fact intensity / hw^2 (gw_mod.f90:1119-1120) and
+ * Sigma^R = 0 Re + i Im(Sigma^> - Sigma^<) / 2 (gw_mod.f90:1121). */
+void qto_gw_sigma_ephoton(int nm, int length, int nen, int nop, double pre_fact, double intensity,
+                          double hw, const qto_c* Mii, const qto_c* Gl, const qto_c* Gg, qto_c* Sr,
+                          qto_c* Sl, qto_c* Sg) {
+    const size_t nn = (size_t)nm * nm;
+    qto_c* A = (qto_c*)malloc(nn * sizeof(qto_c));
+    qto_c* B = (qto_c*)malloc(nn * sizeof(qto_c));
+    for (int ie = 0; ie < nen; ++ie)
+        for (int ix = 0; ix < length; ++ix) {
+            const qto_c* M = Mii + (size_t)ix * nn;
+            const size_t o = ((size_t)ie * length + ix) * nn;
+            const size_t lo = ((size_t)ie * length + ix) * nn;
+            const size_t hi = ((size_t)(ie + 2 * nop) * length + ix) * nn;
+            for (int x = 0; x < 2; ++x) {
+                const qto_c* G = x ? Gg : Gl;
+                qto_c* S = x ? Sg : Sl;
+                for (size_t t = 0; t < nn; ++t) A[t] = G[lo + t] + G[hi + t];
+                qto_zgemm(QTO_OP_N, QTO_OP_N, nm, nm, nm, 1.0, M, nm, A, nm, 0.0, B, nm);
+                qto_zgemm(QTO_OP_N, QTO_OP_N, nm, nm, nm, 1.0, B, nm, M, nm, 0.0, A, nm);
+                memcpy(S + o, A, nn * sizeof(qto_c));
+            }
+        }
+    const size_t ns = nn * (size_t)length * nen;
+    for (size_t t = 0; t < ns; ++t) {
+        Sg[t] = Sg[t] * pre_fact * intensity / (hw * hw);
+        Sl[t] = Sl[t] * pre_fact * intensity / (hw * hw);
+    }
+    for (size_t t = 0; t < ns; ++t) Sr[t] = I * (cimag(Sg[t] - Sl[t]) / 2.0);
+    free(A);
+    free(B);
+}
+
 /* ------------------------------------------------------------------ */
 /* W solve (called at gw_mod.f90:597, not defined in the reference)    */
 /* ------------------------------------------------------------------ */

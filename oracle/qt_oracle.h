@@ -147,6 +147,11 @@ int qto_gw_wsolve(int nm, int nx, const qto_c* Vii, const qto_c* V1i, const qto_
                   const qto_c* PL, const qto_c* PG, qto_c* WR, qto_c* WL, qto_c* WG);
 /* S += alpha (Snew - S), gw_mod.f90:747-749 */
 void qto_gw_mix(int64_t n, double alpha, qto_c* S, const qto_c* Snew);
+/* e-photon self-energies (gw_mod.f90:1075-1122); Fortran layouts: Mii (nm,nm,length),
+ * Gl/Gg (nm,nm,length,nen+2nop), Sr/Sl/Sg (nm,nm,length,nen). */
+void qto_gw_sigma_ephoton(int nm, int length, int nen, int nop, double pre_fact, double intensity,
+                          double hw, const qto_c* Mii, const qto_c* Gl, const qto_c* Gg, qto_c* Sr,
+                          qto_c* Sl, qto_c* Sg);
 
 /* Number of threads the batch driver will use, and the BLAS flavour
  * ("plain" restatement or "mkl"). */

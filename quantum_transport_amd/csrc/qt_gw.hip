@@ -412,6 +412,22 @@ __global__ __launch_bounds__(NT) void gw_sigma_kernel(GwDims d, const int* __res
     }
 }
 
+// e-photon self-energy, monochromatic field: the two energy-shifted blocks
+// summed (S = G(E - hw) + G(E + hw), one contiguous stream shifted by
+// 2 nop energies), then Sigma^R = i Im(Sigma^> - Sigma^<) / 2.
+__global__ void eph_sum_kernel(int64_t n, int64_t shift, const double2* __restrict__ G, double2* S) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = G[t], b = G[t + shift];
+        S[t] = make_double2(a.x + b.x, a.y + b.y);
+    }
+}
+
+__global__ void eph_retarded_kernel(int64_t n, const double2* __restrict__ Sl, const double2* __restrict__ Sg,
+                                    double2* Sr) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        Sr[t] = make_double2(0.0, 0.5 * (Sg[t].y - Sl[t].y));
+}
+
 __global__ void gw_mix_kernel(int64_t n, double alpha, double2* S, const double2* __restrict__ Snew) {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         const double2 a = S[t], b = Snew[t];
@@ -523,6 +539,20 @@ void gw_mix(hipStream_t st, int64_t n, double alpha, double2* S, const double2* 
     if (n <= 0) return;
     const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(gw_mix_kernel, dim3((unsigned)blocks), dim3(256), 0, st, n, alpha, S, Snew);
+    QT_HIP(hipGetLastError());
+}
+
+void eph_sum(hipStream_t st, int64_t n, int64_t shift, const double2* G, double2* S) {
+    if (n <= 0) return;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(eph_sum_kernel, dim3((unsigned)blocks), dim3(256), 0, st, n, shift, G, S);
+    QT_HIP(hipGetLastError());
+}
+
+void eph_retarded(hipStream_t st, int64_t n, const double2* Sl, const double2* Sg, double2* Sr) {
+    if (n <= 0) return;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(eph_retarded_kernel, dim3((unsigned)blocks), dim3(256), 0, st, n, Sl, Sg, Sr);
     QT_HIP(hipGetLastError());
 }
 

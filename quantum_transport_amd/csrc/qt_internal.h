@@ -187,6 +187,10 @@ struct GwArgs {
 void gw_polarization(hipStream_t s, const GwArgs& a, const int* elems, int nelem);
 void gw_selfenergy(hipStream_t s, const GwArgs& a, const int* pairs, int npair);
 void gw_mix(hipStream_t s, int64_t n, double alpha, double2* S, const double2* Snew);
+// S[t] = G[t] + G[t + shift] for t < n (the two photon-shifted energies)
+void eph_sum(hipStream_t s, int64_t n, int64_t shift, const double2* G, double2* S);
+// Sr[t] = i Im(Sg[t] - Sl[t]) / 2
+void eph_retarded(hipStream_t s, int64_t n, const double2* Sl, const double2* Sg, double2* Sr);
 
 // out[b] (real n x n) = Re(X1[b]) - Re(X2[b])
 void real_diff(hipStream_t s, int n, int batch, double* out, int64_t sout, ZOp X1, ZOp X2);

@@ -1770,6 +1770,93 @@ int qt_gw_mix(int64_t n, double alpha, void* S, const void* Snew, int32_t flags)
     }
 }
 
+// ---------------------------------------------------------------- e-photon
+// calc_sigma_ephoton_monochromatic_ext (gw_mod.f90:1075-1122): per slab ix
+// and energy ie, Sigma^<> = f M_ix [G^<>(ie) + G^<>(ie + 2 nop)] M_ix with
+// f = pre_fact intensity / hw^2, then Sigma^R = i Im(Sigma^> - Sigma^<) / 2.
+// Members b = ix + length ie are consecutive nm x nm blocks; M_ix is picked
+// per member through the GEMM's index table (kidx[b] = b mod length), so each
+// product is one launch over all length x nen members.
+int qt_gw_sigma_ephoton(int32_t nm, int32_t length, int32_t nen, int32_t nop, double pre_fact,
+                        double intensity, double hw, const void* Mii, const void* Gl, const void* Gg,
+                        void* Sr, void* Sl, void* Sg, int32_t flags) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (nm <= 0 || length <= 0 || nen < 0 || nop < 0 || !(hw != 0.0)) return QT_ERR_INVALID;
+    if (nen > 0 && (!Mii || !Gl || !Gg || !Sr || !Sl || !Sg)) return QT_ERR_INVALID;
+    if ((int64_t)length * nen > 0x7fffffff) return QT_ERR_UNSUPPORTED;
+    if (nen == 0) return QT_OK;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        const bool dev = (flags & QT_F_DEVICE_OUTPUTS) != 0;
+        const int64_t nn = (int64_t)nm * nm;
+        const int batch = length * nen;
+        const int64_t ns = nn * batch;                          // samples of one Sigma
+        const int64_t ng = nn * length * ((int64_t)nen + 2 * nop);  // samples of one G
+        const size_t sb = sizeof(double2) * (size_t)ns, gb = sizeof(double2) * (size_t)ng;
+        const size_t mb = sizeof(double2) * (size_t)(nn * length);
+        c.arena.reserve(sb + sizeof(int) * (size_t)batch + (dev ? 0 : 2 * gb + mb + 3 * sb) + 8192);
+        c.arena.reset();
+        double2* T = c.arena.take<double2>(ns);
+        int* kidx = c.arena.take<int>(batch);
+        const double2 *dM = (const double2*)Mii, *dGl = (const double2*)Gl, *dGg = (const double2*)Gg;
+        double2 *dSr = (double2*)Sr, *dSl = (double2*)Sl, *dSg = (double2*)Sg;
+        if (!dev) {
+            double2* m = c.arena.take<double2>(nn * length);
+            double2* gl = c.arena.take<double2>(ng);
+            double2* gg = c.arena.take<double2>(ng);
+            QT_HIP(hipMemcpyAsync(m, Mii, mb, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(gl, Gl, gb, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(gg, Gg, gb, hipMemcpyHostToDevice, c.stream));
+            dM = m;
+            dGl = gl;
+            dGg = gg;
+            dSr = c.arena.take<double2>(ns);
+            dSl = c.arena.take<double2>(ns);
+            dSg = c.arena.take<double2>(ns);
+        }
+        {
+            std::vector<int> k((size_t)batch);
+            for (int b = 0; b < batch; ++b) k[(size_t)b] = b % length;
+            QT_HIP(hipMemcpyAsync(kidx, k.data(), sizeof(int) * (size_t)batch, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipStreamSynchronize(c.stream));
+        }
+        const double f = pre_fact * intensity / (hw * hw);
+        const int64_t shift = 2 * (int64_t)nop * length * nn;
+        const double2* G[2] = {dGl, dGg};
+        double2* S[2] = {dSl, dSg};
+        for (int x = 0; x < 2; ++x) {
+            eph_sum(c.stream, ns, shift, G[x], S[x]);  // S = G(E - hw) + G(E + hw)
+            ZGemmArgs g;                                // T = M_ix S
+            g.M = g.N = g.K = nm;
+            g.batch = batch;
+            g.A = ZOp{dM, nm, 0, nn};
+            g.B = ZOp{S[x], nm, nn, 0};
+            g.C = T;
+            g.ldc = nm;
+            g.sbc = nn;
+            g.kidx = kidx;
+            zgemm(c.stream, 0, 0, g, &c.timer);
+            g.A = ZOp{T, nm, nn, 0};                    // Sigma = f T M_ix
+            g.B = ZOp{dM, nm, 0, nn};
+            g.C = S[x];
+            g.alpha = f;
+            zgemm(c.stream, 0, 0, g, &c.timer);
+        }
+        eph_retarded(c.stream, ns, dSl, dSg, dSr);
+        QT_HIP(hipStreamSynchronize(c.stream));
+        if (!dev) {
+            QT_HIP(hipMemcpy(Sr, dSr, sb, hipMemcpyDeviceToHost));
+            QT_HIP(hipMemcpy(Sl, dSl, sb, hipMemcpyDeviceToHost));
+            QT_HIP(hipMemcpy(Sg, dSg, sb, hipMemcpyDeviceToHost));
+        }
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
 // ---------------------------------------------------------------- W solve
 // W^R = V (I - P^R V)^-1 and W^<,> = W^R P^<,> W^R^H (diagonal blocks) for B
 // frequencies at once.  V (Vii, V1i: nm x nm x nx, V1i[i] = V_{i,i-1}) is

@@ -39,7 +39,7 @@ EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham
             "qt_memcpy_d2h", "qt_synchronize", "qt_set_kernel_timing", "qt_last_timing",
             "qt_zgemm_batched", "qt_zgemm_batched_prec", "qt_invert_batched", "qt_corr1d2_batched",
             "qt_conv1d2_batched", "qt_gw_polarization", "qt_gw_selfenergy", "qt_gw_mix",
-            "qt_gw_wsolve_batched"]
+            "qt_gw_wsolve_batched", "qt_gw_sigma_ephoton"]
 
 
 class QTError(RuntimeError):
@@ -105,6 +105,8 @@ def lib() -> C.CDLL:
             + [C.c_int32]
         L.qt_gw_mix.argtypes = [C.c_int64, C.c_double, C.c_void_p, C.c_void_p, C.c_int32]
         L.qt_gw_wsolve_batched.argtypes = [C.c_int32] * 3 + [C.c_void_p] * 8 + [C.c_int32]
+        L.qt_gw_sigma_ephoton.argtypes = [C.c_int32] * 4 + [C.c_double] * 3 + [C.c_void_p] * 6 \
+            + [C.c_int32]
         _lib = L
     return _lib
 
@@ -465,6 +467,31 @@ def gw_selfenergy(Gr, Gl, Gg, Wr, Wl, Wg, nky: int, nkz: int, ndiag: int, dEn: f
     S = [np.empty_like(arrs[0]) for _ in range(3)]
     check(lib().qt_gw_selfenergy(nen, nky, nkz, nx, nm, ndiag, dEn, *(a.ctypes.data for a in arrs),
                                  *(x.ctypes.data for x in S), 0))
+    return tuple(S)
+
+
+def gw_sigma_ephoton(Mii, G_lesser, G_greater, nop: int, pre_fact: float, intensity: float,
+                     hw: float):
+    """e-photon self-energies Sigma^R, Sigma^<, Sigma^> in the monochromatic
+    assumption (calc_sigma_ephoton_monochromatic_ext, gw_mod.f90:1075-1122) on
+    the device.  Mii: (nm, nm, length); G_lesser, G_greater: (nm, nm, length,
+    nen + 2 nop), Fortran-ordered like the reference's arrays.  Returns three
+    (nm, nm, length, nen) arrays."""
+    init()
+    M = np.asfortranarray(Mii, dtype=np.complex128)
+    Gl = np.asfortranarray(G_lesser, dtype=np.complex128)
+    Gg = np.asfortranarray(G_greater, dtype=np.complex128)
+    if M.ndim != 3 or Gl.ndim != 4 or Gl.shape != Gg.shape:
+        raise ValueError("Mii must be (nm, nm, length), G (nm, nm, length, nen + 2 nop)")
+    nm, nm2, length = M.shape
+    if nm != nm2 or Gl.shape[:3] != (nm, nm, length):
+        raise ValueError("block shapes disagree")
+    nen = Gl.shape[3] - 2 * nop
+    if nop < 0 or nen < 0:
+        raise ValueError("energy axis shorter than 2 nop")
+    S = [np.zeros((nm, nm, length, nen), dtype=np.complex128, order="F") for _ in range(3)]
+    check(lib().qt_gw_sigma_ephoton(nm, length, nen, nop, pre_fact, intensity, hw, M.ctypes.data,
+                                    Gl.ctypes.data, Gg.ctypes.data, *(x.ctypes.data for x in S), 0))
     return tuple(S)
 
 
