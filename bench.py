@@ -192,6 +192,57 @@ def run_corrconv(args) -> dict:
             "cpu_baseline": None}
 
 
+def run_ephoton(args) -> dict:
+    """e-photon self-energies in the monochromatic assumption
+    (calc_sigma_ephoton_monochromatic_ext, gw_mod.f90:1075-1122) on
+    device-resident arrays: nm = 128 orbitals, 32 slabs, 512 energies, the
+    photon shift nop = 8 energies.  Algorithmic flops per (slab, energy): two
+    complex products for each of Sigma^< and Sigma^> (4 x 8 nm^3); the rate is
+    over the whole call (shifted sums, four GEMM launches, Sigma^R)."""
+    import torch
+    from quantum_transport_amd import rgf
+
+    nm, length, nen, nop = 128, 32, 512, 8
+    rgf.init(0)
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    nn = nm * nm
+    M = torch.randn(nn * length, dtype=torch.complex128, device=dev, generator=g)
+    G = [torch.randn(nn * length * (nen + 2 * nop), dtype=torch.complex128, device=dev, generator=g)
+         for _ in range(2)]
+    S = [torch.empty(nn * length * nen, dtype=torch.complex128, device=dev) for _ in range(3)]
+    torch.cuda.synchronize()
+    L = rgf.lib()
+
+    def call():
+        rgf.check(L.qt_gw_sigma_ephoton(nm, length, nen, nop, 1.0, 1.0, 0.1, M.data_ptr(),
+                                        G[0].data_ptr(), G[1].data_ptr(), *(x.data_ptr() for x in S),
+                                        rgf.QT_F_DEVICE_OUTPUTS))
+    for _ in range(max(args.warmup, 1)):
+        call()
+    L.qt_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call()
+    L.qt_synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    flops = 32.0 * nm ** 3 * length * nen
+    tf = flops / t / 1e12
+    return {"metric": "e-photon self-energy (slab, energy) blocks/s @nm=128",
+            "value": round(length * nen / t, 1), "unit": "blocks/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": max(args.warmup, 1),
+            "ms_per_step": round(1e3 * t, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (torch normal samples)",
+            "config": {"workload": f"Sigma^R/</> e-photon, nm={nm}, {length} slabs x {nen} energies, nop={nop}",
+                       "nm": nm, "slabs": length, "energies": nen, "nop": nop},
+            "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
+                         "frac": round(tf / 78.6, 4), "traffic": None,
+                         "kernel": "whole call: eph_sum + 4 x zgemm_mfma_kernel + eph_retarded"},
+            "cpu_baseline": None}
+
+
 def run_vendor_zgemm(args) -> dict:
     """The vendor library's batched complex128 GEMM (torch.bmm, which calls
     rocBLAS) at the sweep's product shape, for comparison with the
@@ -452,7 +503,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1002)
-    ap.add_argument("--workload", choices=("rgf", "uneven", "leads", "gw", "corrconv", "vendor-zgemm"),
+    ap.add_argument("--workload", choices=("rgf", "uneven", "leads", "gw", "corrconv", "ephoton",
+                                               "vendor-zgemm"),
                     default="rgf",
                     help="rgf: the headline RGF sweep; uneven: 80 slabs of 128..768 orbitals; "
                          "leads: Sancho-Rubio surface Green's functions at nb=256, eta=1e-6; "
@@ -472,6 +524,9 @@ def main() -> int:
         return 0
     if args.workload == "corrconv":
         print(json.dumps(run_corrconv(args)), flush=True)
+        return 0
+    if args.workload == "ephoton":
+        print(json.dumps(run_ephoton(args)), flush=True)
         return 0
     if args.workload == "vendor-zgemm":
         print(json.dumps(run_vendor_zgemm(args)), flush=True)

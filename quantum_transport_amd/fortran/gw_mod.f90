@@ -10,13 +10,15 @@
 ! reference's buffers: gw_polarization (P loop, gw_mod.f90:444-550),
 ! gw_selfenergy (Sigma loop, :641-743), gw_mix (:747-749) on the
 ! element-major by_blocks arrays (nen, nk, nx, nm*nm), and green_rgf_calc_w
-! (the W solve called at :597) on the energy-major (nm, nm, nx, nw) blocks.
+! (the W solve called at :597) on the energy-major (nm, nm, nx, nw) blocks,
+! and calc_sigma_ephoton_monochromatic_ext (:1075-1122) with its argument list.
 module gw_mod
     use iso_c_binding
     use qt_capi
     implicit none
     private
-    public :: green_RGF_RS, green_rgf_calc_w, gw_polarization, gw_selfenergy, gw_mix
+    public :: green_RGF_RS, green_rgf_calc_w, gw_polarization, gw_selfenergy, gw_mix, &
+              calc_sigma_ephoton_monochromatic_ext
 
     interface
         integer(c_int) function qt_gw_polarization(nen, nky, nkz, nx, nm, ndiag, dEn, spindeg, Gr, Gl, Gg, &
@@ -49,6 +51,14 @@ module gw_mod
             integer(c_int32_t), value :: nm, nx, nw, flags
             type(c_ptr), value :: Vii, V1i, PR, PL, PG, WR, WL, WG
         end function qt_gw_wsolve_batched
+
+        integer(c_int) function qt_gw_sigma_ephoton(nm, length, nen, nop, pre_fact, intensity, hw, Mii, &
+                                                    Gl, Gg, Sr, Sl, Sg, flags) bind(C, name="qt_gw_sigma_ephoton")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            integer(c_int32_t), value :: nm, length, nen, nop, flags
+            real(c_double), value :: pre_fact, intensity, hw
+            type(c_ptr), value :: Mii, Gl, Gg, Sr, Sl, Sg
+        end function qt_gw_sigma_ephoton
     end interface
 
 contains
@@ -173,6 +183,27 @@ contains
         call qt_check(qt_gw_mix(int(n, c_int64_t), real(alpha_mix, c_double), c_loc(Sigma), c_loc(Sigma_new), &
                                 0_c_int32_t), 'gw_mix')
     end subroutine gw_mix
+
+    ! e-photon self-energies in the monochromatic assumption:
+    ! Sig^<> = pre_fact intensity / hw^2 M [G^<>(E - hw) + G^<>(E + hw)] M,
+    ! Sig^R = i Im(Sig^> - Sig^<) / 2; G carries nop extra energies on each side
+    subroutine calc_sigma_ephoton_monochromatic_ext(nm, length, nen, En, nop, Mii, G_lesser, G_greater, &
+                                                    Sig_retarded, Sig_lesser, Sig_greater, pre_fact, &
+                                                    intensity, hw)
+        integer, intent(in) :: nm, length, nen, nop
+        real(8), intent(in) :: En(nen), pre_fact, intensity, hw
+        complex(8), intent(in), dimension(nm, nm, length), target :: Mii
+        complex(8), intent(in), dimension(nm, nm, length, nen + 2 * nop), target :: G_lesser, G_greater
+        complex(8), intent(inout), dimension(nm, nm, length, nen), target :: Sig_retarded, Sig_lesser, &
+                                                                           Sig_greater
+        call qt_ensure_init()
+        call qt_check(qt_gw_sigma_ephoton(int(nm, c_int32_t), int(length, c_int32_t), int(nen, c_int32_t), &
+                                          int(nop, c_int32_t), real(pre_fact, c_double), &
+                                          real(intensity, c_double), real(hw, c_double), c_loc(Mii), &
+                                          c_loc(G_lesser), c_loc(G_greater), c_loc(Sig_retarded), &
+                                          c_loc(Sig_lesser), c_loc(Sig_greater), 0_c_int32_t), &
+                      'calc_sigma_ephoton_monochromatic_ext')
+    end subroutine calc_sigma_ephoton_monochromatic_ext
 
     ! W^R = V (I - P^R V)^-1, W^<,> = W^R P^<,> W^R^H for nw frequencies in one
     ! call (the reference calls it per energy, gw_mod.f90:592-599); nbc = 0
