@@ -506,6 +506,35 @@ def gw_mix(S, Snew, alpha: float) -> np.ndarray:
     return S
 
 
+def save_sigma_checkpoint(path: str, iteration: int, Sigma_r, Sigma_lesser, Sigma_greater) -> None:
+    """Write the SCBA self-energies of one iteration to ``path`` (.npz, no
+    pickled objects).  The reference keeps no SCBA state between runs (it only
+    appends Id_iteration.dat per iteration, gw_mod.f90:426-428); this lets a
+    GW-SCBA loop resume at ``iteration + 1``.  The file is written under a
+    temporary name and renamed, so an interrupted write leaves the previous
+    checkpoint intact."""
+    arrs = {name: np.asarray(a, dtype=np.complex128) for name, a in
+            (("sigma_r", Sigma_r), ("sigma_lesser", Sigma_lesser), ("sigma_greater", Sigma_greater))}
+    if len({a.shape for a in arrs.values()}) != 1:
+        raise ValueError("the three self-energies must have one shape")
+    tmp = path + ".tmp"
+    with open(tmp, "wb") as fh:
+        np.savez(fh, iteration=np.int64(iteration),
+                 fortran_order=np.bool_(all(a.flags.f_contiguous and not a.flags.c_contiguous
+                                            for a in arrs.values())), **arrs)
+    os.replace(tmp, path)
+
+
+def load_sigma_checkpoint(path: str):
+    """(iteration, Sigma_r, Sigma_lesser, Sigma_greater) from a file written by
+    :func:`save_sigma_checkpoint`; arrays come back in the memory order saved."""
+    with np.load(path, allow_pickle=False) as z:
+        it = int(z["iteration"])
+        order = "F" if bool(z["fortran_order"]) else "C"
+        out = tuple(np.array(z[n], order=order) for n in ("sigma_r", "sigma_lesser", "sigma_greater"))
+    return (it,) + out
+
+
 def gw_wsolve(Vii, V1i, PR, PL, PG):
     """W^R, W^<, W^> diagonal blocks for a batch of frequencies on the device.
     Vii, V1i: (nm, nm, nx); PR/PL/PG: (nm, nm, nx, nw) (Fortran order, the

@@ -117,3 +117,25 @@ def test_gpu_ephoton_rejects_bad_arguments(qt):
         qt.gw_sigma_ephoton(M, Gl, Gg, 1, 1.0, 1.0, 0.0)  # hw = 0
     with pytest.raises(ValueError):
         qt.gw_sigma_ephoton(M, Gl, Gg, 3, 1.0, 1.0, 1.0)  # axis shorter than 2 nop
+
+
+def test_sigma_checkpoint_round_trip(tmp_path):
+    """SCBA self-energy checkpoint: bit-identical round trip, memory order and
+    iteration kept, a later save replaces the earlier one, shapes checked."""
+    from quantum_transport_amd import rgf
+    rng = np.random.default_rng(2)
+    S = [np.asfortranarray(_rand(rng, 3, 3, 2, 5)) for _ in range(3)]
+    path = str(tmp_path / "sigma.npz")
+    rgf.save_sigma_checkpoint(path, 4, *S)
+    it, *got = rgf.load_sigma_checkpoint(path)
+    assert it == 4
+    for g, s in zip(got, S):
+        np.testing.assert_array_equal(g, s)
+        assert g.flags.f_contiguous
+    rgf.save_sigma_checkpoint(path, 5, *(2 * s for s in S))
+    it, *got = rgf.load_sigma_checkpoint(path)
+    assert it == 5
+    np.testing.assert_array_equal(got[1], 2 * S[1])
+    assert not (tmp_path / "sigma.npz.tmp").exists()
+    with pytest.raises(ValueError):
+        rgf.save_sigma_checkpoint(path, 6, S[0], S[1], S[2][..., :2])
