@@ -110,6 +110,33 @@ def test_gpu_ephoton_matches_oracle(qt, oracle_mod, nm, length, nen, nop):
 
 
 @pytest.mark.gpu
+def test_gpu_ephoton_device_pointers(qt, oracle_mod):
+    """QT_F_DEVICE_OUTPUTS: every array resident on the device (the bench's
+    route), two calls in a row with the second smaller, against the oracle."""
+    import ctypes as C
+    for nm, length, nen, nop in ((20, 3, 6, 2), (9, 2, 3, 1)):
+        print("ephoton device", nm, length, nen, nop, flush=True)
+        rng = np.random.default_rng(nm)
+        M, Gl, Gg = (np.asfortranarray(a) for a in _inputs(rng, nm, length, nen, nop))
+        ins = []
+        for a in (M, Gl, Gg):
+            b = qt.DeviceBuffer(a.nbytes)
+            qt.check(qt.lib().qt_memcpy_h2d(b.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes)))
+            ins.append(b)
+        nbytes = 16 * nm * nm * length * nen
+        outs = [qt.DeviceBuffer(nbytes) for _ in range(3)]
+        qt.check(qt.lib().qt_gw_sigma_ephoton(nm, length, nen, nop, 0.5, 2.0, 0.7,
+                                              *(b.ptr for b in ins), *(b.ptr for b in outs),
+                                              qt.QT_F_DEVICE_OUTPUTS))
+        got = [b.to_host(np.complex128, (nen, length, nm, nm)).transpose(3, 2, 1, 0) for b in outs]
+        ref = oracle_mod.gw_sigma_ephoton(M, Gl, Gg, nop, 0.5, 2.0, 0.7)
+        for g, r in zip(got, ref):
+            assert np.abs(g - r).max() <= 1e-12 * np.abs(r).max()
+        for b in ins + outs:
+            b.free()
+
+
+@pytest.mark.gpu
 def test_gpu_ephoton_rejects_bad_arguments(qt):
     rng = np.random.default_rng(3)
     M, Gl, Gg = _inputs(rng, 2, 1, 2, 1)
