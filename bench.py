@@ -240,7 +240,41 @@ def run_ephoton(args) -> dict:
             "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
                          "frac": round(tf / 78.6, 4), "traffic": None,
                          "kernel": "whole call: eph_sum + 4 x zgemm_mfma_kernel + eph_retarded"},
-            "cpu_baseline": None}
+            "cpu_baseline": None if args.no_cpu else ephoton_cpu_baseline(args, nm, nop)}
+
+
+def ephoton_cpu_baseline(args, nm: int, nop: int) -> dict:
+    """The oracle on host cores: every thread computes one slab of 256 energies
+    20 times (one BLAS thread each; ~10 s on 16 cores), all threads on the same
+    seeded blocks."""
+    import concurrent.futures as cf
+
+    import oracle
+    flavour = "mkl" if oracle.available("mkl") else "plain"
+    if flavour == "mkl":
+        for name in ("MKL_Set_Num_Threads", "mkl_set_num_threads_"):
+            fn = getattr(oracle.lib("mkl"), name, None)
+            if fn is not None:
+                fn(C.c_int(1)) if name[0] == "M" else fn(C.byref(C.c_int(1)))
+                break
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    nen, reps = 256, 20
+    rng = np.random.default_rng(args.seed)
+    cr = lambda *sh: np.asfortranarray(rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+    M, Gl, Gg = cr(nm, nm, 1), cr(nm, nm, 1, nen + 2 * nop), cr(nm, nm, 1, nen + 2 * nop)
+    with cf.ThreadPoolExecutor(threads) as ex:
+        # untimed warm-up: the BLAS library's first call in each thread
+        w = 2 * nop + 2
+        list(ex.map(lambda _: oracle.gw_sigma_ephoton(M, Gl[..., :w], Gg[..., :w], nop, 1.0, 1.0, 0.1,
+                                                      flavour=flavour), range(threads)))
+        t0 = time.perf_counter()
+        list(ex.map(lambda _: [oracle.gw_sigma_ephoton(M, Gl, Gg, nop, 1.0, 1.0, 0.1, flavour=flavour)
+                               for _ in range(reps)], range(threads)))
+        tc = time.perf_counter() - t0
+    return {"value": round(threads * reps * nen / tc, 1), "unit": "blocks/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{threads} threads x {reps} calls of (1 slab x {nen} energies) at nm={nm} "
+                      f"({flavour} ZGEMM, one BLAS thread each), {tc:.1f}s"}
 
 
 def run_vendor_zgemm(args) -> dict:
