@@ -144,6 +144,9 @@ def test_gpu_ephoton_rejects_bad_arguments(qt):
         qt.gw_sigma_ephoton(M, Gl, Gg, 1, 1.0, 1.0, 0.0)  # hw = 0
     with pytest.raises(ValueError):
         qt.gw_sigma_ephoton(M, Gl, Gg, 3, 1.0, 1.0, 1.0)  # axis shorter than 2 nop
+    # no output energies (the axis holds only the 2 nop shifted points): empty results
+    Sr, Sl, Sg = qt.gw_sigma_ephoton(M, Gl[..., :2], Gg[..., :2], 1, 1.0, 1.0, 1.0)
+    assert Sr.shape == Sl.shape == Sg.shape == (2, 2, 1, 0)
 
 
 def test_sigma_checkpoint_round_trip(tmp_path):
