@@ -141,6 +141,56 @@ class Wannier:
         return Hii, H1i
 
 
+def band_edges(w: Wannier, chunk: int = 512) -> dict:
+    """Band-structure scan for the band edges, wannierHam3d.f90:153-201: the bulk
+    H(k) of w90_MAT_DEF_3D (:304-323, the sum of Hr over every cell with
+    e^{-i r.k}) diagonalised (upper triangle, as zheev 'U') on the k grid of
+    w90_PLOT_BZ (:236-270): nk = floor(1/L / 0.01) + 1 points per axis (1 where
+    Hr spans one cell), k_i = i dk - pi/L for i = 1..nk, dk = 2 pi/L/(nk - 1) or
+    pi/L when nk = 1.  VBM = max over k of band nvb, CBM = min of band nvb + 1;
+    kt_vbm / kt_cbm follow :184-191, including their use of nkx with the
+    transverse step 2 pi/Ly/(nky - 1) (or 2 pi/Ly) and 0 when Hr spans one
+    cell in y.  Returns CBM, VBM, Eg, kt_cbm, kt_vbm (in 2 pi/Ly, as printed)
+    and the eigenvalues EN (nb, nkx nky nkz)."""
+    alpha, beta, gamm, xhat, yhat, zhat = w._axes()
+    Lx = abs(float(np.dot(alpha, xhat)))
+    Ly = abs(float(np.dot(beta, yhat)))
+    Lz = float(np.linalg.norm(gamm))
+    nb = w.nb
+    ncx, ncy, ncz = w.Hr.shape[2:]
+    nk = [1 if nc == 1 else int(np.floor((1.0 / L) / 0.01)) + 1
+          for nc, L in ((ncx, Lx), (ncy, Ly), (ncz, Lz))]
+    axes = []
+    for n, L in zip(nk, (Lx, Ly, Lz)):
+        dk = 2 * np.pi / L / (n - 1) if n > 1 else np.pi / L
+        axes.append(np.arange(1, n + 1) * dk - np.pi / L)
+    # column i + j nkx + l nkx nky (0-based) holds (kx_i, ky_j, kz_l)
+    KZ, KY, KX = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    kv = (KX.reshape(-1)[:, None] * xhat + KY.reshape(-1)[:, None] * yhat
+          + KZ.reshape(-1)[:, None] * zhat)
+    cells = [(i, j, l) for i in range(w.xmin, w.xmax + 1) for j in range(w.ymin, w.ymax + 1)
+             for l in range(w.zmin, w.zmax + 1)]
+    R = np.array([i * alpha + j * beta + l * gamm for i, j, l in cells])
+    Hc = np.stack([w.Hr[:, :, i - w.xmin, j - w.ymin, l - w.zmin] for i, j, l in cells])
+    EN = np.empty((nb, kv.shape[0]))
+    for k0 in range(0, kv.shape[0], chunk):
+        ph = np.exp(-1j * (kv[k0:k0 + chunk] @ R.T))          # (k, cells)
+        H = np.einsum("kc,cab->kab", ph, Hc)
+        EN[:, k0:k0 + chunk] = np.linalg.eigvalsh(H, UPLO="U").T
+    nvb = w.nvb
+    iv = int(np.argmax(EN[nvb - 1])) + 1   # 1-based, first maximum (maxloc)
+    ic = int(np.argmin(EN[nvb])) + 1
+    dky = 2 * np.pi / Ly / (nk[1] - 1) if nk[1] > 1 else 2 * np.pi / Ly
+    if ncy == 1:
+        kt_vbm = kt_cbm = 0.0
+    else:
+        kt_vbm = (iv % nk[0]) * dky - np.pi / Ly
+        kt_cbm = (ic % nk[0]) * dky - np.pi / Ly
+    VBM, CBM = float(EN[nvb - 1, iv - 1]), float(EN[nvb, ic - 1])
+    return {"CBM": CBM, "VBM": VBM, "Eg": CBM - VBM, "kt_cbm": kt_cbm / (2 * np.pi / Ly),
+            "kt_vbm": kt_vbm / (2 * np.pi / Ly), "nk": tuple(nk), "EN": EN}
+
+
 def read_ham_dat(path: str) -> Wannier:
     """Free-format reader with the record order of wannierHam3d.f90:89-111,
     147-152 and 204-208."""

@@ -122,3 +122,41 @@ def test_coo_route_real_one_based_and_errors(tmp_path):
     hm.write_coo(str(path), H, use0index=False)
     with pytest.raises(ValueError):
         hm.device_from_coo(str(path), bs, use0index=False)
+
+
+def test_band_edges_two_band_chain_closed_form():
+    """band_edges (wannierHam3d.f90:153-201) on a two-band chain along x with
+    complex hoppings: E_b(k) = h_b + 2 Re(t_b e^{-i k Lx}), evaluated on the
+    reference's grid k_i = i dk - pi/Lx, i = 1..nkx, nkx = floor(1/Lx/0.01) + 1.
+    The sine term fixes the sign of the phase; one cell in y and z gives one
+    k point there and zero transverse edge positions."""
+    Lx = 4.0
+    cell = np.diag([Lx, 3.0, 2.0])
+    h, t = np.array([-1.0, 1.0]), np.array([0.2 + 0.3j, -0.25 + 0.1j])
+    Hr = np.zeros((2, 2, 3, 1, 1), complex)
+    Hr[:, :, 1, 0, 0] = np.diag(h)
+    Hr[:, :, 2, 0, 0] = np.diag(t)
+    Hr[:, :, 0, 0, 0] = np.diag(t.conj())
+    w = hm.Wannier(nvb=1, spin_deg=2, cell=cell, Hr=Hr, xmin=-1, ymin=0, zmin=0,
+                   centres=np.zeros((3, 2)))
+    r = hm.band_edges(w)
+    nkx = int(np.floor(1 / Lx / 0.01)) + 1
+    assert r["nk"] == (nkx, 1, 1)
+    k = np.arange(1, nkx + 1) * (2 * np.pi / Lx / (nkx - 1)) - np.pi / Lx
+    E = h[:, None] + 2 * np.real(t[:, None] * np.exp(-1j * k[None, :] * Lx))
+    np.testing.assert_allclose(r["EN"], E, atol=1e-13)
+    assert r["VBM"] == pytest.approx(E[0].max(), abs=1e-13)
+    assert r["CBM"] == pytest.approx(E[1].min(), abs=1e-13)
+    assert r["Eg"] == pytest.approx(E[1].min() - E[0].max(), abs=1e-13)
+    assert r["kt_cbm"] == 0.0 and r["kt_vbm"] == 0.0
+
+
+def test_band_edges_golden_cnt_gap_is_positive():
+    """The reference's CNT input (test_cnt/ham_dat): a semiconducting tube, so the
+    scan finds CBM above VBM on the 24-point axial grid."""
+    import os
+    w = hm.read_ham_dat(os.path.join(os.path.dirname(__file__), "golden", "ham_dat"))
+    r = hm.band_edges(w)
+    assert r["nk"] == (24, 1, 1)
+    assert 0.1 < r["Eg"] < 2.0
+    assert np.all(np.diff(r["EN"], axis=0) >= -1e-12)
