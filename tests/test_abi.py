@@ -95,3 +95,39 @@ def test_fortran_modules_export_reference_names():
                      ("selfenergy_mod", "sancho"), ("output", "write_spectrum_summed_over_k"),
                      ("devicehammod", "devh_build_fromwannierfile")):
         assert sub in out, sub
+
+
+def _header_struct_fields(name):
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct \{([^{}]*)\}\s*" + name + r"\s*;", text).group(1)
+    fields = []
+    for decl in body.split(";"):
+        parts = [p.strip() for p in decl.strip().split(",")] if decl.strip() else []
+        fields += [re.split(r"[\s*]+", p)[-1] for p in parts]
+    return fields
+
+
+def _fortran_type_fields(text, name):
+    body = re.search(r"type, bind\(C\) :: " + name + r"(.*?)end type", text, flags=re.S | re.I).group(1)
+    fields = []
+    for line in body.splitlines():
+        line = line.split("!")[0]
+        if "::" not in line:
+            continue
+        for item in line.split("::", 1)[1].split(","):
+            fields.append(item.split("=")[0].strip())
+    return fields
+
+
+@pytest.mark.parametrize("where", ["INTEGRATION.md", "quantum_transport_amd/fortran/qt_capi.f90"])
+def test_documented_fortran_binding_matches_header(where):
+    """The Fortran mirror of qt_rgf_outputs a maintainer copies has every
+    pointer of the C struct, in the C order (a shorter type would let
+    qt_rgf_solve read past the caller's struct)."""
+    text = open(os.path.join(ROOT, where)).read()
+    c_fields = [f.lower() for f in _header_struct_fields("qt_rgf_outputs")]
+    f_fields = [f.lower() for f in _fortran_type_fields(text, "qt_rgf_outputs")]
+    assert f_fields == c_fields, (where, f_fields, c_fields)
+    c_par = [f.lower() for f in _header_struct_fields("qt_rgf_params")]
+    f_par = [f.lower() for f in _fortran_type_fields(text, "qt_rgf_params")]
+    assert f_par == c_par, (where, f_par, c_par)
