@@ -38,7 +38,9 @@ enum {
     QT_ERR_SINGULAR = 5,     /* a block inversion met an exactly zero pivot;
                                 that block was zeroed as linalg.f90:50-58 does */
     QT_ERR_UNSUPPORTED = 6,  /* shape / option combination not provided */
-    QT_ERR_NO_DEVICE = 7     /* no HIP device visible */
+    QT_ERR_NO_DEVICE = 7,    /* no HIP device visible */
+    QT_ERR_TOO_LARGE = 8     /* request exceeds the fixed scratch of a
+                                no-allocation entry point */
 };
 
 /* per-point info bits (qt_rgf_outputs.info) */
@@ -185,6 +187,28 @@ int qt_zgemm_batched_prec(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_
  * meets an exactly zero pivot is zeroed and info[b] = that column (1-based),
  * else info[b] = 0.  Returns QT_ERR_SINGULAR when any block failed. */
 int qt_invert_batched(int32_t n, int32_t batch, void* A, int32_t* info);
+
+/* Capacity, in int32, of the fixed per-device scratch array that
+ * qt_invert_batched_noalloc uses (4 MiB). */
+#define QT_NOALLOC_SCRATCH_INTS (1 << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the inversion of qt_invert_batched on device-resident
+ * blocks, with no allocation of any kind.
+ *   A:    device, batch blocks of n x n complex128 (stride n*n), destroyed;
+ *   X:    device output, inv(A[b]) in the same layout (must not overlap A);
+ *   info: device int32[batch] (0, or the 1-based column of the first zero
+ *         pivot, that block of X zeroed) or NULL.
+ * The pivot lists (n per block) and the info flags live in one fixed array of
+ * QT_NOALLOC_SCRATCH_INTS int32 per device, owned by the library for the life
+ * of the process: n*batch + batch + 256 must not exceed it, else
+ * QT_ERR_TOO_LARGE and nothing is launched.  Returns after the inversions
+ * finished (QT_ERR_SINGULAR when any block was singular).  Every call on a
+ * device shares that device's array: the library serialises its own calls,
+ * so two calls overlapping on one device (e.g. from a second copy of the
+ * library in the process) would race on it. */
+int qt_invert_batched_noalloc(int32_t n, int32_t batch, void* A, void* X, int32_t* info);
 
 /* ---- energy-axis correlation / convolution ---------------------------- */
 

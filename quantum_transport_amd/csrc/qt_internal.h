@@ -78,6 +78,9 @@ struct InvArgs {
     int flop_batch = -1;        // members counted by the timer (-1: batch)
 };
 void zinv(hipStream_t s, const InvArgs& a, Timer* t);
+// The same inversion with every member on stream s: no side streams, no
+// events, no timer (nothing is created or allocated on the way).
+void zinv_single(hipStream_t s, const InvArgs& a);
 
 // ---- elementwise kernels (qt_elem.hip) ----
 

@@ -602,6 +602,12 @@ int inv_parts() {
 // bound column sweep, leaving most CUs idle; the batch is therefore split in
 // parts on as many streams so that the panels of one part run while the
 // MFMA updates of the others fill the remaining CUs.
+void zinv_single(hipStream_t s, const InvArgs& a) {
+    if (a.n <= 0 || a.batch <= 0) return;
+    if (a.n > 1024) throw hip_error("zinv: block size above 1024", 6);
+    zinv_part(s, a);
+}
+
 void zinv(hipStream_t s, const InvArgs& a, Timer* t) {
     const int n = a.n;
     if (n <= 0 || a.batch <= 0) return;

@@ -70,9 +70,16 @@ struct Arena {
     }
 };
 
+// Fixed scratch of qt_invert_batched_noalloc: pivot lists, info flags and a
+// status word, QT_NOALLOC_SCRATCH_INTS int32 per device, held until the
+// process ends.  Host code uses the device address that qt_init takes with
+// hipGetSymbolAddress (Ctx::scratch), never this name.
+static __device__ __align__(256) int scratch[QT_NOALLOC_SCRATCH_INTS];
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    int* scratch = nullptr;     // device address of `scratch` on this device
     Timer timer;
     Arena arena;
     qt_timing last{};
@@ -957,6 +964,8 @@ const char* qt_status_string(int st) {
         case QT_ERR_SINGULAR: return "singular block inversion (block zeroed)";
         case QT_ERR_UNSUPPORTED: return "unsupported shape or option";
         case QT_ERR_NO_DEVICE: return "no HIP device";
+        case QT_ERR_TOO_LARGE:
+            return "n*batch + batch + 256 exceeds the QT_NOALLOC_SCRATCH_INTS scratch of the no-allocation entry point";
         default: return "unknown status";
     }
 }
@@ -975,6 +984,8 @@ int qt_init(int device) {
         Ctx* c = new Ctx();
         c->device = device;
         QT_HIP(hipStreamCreate(&c->stream));
+        QT_HIP(hipGetSymbolAddress((void**)&c->scratch, HIP_SYMBOL(scratch)));
+        if (!c->scratch) throw hip_error("hipGetSymbolAddress(scratch) returned null", QT_ERR_HIP);
         QT_HIP(hipHostMalloc((void**)&c->host_count, sizeof(int), hipHostMallocDefault));
         for (int i = 0; i < 4; ++i) QT_HIP(hipEventCreate(&c->ev[i]));
         g_ctx = c;
@@ -1540,6 +1551,51 @@ int qt_invert_batched(int32_t n, int32_t batch, void* A, int32_t* info) {
             if (hinfo[b]) st = QT_ERR_SINGULAR;
         }
         return st;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+// flag[0] |= (info[b] != 0) over the batch
+__global__ void info_any_kernel(const int* info, int batch, int* flag) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b < batch && info[b] != 0) atomicOr(flag, 1);
+}
+
+int qt_invert_batched_noalloc(int32_t n, int32_t batch, void* A, void* X, int32_t* info) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (n <= 0 || n > 1024 || batch <= 0 || !A || !X) return QT_ERR_INVALID;
+    // scratch layout (int32): [0, 256) status word and padding,
+    // [256, 256 + batch) info (when the caller passes none), then n*batch pivots
+    if ((int64_t)n * batch + batch + 256 > (int64_t)QT_NOALLOC_SCRATCH_INTS) return QT_ERR_TOO_LARGE;
+    const int64_t nn = (int64_t)n * n;
+    const char *a0 = (const char*)A, *x0 = (const char*)X;
+    const int64_t bytes = (int64_t)sizeof(double2) * nn * batch;
+    if (a0 < x0 + bytes && x0 < a0 + bytes) return QT_ERR_INVALID;  // A and X overlap
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        int* flag = c.scratch;
+        int* sinfo = info ? info : c.scratch + 256;
+        int* piv = c.scratch + 256 + batch;  // ends at n*batch + batch + 256 <= capacity
+        fill_int(c.stream, flag, 1, 0);  // the array keeps the last call's contents
+        InvArgs a;
+        a.n = n;
+        a.batch = batch;
+        a.A = (double2*)A;
+        a.sa = nn;
+        a.X = (double2*)X;
+        a.sx = nn;
+        a.piv = piv;
+        a.info = sinfo;  // cleared by the inversion itself
+        zinv_single(c.stream, a);
+        hipLaunchKernelGGL(info_any_kernel, dim3((batch + 255) / 256), dim3(256), 0, c.stream, sinfo, batch,
+                           flag);
+        QT_HIP(hipGetLastError());
+        QT_HIP(hipMemcpyAsync(c.host_count, flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        QT_HIP(hipStreamSynchronize(c.stream));
+        return *c.host_count ? QT_ERR_SINGULAR : QT_OK;
     } catch (const hip_error& e) {
         return status_of(e);
     }

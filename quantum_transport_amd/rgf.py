@@ -33,13 +33,15 @@ QT_F_LEAD_RESET = 0x4
 QT_INFO_SINGULAR = 0x1
 QT_INFO_LEAD_NMAX = 0x2
 QT_ERR_SINGULAR = 5
+QT_ERR_TOO_LARGE = 8
+QT_NOALLOC_SCRATCH_INTS = 1 << 20
 
 EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham_destroy",
             "qt_rgf_solve", "qt_sancho", "qt_device_alloc", "qt_device_free", "qt_memcpy_h2d",
             "qt_memcpy_d2h", "qt_synchronize", "qt_set_kernel_timing", "qt_last_timing",
             "qt_zgemm_batched", "qt_zgemm_batched_prec", "qt_invert_batched", "qt_corr1d2_batched",
             "qt_conv1d2_batched", "qt_gw_polarization", "qt_gw_selfenergy", "qt_gw_mix",
-            "qt_gw_wsolve_batched", "qt_gw_sigma_ephoton"]
+            "qt_gw_wsolve_batched", "qt_gw_sigma_ephoton", "qt_invert_batched_noalloc"]
 
 
 class QTError(RuntimeError):
@@ -104,6 +106,8 @@ def lib() -> C.CDLL:
         L.qt_gw_selfenergy.argtypes = [C.c_int32] * 6 + [C.c_double] + [C.c_void_p] * 9 \
             + [C.c_int32]
         L.qt_gw_mix.argtypes = [C.c_int64, C.c_double, C.c_void_p, C.c_void_p, C.c_int32]
+        L.qt_invert_batched_noalloc.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                                C.c_void_p]
         L.qt_gw_wsolve_batched.argtypes = [C.c_int32] * 3 + [C.c_void_p] * 8 + [C.c_int32]
         L.qt_gw_sigma_ephoton.argtypes = [C.c_int32] * 4 + [C.c_double] * 3 + [C.c_void_p] * 6 \
             + [C.c_int32]
@@ -397,6 +401,19 @@ def invert_batched(A: np.ndarray):
     if st not in (0, QT_ERR_SINGULAR):
         check(st)
     return np.transpose(Af, (0, 2, 1)), info
+
+
+def invert_batched_noalloc(n: int, batch: int, A: "DeviceBuffer", X: "DeviceBuffer",
+                           info: Optional["DeviceBuffer"] = None) -> int:
+    """qt_invert_batched_noalloc on device buffers: X[b] = inv(A[b]) for
+    batch column-major n x n blocks, A destroyed, no device allocation.
+    Returns the status (QT_OK or QT_ERR_SINGULAR); raises on other errors."""
+    init()
+    st = lib().qt_invert_batched_noalloc(C.c_int32(n), C.c_int32(batch), A.ptr, X.ptr,
+                                         info.ptr if info is not None else None)
+    if st not in (0, QT_ERR_SINGULAR):
+        check(st)
+    return st
 
 
 def _corr_conv(name: str, X, Y) -> np.ndarray:
