@@ -63,6 +63,11 @@ def test_ferm(oracle_mod):
     assert oracle_mod.ferm(0.0) == 0.5
     x = 3.7
     assert abs(oracle_mod.ferm(x) + oracle_mod.ferm(-x) - 1.0) < 1e-15
+    # f(x) = 1 / (1 + e^x) (rgf_mod.f90:475-479) at nonzero arguments: the
+    # occupation falls with energy, so 1 / (1 + e^-x) would fail here
+    assert abs(oracle_mod.ferm(1.0) - 1.0 / (1.0 + np.e)) < 1e-16
+    assert abs(oracle_mod.ferm(-2.5) - 0.9241418199787566) < 1e-15
+    assert oracle_mod.ferm(40.0) < 1e-17
 
 
 # --------------------------------------------------------------- Sancho-Rubio
@@ -258,6 +263,49 @@ def test_rgf_backward_matches_forward_for_hermitian_coupling(oracle_mod):
     for key in ("G_r", "G_lesser", "G_greater"):
         for s in range(4):
             np.testing.assert_allclose(a[key][s], b[key][s], rtol=0, atol=1e-9 * np.abs(a[key][s]).max())
+
+
+@pytest.mark.parametrize("E", [-0.25, -0.05, 0.08])
+def test_rgf_backward_dense_unequal_contacts(oracle_mod, E):
+    """Backward variant against the dense device inverse with different
+    chemical potentials and temperatures at the two contacts.  The backward
+    sweep reads H1i(s) as the coupling H_{s-1,s} (rgf_mod.f90:340, 365), takes
+    the right lead's self-energy as H1i(nx+1) G00 H1i(nx+1)^H (:316) filled
+    with f(E - mur, TEMPr) (:321) and the left one as H1i(1)^H G00 H1i(1)
+    (:364) filled with f(E - mul, TEMPl) (:395); tr is taken at the right
+    contact, tre at the left (:404-415, :459-469).  Swapping the two
+    occupations or the two temperatures changes G^< by far more than the
+    bar."""
+    n, nx = 7, 5
+    _, dev = hm.synthetic_device(n, nx, seed=21, coupling_density=0.4)
+    Hii = dev.Hii[0]
+    rng = np.random.default_rng(22)
+    # a non-Hermitian coupling (distinct H_{s-1,s} and H_{s,s-1}^H readings)
+    H1i = [h + 0.2 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) for h in dev.H1i[0]]
+    mul, mur, TL, TR = 0.05, -0.2, 150.0, 600.0
+    out = oracle_mod.rgf(Hii, H1i, E, mul, mur, TL, TR, variant="backward")
+    G_L, _, _, _ = oracle_mod.sancho(None, Hii[0], H1i[0].conj().T, E)
+    G_R, _, _, _ = oracle_mod.sancho(None, Hii[-1], H1i[nx], E)
+    sigL = H1i[0].conj().T @ G_L @ H1i[0]
+    sigR = H1i[nx] @ G_R @ H1i[nx].conj().T
+    fL = _dense.ferm((E - mul) / (BOLTZ * TL))
+    fR = _dense.ferm((E - mur) / (BOLTZ * TR))
+    lower = [h.conj().T for h in H1i]  # H_{s,s-1} = (H_{s-1,s})^H
+    d = _dense.dense_negf(Hii, lower, E, sigL, sigR, fL, fR)
+    swapped = _dense.dense_negf(Hii, lower, E, sigL, sigR, fR, fL)
+    for key in ("G_r", "G_lesser", "G_greater"):
+        for s in range(nx):
+            ref = d[key][s]
+            scale = max(np.abs(ref).max(), np.abs(d["G_r"][s]).max())
+            assert np.abs(out[key][s] - ref).max() <= 1e-10 * scale, (key, s)
+    gap = max(np.abs(swapped["G_lesser"][s] - d["G_lesser"][s]).max() for s in range(nx))
+    assert gap > 1e-4 * max(np.abs(d["G_r"][s]).max() for s in range(nx))
+    # currents: tr at the right contact, tre at the left, opposite; their
+    # size is the Caroli transmission times the Fermi window
+    Tw = d["T"] * (fL - fR)
+    assert abs(Tw) > 1e-4
+    assert abs(out["tr"] + out["tre"]) <= 1e-9 * abs(Tw)
+    assert abs(out["tre"] - Tw) <= 1e-9 * abs(Tw)
 
 
 def test_plain_and_vendor_lapack_agree(oracle_mod):
@@ -502,6 +550,109 @@ def test_gw_selfenergy_single_k(oracle_mod):
                 ref = (raw(ij, ix) + raw(ji, ix)) / 2
                 for got, r in zip((Sr, Sl, Sg), ref):
                     np.testing.assert_allclose(got[ij, ix, 0], r, atol=1e-13)
+
+
+PI_REF = 3.14159265359  # the reference's pi (static.f90:15)
+
+
+def test_gw_polarization_delta_closed_form(oracle_mod):
+    """Single-sample series give the closed form of gw_mod.f90:495-503 with
+    corr1d2(X, Y)(l) = sum_e X(e) Y(e - l), lag l at index n/2 + l
+    (fft_mod.f90:57-91): with G^<_ij = i a d(e1), G^>_ji = i b d(e2),
+    P^<_ij has the single entry dE (i a)(i b) at n/2 + e1 - e2,
+    dE = -i dEn/(2 pi) spindeg/nk (gw_mod.f90:516).  G enters through i Im G
+    (:444-446), so the real parts planted here must not matter.  A pairing of
+    ij with ij instead of ji, a flipped prefactor or a mirrored lag fails."""
+    nm, nx, nen = 2, 1, 16
+    h = nen // 2
+    ij, ji = 0 + 1 * nm, 1 + 0 * nm  # (row 0, col 1) and its transpose
+    Gr = np.zeros((nm * nm, nx, 1, nen), complex)
+    Gl = np.zeros_like(Gr)
+    Gg = np.zeros_like(Gr)
+    a, b, c, d_, g = 0.7, -1.3, 0.4, 2.1, -0.6
+    e1, e2, e3, e4, e5 = 5, 3, 9, 6, 2
+    Gl[ij, 0, 0, e1] = 3.0 + 1j * a
+    Gg[ji, 0, 0, e2] = -5.0 + 1j * b
+    Gr[ij, 0, 0, e3] = 1.5 + 1j * c
+    Gl[ji, 0, 0, e4] = 0.25 + 1j * d_
+    Gg[ij, 0, 0, e5] = 7.0 + 1j * g
+    dEn, spin = 0.03, 2.0
+    Pr, Pl, Pg = oracle_mod.gw_polarization(Gr, Gl, Gg, 1, 1, 1, dEn, spin)
+    dE = -1j * dEn / (2 * PI_REF) * spin
+
+    def only(arr, entries):
+        ref = np.zeros(nen, complex)
+        for idx, v in entries:
+            ref[idx] += v
+        np.testing.assert_allclose(arr, ref, rtol=1e-14, atol=1e-16)
+
+    only(Pl[ij, 0, 0], [(h + e1 - e2, dE * (1j * a) * (1j * b))])
+    only(Pg[ij, 0, 0], [(h + e5 - e4, dE * (1j * g) * (1j * d_))])
+    # P^R = corr(G^<_ij, conj G^R_ij) + corr(G^R_ij, G^<_ji)
+    only(Pr[ij, 0, 0], [(h + e1 - e3, dE * (1j * a) * (-1j * c)), (h + e3 - e4, dE * (1j * c) * (1j * d_))])
+    # element ji pairs G_ji with G_ij
+    only(Pl[ji, 0, 0], [(h + e4 - e5, dE * (1j * d_) * (1j * g))])
+    # diagonal elements saw no input
+    assert not Pl[0].any() and not Pl[3].any() and not Pr[0].any()
+
+
+def test_gw_polarization_relabelling(oracle_mod):
+    """P^>_ij(w) = P^<_ji(-w) for any input (nk = 1): both are
+    sum_e G^>_ij(e) G^<_ji(e - w) by the definitions of gw_mod.f90:495-499,
+    i.e. index n/2 + l of P^>_ij equals index n/2 - l of P^<_ji for every lag
+    both hold (all but index 0)."""
+    nm, nx, nen = 3, 2, 12
+    rng = np.random.default_rng(31)
+    Gr, Gl, Gg = _gw_random(rng, nm, nx, 1, nen, 3)
+    Pr, Pl, Pg = oracle_mod.gw_polarization(Gr, Gl, Gg, 1, 1, nm, 0.02, 1.0)
+    for row in range(nm):
+        for col in range(nm):
+            ij, ji = row + col * nm, col + row * nm
+            for ix in range(nx):
+                a = Pg[ij, ix, 0, 1:]
+                b = Pl[ji, ix, 0, ::-1][:-1]
+                np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-14 * np.abs(a).max())
+
+
+def test_gw_selfenergy_delta_closed_form(oracle_mod):
+    """Single-sample series against the closed form of gw_mod.f90:672-698:
+    conv1d2(X, Y)(e) = sum_l X(e - l) Y(l + n/2) (fft_mod.f90:128-168) puts
+    X = i a d(e1) against Y = w d(m) at index e1 + m - n/2; G enters as i Im G
+    and pairs with W of the same element ij; dE = i dEn/(2 pi)/nk;
+    Sigma^R = Re[dE(conv(G<,W^R) + conv(G^R,W<) + conv(G^R,W^R))]
+    + i Im(Sigma^> - Sigma^<)/2; then (S_ij + S_ji)/2 (:730-743) halves
+    everything and copies it to ji.  A W paired with the wrong G, a dropped
+    term of Sigma^R or a conjugate symmetrisation fails."""
+    nm, nx, nen = 2, 1, 16
+    h = nen // 2
+    ij, ji = 0 + 1 * nm, 1 + 0 * nm
+    Z = lambda: np.zeros((nm * nm, nx, 1, nen), complex)
+    Gr, Gl, Gg, Wr, Wl, Wg = Z(), Z(), Z(), Z(), Z(), Z()
+    a, b, c = 0.8, -0.45, 1.7
+    w1, w2, w3 = 0.3 - 1.1j, -0.7 + 0.2j, 1.9 + 0.6j
+    e1, m1, e2, m2, e3, m3 = 3, 9, 6, 10, 10, 12
+    Gl[ij, 0, 0, e1] = -2.0 + 1j * a
+    Gg[ij, 0, 0, e2] = 4.0 + 1j * b
+    Gr[ij, 0, 0, e3] = 0.5 + 1j * c
+    Wl[ij, 0, 0, m1] = w1
+    Wg[ij, 0, 0, m2] = w2
+    Wr[ij, 0, 0, m3] = w3
+    dEn = 0.04
+    Sr, Sl, Sg = oracle_mod.gw_selfenergy(Gr, Gl, Gg, Wr, Wl, Wg, 1, 1, 1, dEn)
+    dE = 1j * dEn / (2 * PI_REF)
+    sl = np.zeros(nen, complex)
+    sg = np.zeros(nen, complex)
+    sr = np.zeros(nen, complex)
+    sl[e1 + m1 - h] += dE * (1j * a) * w1
+    sg[e2 + m2 - h] += dE * (1j * b) * w2
+    sr[e1 + m3 - h] += dE * (1j * a) * w3
+    sr[e3 + m1 - h] += dE * (1j * c) * w1
+    sr[e3 + m3 - h] += dE * (1j * c) * w3
+    sr = sr.real + 1j * (sg - sl).imag / 2
+    for got, ref in ((Sl, sl), (Sg, sg), (Sr, sr)):
+        np.testing.assert_allclose(got[ij, 0, 0], ref / 2, rtol=1e-14, atol=1e-16)
+        np.testing.assert_allclose(got[ji, 0, 0], ref / 2, rtol=1e-14, atol=1e-16)
+        assert not got[0].any() and not got[3].any()
 
 
 def test_gw_mix(oracle_mod):
