@@ -188,9 +188,9 @@ int qt_zgemm_batched_prec(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_
  * else info[b] = 0.  Returns QT_ERR_SINGULAR when any block failed. */
 int qt_invert_batched(int32_t n, int32_t batch, void* A, int32_t* info);
 
-/* Capacity, in int32, of the fixed per-device scratch array that
- * qt_invert_batched_noalloc uses (4 MiB). */
-#define QT_NOALLOC_SCRATCH_INTS (1 << 20)
+/* Size of the fixed per-device scratch array of qt_invert_batched_noalloc
+ * (64 MiB). */
+#define QT_NOALLOC_SCRATCH_BYTES (64u << 20)
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
@@ -200,14 +200,17 @@ int qt_invert_batched(int32_t n, int32_t batch, void* A, int32_t* info);
  *   X:    device output, inv(A[b]) in the same layout (must not overlap A);
  *   info: device int32[batch] (0, or the 1-based column of the first zero
  *         pivot, that block of X zeroed) or NULL.
- * The pivot lists (n per block) and the info flags live in one fixed array of
- * QT_NOALLOC_SCRATCH_INTS int32 per device, owned by the library for the life
- * of the process: n*batch + batch + 256 must not exceed it, else
+ * The pivot lists, the info flags and the panel multipliers live in one
+ * fixed array of QT_NOALLOC_SCRATCH_BYTES per device, owned by the library
+ * for the life of the process; a call needs
+ *   1280 + 4 batch + 4 n batch + 512 n batch  bytes
+ * of it (e.g. up to 254 blocks of 512 or 2000 blocks of 64), else
  * QT_ERR_TOO_LARGE and nothing is launched.  Returns after the inversions
  * finished (QT_ERR_SINGULAR when any block was singular).  Every call on a
- * device shares that device's array: the library serialises its own calls,
- * so two calls overlapping on one device (e.g. from a second copy of the
- * library in the process) would race on it. */
+ * device shares that device's array, which keeps the last call's contents:
+ * the library serialises its own calls, so two calls overlapping on one
+ * device (e.g. from a second copy of the library in the process) would race
+ * on it. */
 int qt_invert_batched_noalloc(int32_t n, int32_t batch, void* A, void* X, int32_t* info);
 
 /* ---- energy-axis correlation / convolution ---------------------------- */

@@ -66,6 +66,7 @@ void zgemm_i8(hipStream_t s, int opa, int opb, const ZGemmArgs& a);
 // Batched in-place-free inversion: Ainv[b] = inv(A[b]); A is destroyed.
 // piv: n ints per member, info: one int per member (0 or first zero pivot
 // column + 1).
+constexpr int kInvPanelMax = 32;  // widest inversion panel (columns)
 struct InvArgs {
     int n = 0, batch = 0;
     double2* A = nullptr;       // work matrices, lda = n, stride sa
@@ -74,6 +75,8 @@ struct InvArgs {
     int64_t sx = 0;
     int* piv = nullptr;         // n per member
     int* info = nullptr;        // 1 per member
+    double2* G = nullptr;       // multipliers of the panel steps, stride sg per member
+    int64_t sg = 0;             // >= n * kInvPanelMax
     const int* active = nullptr;
     int flop_batch = -1;        // members counted by the timer (-1: batch)
 };

@@ -14,20 +14,25 @@
 //      panel in registers): unblocked Gauss-Jordan on the n x NB panel; the
 //      pivot of column j is the first-indexed row of maximal |re|+|im|
 //      (the IZAMAX choice of ZGETRF) among the rows not yet pivoted; the
-//      pivot row is broadcast through LDS, two barriers per column.
+//      pivot row is broadcast through LDS, two barriers per column.  Every
+//      row's multiplier g_rc = a_rc / pivot_c of step c is also recorded
+//      (G, n x NB per member), with 1/pivot_c at the pivot row's entry.
 //   2. update kernel (one workgroup per 32-column tile outside the panel
-//      and member): copies the NB pivot rows of its columns to LDS (R), then
-//      for every 16-row block adds P R on the FP64 matrix cores, with P the
-//      new panel columns and the pivot rows' old values dropped:
-//        A_KO <- inv(A_KK) A_KO,  A_OO <- A_OO - A_OK inv(A_KK) A_KO.
-// The first versions of both kernels (fully unrolled panel, 64-column
-// update tiles) are kept behind QT_INV_PANEL=unrolled / QT_INV_UPDATE=tiles64.
+//      and member) replays the panel's NB elementary row operations on its
+//      columns in the order unblocked Gauss-Jordan applies them:
+//        y   = forward substitution of the pivot rows' values with the
+//              multipliers pivot rows received before their own step,
+//        row r (not a pivot of the panel): a_r - sum_c g_rc y_c,
+//        pivot row of step c: (y_c - sum_{c' > c} g_rc' y_c') / pivot_c,
+//      the sums on the FP64 matrix cores.  The pivot rows keep the y_c the
+//      other rows were eliminated with, which is what gives Gauss-Jordan its
+//      small residual |A X - I|; multiplying by the panel's explicit
+//      inv(A_KK) instead loses up to cond(A_KK) of it.
 // A zero pivot sets info (first column, 1-based) and the output block is
 // zeroed (linalg.f90:50-58).
 #include <algorithm>
 #include <climits>
 #include <cstdlib>
-#include <string>
 
 #include "qt_internal.h"
 
@@ -49,99 +54,6 @@ __device__ __forceinline__ double2 crecip(double2 a) {
     } else {
         const double r = a.x / a.y, d = a.y + a.x * r;
         return make_double2(r / d, -1.0 / d);
-    }
-}
-
-// ------------------------------------------------------------ panel
-template <int NB, int NT>
-__global__ __launch_bounds__(NT) void gj_panel_kernel(double2* A, int64_t sa, int n, int kb,
-                                                      int* pivp, int* info, const int* active) {
-    constexpr int NW = NT / 64;
-    const int b = blockIdx.x;
-    if (active && !active[b]) return;
-    __shared__ unsigned used[32];          // rows already pivoted (n <= 1024)
-    __shared__ double red_v[2][NW];
-    __shared__ int red_i[2][NW];
-    __shared__ double2 prow[2][NB];
-    double2* Ab = A + b * sa;
-    int* pv = pivp + (int64_t)b * n;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ncols = min(NB, n - kb);
-    const int r = tid;                     // this thread's physical row (n <= NT)
-
-    if (tid < 32) used[tid] = 0u;
-    __syncthreads();
-    for (int j = tid; j < kb; j += NT) atomicOr(&used[pv[j] >> 5], 1u << (pv[j] & 31));
-    __syncthreads();
-    bool mine = r < n && !((used[r >> 5] >> (r & 31)) & 1u);
-
-    double2 v[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c)
-        v[c] = (r < n && c < ncols) ? Ab[r + (int64_t)(kb + c) * n] : make_double2(0.0, 0.0);
-
-#pragma clang loop unroll(full)
-    for (int c = 0; c < NB; ++c) {
-        if (c >= ncols) continue;  // uniform over the workgroup
-        const int buf = c & 1;
-        // pivot search among rows not yet pivoted
-        double best = -1.0;
-        int bi = INT_MAX;
-        if (mine) {
-            best = cabs1(v[c]);
-            bi = r;
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
-        if (lane == 0) { red_v[buf][wave] = best; red_i[buf][wave] = bi; }
-        __syncthreads();
-        double bv = red_v[buf][0];
-        int p = red_i[buf][0];
-#pragma unroll
-        for (int w = 1; w < NW; ++w)
-            if (red_v[buf][w] > bv || (red_v[buf][w] == bv && red_i[buf][w] < p)) {
-                bv = red_v[buf][w];
-                p = red_i[buf][w];
-            }
-        const int j = kb + c;
-        if (p < 0 || p >= n) p = 0;  // unreachable: n - j rows are always unpivoted
-        if (r == p) {
-#pragma unroll
-            for (int cc = 0; cc < NB; ++cc) prow[buf][cc] = v[cc];
-            mine = false;
-        }
-        if (tid == 0) {
-            pv[j] = p;
-            if (!(bv > 0.0) && info[b] == 0) info[b] = j + 1;
-        }
-        __syncthreads();
-        const double2 pivv = prow[buf][c];
-        const double2 d = (pivv.x == 0.0 && pivv.y == 0.0) ? make_double2(0.0, 0.0) : crecip(pivv);
-        if (r == p) {
-#pragma unroll
-            for (int cc = 0; cc < NB; ++cc) v[cc] = (cc == c) ? d : cmul(v[cc], d);
-        } else if (r < n) {
-            const double2 g = cmul(v[c], d);  // f / pivot
-#pragma unroll
-            for (int cc = 0; cc < NB; ++cc) {
-                if (cc == c) {
-                    v[cc] = make_double2(-g.x, -g.y);
-                } else {
-                    const double2 u = prow[buf][cc];
-                    v[cc].x -= g.x * u.x - g.y * u.y;
-                    v[cc].y -= g.x * u.y + g.y * u.x;
-                }
-            }
-        }
-    }
-    if (r < n) {
-#pragma unroll
-        for (int c = 0; c < NB; ++c)
-            if (c < ncols) Ab[r + (int64_t)(kb + c) * n] = v[c];
     }
 }
 
@@ -181,8 +93,9 @@ __device__ __forceinline__ double wave_max(double x) {
 // thread owning the pivot does no extra work per column; the panel is FP64
 // vector-bound and every wave waits at the barrier for the slowest one.
 template <int NB, int NT>
-__global__ __launch_bounds__(NT) void gj_panel_loop_kernel(double2* A, int64_t sa, int n, int kb,
-                                                           int* pivp, int* info, const int* active) {
+__global__ __launch_bounds__(NT) void gj_panel_loop_kernel(double2* A, int64_t sa, double2* Gm, int64_t sg,
+                                                           int n, int kb, int* pivp, int* info,
+                                                           const int* active) {
     constexpr int NW = NT / 64;
     const int b = blockIdx.x;
     if (active && !active[b]) return;
@@ -192,6 +105,7 @@ __global__ __launch_bounds__(NT) void gj_panel_loop_kernel(double2* A, int64_t s
     __shared__ double2 prow[2][NB];
     __shared__ int pcol[NB];               // pivot rows chosen in this panel
     double2* Ab = A + b * sa;
+    double2* Gb = Gm + b * sg;             // multiplier of row r at step c: Gb[c*n + r]
     int* pv = pivp + (int64_t)b * n;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ncols = min(NB, n - kb);
@@ -259,11 +173,21 @@ __global__ __launch_bounds__(NT) void gj_panel_loop_kernel(double2* A, int64_t s
         const double2 pivv = prow[buf][c];
         const double2 d = (pivv.x == 0.0 && pivv.y == 0.0) ? make_double2(0.0, 0.0) : crecip(pivv);
         double2 vnew;
+        const double2 gf = cmul(vc, d);  // f / pivot
+        // the record the update kernel replays (stores are not waited for
+        // at the barriers, which wait on LDS traffic only); 32-bit offsets
+        // (c n + r < 2^15), components selected one by one
+        if (r < n) {
+            const bool piv_row = r == p;
+            double2 rec;
+            rec.x = piv_row ? d.x : gf.x;
+            rec.y = piv_row ? d.y : gf.y;
+            Gb[(unsigned)(c * n + r)] = rec;
+        }
         if (r == p) {
             srow = d;  // the row stays unscaled; true row = d * v, true v[c] = d
             vnew = make_double2(1.0, 0.0);
         } else {
-            const double2 gf = cmul(vc, d);  // f / pivot
 #pragma unroll
             for (int cc = 0; cc < NB; ++cc) {
                 const double2 u = prow[buf][cc];
@@ -291,129 +215,84 @@ __global__ __launch_bounds__(NT) void gj_panel_loop_kernel(double2* A, int64_t s
     if (tid == 0 && zero_col != 0 && info[b] == 0) info[b] = zero_col;
 }
 
-// ------------------------------------------------------------ update
-// Workgroup = (64-column tile, member); 4 waves, each a 32x32 sub-tile of
-// every 64-row block; K = panel width, complex product as four
-// v_mfma_f64_16x16x4 (operand order as in the GEMM kernel).
-template <int NB>
-__global__ __launch_bounds__(256) void gj_update_kernel(double2* A, int64_t sa, int n, int kb, int w,
-                                                        const int* pivp, const int* active) {
-    const int b = blockIdx.y;
-    if (active && !active[b]) return;
-    const int c0 = blockIdx.x * 64;
-    if (c0 >= kb && c0 + 64 <= kb + w) return;  // tile entirely inside the panel
-    __shared__ double2 Rs[NB][64];
-    __shared__ double2 Ps[NB][64];
-    __shared__ unsigned pmask[32];
-    double2* Ab = A + b * sa;
-    const int* pv = pivp + (int64_t)b * n;
-    const int tid = threadIdx.x;
-    if (tid < 32) pmask[tid] = 0u;
-    __syncthreads();
-    if (tid < w) atomicOr(&pmask[pv[kb + tid] >> 5], 1u << (pv[kb + tid] & 31));
-    // R = pivot rows of this tile's columns (old values)
-    for (int e = tid; e < NB * 64; e += 256) {
-        const int t = e >> 6, c = e & 63, col = c0 + c;
-        Rs[t][c] = (t < w && col < n && !(col >= kb && col < kb + w))
-                       ? Ab[pv[kb + t] + (int64_t)col * n]
-                       : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-
-    const int wave = tid >> 6, lane = tid & 63;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int li = lane & 15, lk = lane >> 4;
-    for (int rb = 0; rb < n; rb += 64) {
-        // P block: rows rb..rb+63 of the panel columns, stored [k][i]
-        for (int e = tid; e < NB * 64; e += 256) {
-            const int k = e >> 6, i = e & 63, row = rb + i;
-            Ps[k][i] = (k < w && row < n) ? Ab[row + (int64_t)(kb + k) * n] : make_double2(0.0, 0.0);
-        }
-        __syncthreads();
-        double4_t accR[2][2], accI[2][2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                accR[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
-                accI[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
-            }
-#pragma unroll
-        for (int kq = 0; kq < NB / 4; ++kq) {
-            const int k = kq * 4 + lk;
-            double2 av[2], bv[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) av[i] = Ps[k][wm * 32 + i * 16 + li];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) bv[j] = Rs[k][wn * 32 + j * 16 + li];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
-                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[j].y, av[i].y, accR[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].x, accI[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].y, accI[i][j], 0, 0, 0);
-                }
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int row = rb + wm * 32 + i * 16 + li;
-            if (row >= n) continue;
-            const bool prow_ = (pmask[row >> 5] >> (row & 31)) & 1u;
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int col = c0 + wn * 32 + j * 16 + 4 * q + lk;
-                    if (col >= n || (col >= kb && col < kb + w)) continue;
-                    double2* dst = Ab + row + (int64_t)col * n;
-                    double2 base = prow_ ? make_double2(0.0, 0.0) : *dst;
-                    *dst = make_double2(base.x + accR[i][j][q], base.y + accI[i][j][q]);
-                }
-        }
-        __syncthreads();
-    }
-}
-
 // ------------------------------------------------------------ update (narrow tiles)
-// Workgroup = (TC-column tile, member), all n rows, NT/64 waves.  Wave w
-// takes the 16-row blocks w, w + NT/64, ...; the panel operand P is read
-// from global memory straight into MFMA operand registers (it is written by
-// the panel kernel of the same member just before and is shared by all the
-// member's tiles, so it comes from L2), the pivot rows R of the tile's
-// columns sit in LDS.  Columns are owned by exactly one workgroup, so the
-// old pivot-row values each workgroup needs are never written by another
-// one.  Workgroup ids are spread so that all tiles of member m run on the
-// XCD the panel kernel of m ran on (ids go round-robin over 8 XCDs).
+// Workgroup = (TC-column tile, member), all n rows, NT/64 waves (NT = 512,
+// TC = 32).  Phase 1: wave w owns columns 4w..4w+3 of the tile, 16 lanes per
+// column, lane l holding the values of the panel's pivot rows l and l + 16 in
+// that column; the forward substitution y_c' -= g_{p_c', c} y_c runs in
+// registers with y_c broadcast by a lane shuffle (no barrier), then y goes to
+// LDS.  Phase 2: wave w takes the 16-row blocks w, w + NT/64, ...; the
+// multipliers G are read from global memory straight into MFMA operand
+// registers (written by the panel kernel of the same member just before and
+// shared by all its tiles, so they come from L2), with a pivot row's entries
+// of its own and earlier steps zeroed; y is the other operand.  Columns are
+// owned by exactly one workgroup, so the old pivot-row values a workgroup
+// needs are never written by another one.  Workgroup ids are spread so that
+// all tiles of member m run on the XCD the panel kernel of m ran on (ids go
+// round-robin over 8 XCDs).
 template <int NB, int TC, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gj_update_cols_kernel(double2* A, int64_t sa, int n, int batch,
-                                                            int ntiles, int kb, int w,
-                                                            const int* pivp, const int* active) {
-    static_assert(TC % 16 == 0 && NB % 4 == 0, "tile");
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gj_update_cols_kernel(
+    double2* A, int64_t sa, const double2* Gm, int64_t sg, int n, int batch, int ntiles, int kb, int w,
+    const int* pivp, const int* active) {
+    static_assert(TC % 16 == 0 && NB % 4 == 0 && NB <= 32, "tile");
+    static_assert(NT / 64 * 4 == TC, "phase 1 maps 4 columns to each wave");
     constexpr int NJ = TC / 16, NW = NT / 64, KS = NB / 4;
     const int lin = blockIdx.x;
     const int b = ((lin >> 3) / ntiles) * 8 + (lin & 7);
     const int c0 = ((lin >> 3) % ntiles) * TC;
     if (b >= batch || (active && !active[b])) return;
     if (c0 >= kb && c0 + TC <= kb + w) return;  // tile entirely inside the panel
-    __shared__ double2 Rs[NB][TC];
-    __shared__ unsigned pmask[32];
+    __shared__ double2 Ys[NB][TC];     // y of the pivot rows, tile columns
+    __shared__ double2 Lk[NB][NB];     // Lk[t][c] = g_{p_t, c} for c < t
+    __shared__ double2 dpv[NB];        // 1 / pivot of step t
+    __shared__ signed char pos[1024];  // step of row r in this panel, -1 if none
     double2* Ab = A + b * sa;
+    const double2* Gb = Gm + b * sg;
     const int* pv = pivp + (int64_t)b * n;
     const int tid = threadIdx.x;
-    if (tid < 32) pmask[tid] = 0u;
+    // offsets below are 32-bit: n <= 1024, so row + col n < 2^20
+    for (int r = tid; r < n; r += NT) pos[r] = -1;
+    for (int e = tid; e < NB * NB; e += NT) {
+        const int t = e / NB, c = e % NB;
+        Lk[t][c] = (t < w && c < t) ? Gb[(unsigned)(c * n + pv[kb + t])] : make_double2(0.0, 0.0);
+    }
+    if (tid < w) dpv[tid] = Gb[(unsigned)(tid * n + pv[kb + tid])];
     __syncthreads();
-    if (tid < w) atomicOr(&pmask[pv[kb + tid] >> 5], 1u << (pv[kb + tid] & 31));
-    for (int e = tid; e < NB * TC; e += NT) {
-        const int t = e / TC, c = e % TC, col = c0 + c;
-        Rs[t][c] = (t < w && col < n && !(col >= kb && col < kb + w))
-                       ? Ab[pv[kb + t] + (int64_t)col * n]
-                       : make_double2(0.0, 0.0);
+    if (tid < w) pos[pv[kb + tid]] = (signed char)tid;
+
+    const int wave = tid >> 6, lane = tid & 63;
+    {
+        // phase 1: y = forward substitution, column c0 + 4 wave + lane / 16
+        const int cc = wave * 4 + (lane >> 4), col = c0 + cc;
+        const bool cok = col < n && !(col >= kb && col < kb + w);
+        const int ta = lane & 15, tb = ta + 16;
+        double2 ya = (cok && ta < w) ? Ab[(unsigned)(pv[kb + ta] + col * n)] : make_double2(0.0, 0.0);
+        double2 yb = (cok && tb < w) ? Ab[(unsigned)(pv[kb + tb] + col * n)] : make_double2(0.0, 0.0);
+        const int src0 = lane & ~15;
+        for (int c = 0; c + 1 < w; ++c) {
+            const double2 own = c < 16 ? ya : yb;  // uniform choice
+            const int src = src0 + (c & 15);
+            double2 yc;
+            yc.x = __shfl(own.x, src, 64);
+            yc.y = __shfl(own.y, src, 64);
+            if (ta > c) {
+                const double2 l = Lk[ta][c];
+                ya.x -= l.x * yc.x - l.y * yc.y;
+                ya.y -= l.x * yc.y + l.y * yc.x;
+            }
+            if (tb > c && tb < NB) {
+                const double2 l = Lk[tb][c];
+                yb.x -= l.x * yc.x - l.y * yc.y;
+                yb.y -= l.x * yc.y + l.y * yc.x;
+            }
+        }
+        Ys[ta][cc] = ya;
+        if (tb < NB) Ys[tb][cc] = yb;
     }
     __syncthreads();
 
-    const int wave = tid >> 6, lane = tid & 63;
+    // phase 2: out_r = base_r - sum_k g'_rk y_k; every global load of a row
+    // block is issued before the LDS lookups that decide how it is used
     const int li = lane & 15, lk = lane >> 4;
     for (int r0 = wave * 16; r0 < n; r0 += NW * 16) {
         const int row = r0 + li;
@@ -422,25 +301,41 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gj
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             const int k = ks * 4 + lk;
-            av[ks] = (rok && k < w) ? Ab[row + (int64_t)(kb + k) * n] : make_double2(0.0, 0.0);
+            av[ks] = (rok && k < w) ? Gb[(unsigned)(k * n + row)] : make_double2(0.0, 0.0);
         }
-        const bool prow_ = rok && ((pmask[row >> 5] >> (row & 31)) & 1u);
-        // lane index through an empty asm: the R operands are re-read from
+        double2 base[NJ][4];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = c0 + j * 16 + 4 * q + lk;
+                base[j][q] = (rok && col < n) ? Ab[(unsigned)(row + col * n)] : make_double2(0.0, 0.0);
+            }
+        const int prow = rok ? pos[row] : -1;  // step of this pivot row, -1 otherwise
+        const int py = prow < 0 ? 0 : prow;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const bool keep = ks * 4 + lk > prow;  // a pivot row keeps the later steps only
+            av[ks].x = keep ? av[ks].x : 0.0;
+            av[ks].y = keep ? av[ks].y : 0.0;
+        }
+        const double2 dp = dpv[py];
+        // lane index through an empty asm: the y operands are re-read from
         // LDS for every row block instead of being hoisted into registers
         int ln = lane;
         asm volatile("" : "+v"(ln));
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-            double2 base[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int col = c0 + j * 16 + 4 * q + lk;
-                base[q] = (rok && !prow_ && col < n) ? Ab[row + (int64_t)col * n] : make_double2(0.0, 0.0);
+                const double2 yv = Ys[py][j * 16 + 4 * q + lk];  // pivot rows start from y
+                base[j][q].x = prow >= 0 ? yv.x : base[j][q].x;
+                base[j][q].y = prow >= 0 ? yv.y : base[j][q].y;
             }
             double4_t accR = {0.0, 0.0, 0.0, 0.0}, accI = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
-                const double2 bv = Rs[ks * 4 + (ln >> 4)][j * 16 + (ln & 15)];
+                const double2 bv = Ys[ks * 4 + (ln >> 4)][j * 16 + (ln & 15)];
                 accR = __builtin_amdgcn_mfma_f64_16x16x4f64(bv.x, av[ks].x, accR, 0, 0, 0);
                 accR = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv.y, av[ks].y, accR, 0, 0, 0);
                 accI = __builtin_amdgcn_mfma_f64_16x16x4f64(bv.y, av[ks].x, accI, 0, 0, 0);
@@ -451,7 +346,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void gj
             for (int q = 0; q < 4; ++q) {
                 const int col = c0 + j * 16 + 4 * q + lk;
                 if (col >= n || (col >= kb && col < kb + w)) continue;
-                Ab[row + (int64_t)col * n] = make_double2(base[q].x + accR[q], base[q].y + accI[q]);
+                const double2 o = make_double2(base[j][q].x - accR[q], base[j][q].y - accI[q]);
+                const double2 os = cmul(o, dp);
+                Ab[(unsigned)(row + col * n)] = make_double2(prow >= 0 ? os.x : o.x, prow >= 0 ? os.y : o.y);
             }
         }
     }
@@ -490,45 +387,20 @@ __global__ void zero_int_kernel(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// Kernel choices, read once from the environment: QT_INV_UPDATE=tiles64
-// selects the 64-column update kernel (default: 32-column tiles),
-// QT_INV_PANEL=unrolled the fully unrolled panel kernel (default: rolled).
-bool env_is(const char* name, const char* value) {
-    const char* e = std::getenv(name);
-    return e && std::string(e) == value;
-}
-int update_variant() {
-    static const int v = env_is("QT_INV_UPDATE", "tiles64") ? 0 : 1;
-    return v;
-}
-int panel_variant() {
-    static const int v = env_is("QT_INV_PANEL", "unrolled") ? 0 : 1;
-    return v;
-}
-
 template <int NB, int NT>
 void run_panels(hipStream_t s, const InvArgs& a) {
     const int n = a.n;
     for (int kb = 0; kb < n; kb += NB) {
         const int w = std::min(NB, n - kb);
-        if (panel_variant() == 1)
-            hipLaunchKernelGGL((gj_panel_loop_kernel<NB, NT>), dim3(a.batch), dim3(NT), 0, s, a.A, a.sa, n,
-                               kb, a.piv, a.info, a.active);
-        else
-            hipLaunchKernelGGL((gj_panel_kernel<NB, NT>), dim3(a.batch), dim3(NT), 0, s, a.A, a.sa, n, kb,
-                               a.piv, a.info, a.active);
+        hipLaunchKernelGGL((gj_panel_loop_kernel<NB, NT>), dim3(a.batch), dim3(NT), 0, s, a.A, a.sa, a.G, a.sg,
+                           n, kb, a.piv, a.info, a.active);
         QT_HIP(hipGetLastError());
         if (w == n) continue;
-        if (update_variant() == 1) {
-            constexpr int TC = 32;
-            const int ntiles = (n + TC - 1) / TC;
-            const int grid = ntiles * ((a.batch + 7) / 8) * 8;
-            hipLaunchKernelGGL((gj_update_cols_kernel<NB, TC, 512>), dim3(grid), dim3(512), 0, s, a.A, a.sa,
-                               n, a.batch, ntiles, kb, w, a.piv, a.active);
-        } else {
-            hipLaunchKernelGGL((gj_update_kernel<NB>), dim3((n + 63) / 64, a.batch), dim3(256), 0, s, a.A,
-                               a.sa, n, kb, w, a.piv, a.active);
-        }
+        constexpr int TC = 32;
+        const int ntiles = (n + TC - 1) / TC;
+        const int grid = ntiles * ((a.batch + 7) / 8) * 8;
+        hipLaunchKernelGGL((gj_update_cols_kernel<NB, TC, 512>), dim3(grid), dim3(512), 0, s, a.A, a.sa, a.G,
+                           a.sg, n, a.batch, ntiles, kb, w, a.piv, a.active);
         QT_HIP(hipGetLastError());
     }
 }
@@ -559,6 +431,7 @@ InvArgs members(const InvArgs& a, int b0, int cnt) {
     h.A = a.A + b0 * a.sa;
     h.X = a.X + b0 * a.sx;
     h.piv = a.piv + (int64_t)b0 * a.n;
+    h.G = a.G + b0 * a.sg;
     h.info = a.info + b0;
     h.active = a.active ? a.active + b0 : nullptr;
     return h;
@@ -605,6 +478,7 @@ int inv_parts() {
 void zinv_single(hipStream_t s, const InvArgs& a) {
     if (a.n <= 0 || a.batch <= 0) return;
     if (a.n > 1024) throw hip_error("zinv: block size above 1024", 6);
+    if (!a.G || a.sg < (int64_t)a.n * kInvPanelMax) throw hip_error("zinv: multiplier workspace missing", 1);
     zinv_part(s, a);
 }
 
@@ -612,6 +486,7 @@ void zinv(hipStream_t s, const InvArgs& a, Timer* t) {
     const int n = a.n;
     if (n <= 0 || a.batch <= 0) return;
     if (n > 1024) throw hip_error("zinv: block size above 1024", 6);
+    if (!a.G || a.sg < (int64_t)n * kInvPanelMax) throw hip_error("zinv: multiplier workspace missing", 1);
     if (t && t->enabled) t->begin(s, 1);
     if (a.batch >= 8 && n >= 64) {
         Side& sd = side();

@@ -70,11 +70,11 @@ struct Arena {
     }
 };
 
-// Fixed scratch of qt_invert_batched_noalloc: pivot lists, info flags and a
-// status word, QT_NOALLOC_SCRATCH_INTS int32 per device, held until the
-// process ends.  Host code uses the device address that qt_init takes with
-// hipGetSymbolAddress (Ctx::scratch), never this name.
-static __device__ __align__(256) int scratch[QT_NOALLOC_SCRATCH_INTS];
+// Fixed scratch of qt_invert_batched_noalloc: status word, info flags, pivot
+// lists and the inversion's panel multipliers, QT_NOALLOC_SCRATCH_BYTES per
+// device, held until the process ends.  Host code uses the device address
+// that qt_init takes with hipGetSymbolAddress (Ctx::scratch), never this name.
+static __device__ __align__(256) int scratch[QT_NOALLOC_SCRATCH_BYTES / sizeof(int)];
 
 struct Ctx {
     int device = 0;
@@ -168,6 +168,7 @@ struct Work {
     double2* sigF = nullptr; // first-contact self-energy [B][nn]
     double2* sigL = nullptr; // last-contact self-energy  [B][nn]
     int* piv = nullptr;      // 2B * nmax
+    double2* ginv = nullptr; // 2B * nmax * kInvPanelMax inversion multipliers
     int* info = nullptr;     // 2B
     double* zre = nullptr;   // 2B energies (the batch twice, for the lead members)
     int* kidx = nullptr;     // 2B
@@ -199,6 +200,7 @@ static size_t work_bytes(const qt_ham& h, int B, bool sigma) {
     for (int i = 0; i < NTEMP; ++i) add(sizeof(double2) * (size_t)nn * 2 * B);
     add(sizeof(double2) * (size_t)nn * B * 2);
     add(sizeof(int) * (size_t)2 * B * h.nmax);
+    add(sizeof(double2) * (size_t)2 * B * h.nmax * kInvPanelMax);
     int64_t ndiag = 0;
     for (int s = 0; s < h.nx; ++s) ndiag += h.nm[s];
     add(sizeof(double2) * (size_t)3 * B * ndiag);
@@ -223,6 +225,7 @@ static void work_carve(Arena& a, Work& w, const qt_ham& h, int B, bool sigma) {
     w.sigF = a.take<double2>((size_t)w.nn * B);
     w.sigL = a.take<double2>((size_t)w.nn * B);
     w.piv = a.take<int>((size_t)2 * B * h.nmax);
+    w.ginv = a.take<double2>((size_t)2 * B * h.nmax * kInvPanelMax);
     w.info = a.take<int>((size_t)2 * B);
     w.zre = a.take<double>((size_t)2 * B);
     w.kidx = a.take<int>((size_t)2 * B);
@@ -256,6 +259,8 @@ static void inv_batch(Ctx& c, Work& w, int n, int batch, double2* A, int64_t sa,
     a.sx = sx;
     a.piv = w.piv;
     a.info = w.info;
+    a.G = w.ginv;
+    a.sg = (int64_t)n * kInvPanelMax;
     a.active = active;
     a.flop_batch = flop_batch;
     zinv(c.stream, a, &c.timer);
@@ -965,7 +970,7 @@ const char* qt_status_string(int st) {
         case QT_ERR_UNSUPPORTED: return "unsupported shape or option";
         case QT_ERR_NO_DEVICE: return "no HIP device";
         case QT_ERR_TOO_LARGE:
-            return "n*batch + batch + 256 exceeds the QT_NOALLOC_SCRATCH_INTS scratch of the no-allocation entry point";
+            return "request exceeds the QT_NOALLOC_SCRATCH_BYTES scratch of the no-allocation entry point";
         default: return "unknown status";
     }
 }
@@ -1532,12 +1537,14 @@ int qt_invert_batched(int32_t n, int32_t batch, void* A, int32_t* info) {
         QT_HIP(hipSetDevice(c.device));
         const int64_t nn = (int64_t)n * n;
         c.arena.reserve(sizeof(double2) * (size_t)(2 * nn * batch) +
+                        sizeof(double2) * (size_t)n * kInvPanelMax * batch +
                         sizeof(int) * (size_t)(n + 1) * batch + 8192);
         c.arena.reset();
         Work w;
         double2* dA = c.arena.take<double2>(nn * batch);
         double2* dX = c.arena.take<double2>(nn * batch);
         w.piv = c.arena.take<int>((int64_t)n * batch);
+        w.ginv = c.arena.take<double2>((int64_t)n * kInvPanelMax * batch);
         w.info = c.arena.take<int>(batch);
         QT_HIP(hipMemcpy(dA, A, sizeof(double2) * nn * batch, hipMemcpyHostToDevice));
         inv_batch(c, w, n, batch, dA, nn, dX, nn, nullptr);
@@ -1562,12 +1569,16 @@ __global__ void info_any_kernel(const int* info, int batch, int* flag) {
     if (b < batch && info[b] != 0) atomicOr(flag, 1);
 }
 
+// bytes of scratch one call needs: status word (1 KiB), info, pivots, and the
+// panel multipliers (n x kInvPanelMax complex per block, 256-byte aligned)
+static int64_t noalloc_bytes(int64_t n, int64_t batch) {
+    return 1024 + 4 * batch + 4 * n * batch + 256 + (int64_t)sizeof(double2) * kInvPanelMax * n * batch;
+}
+
 int qt_invert_batched_noalloc(int32_t n, int32_t batch, void* A, void* X, int32_t* info) {
     if (!g_ctx) return QT_ERR_NOT_INIT;
     if (n <= 0 || n > 1024 || batch <= 0 || !A || !X) return QT_ERR_INVALID;
-    // scratch layout (int32): [0, 256) status word and padding,
-    // [256, 256 + batch) info (when the caller passes none), then n*batch pivots
-    if ((int64_t)n * batch + batch + 256 > (int64_t)QT_NOALLOC_SCRATCH_INTS) return QT_ERR_TOO_LARGE;
+    if (noalloc_bytes(n, batch) > (int64_t)QT_NOALLOC_SCRATCH_BYTES) return QT_ERR_TOO_LARGE;
     const int64_t nn = (int64_t)n * n;
     const char *a0 = (const char*)A, *x0 = (const char*)X;
     const int64_t bytes = (int64_t)sizeof(double2) * nn * batch;
@@ -1576,9 +1587,12 @@ int qt_invert_batched_noalloc(int32_t n, int32_t batch, void* A, void* X, int32_
     std::lock_guard<std::mutex> lk(c.mu);
     try {
         QT_HIP(hipSetDevice(c.device));
-        int* flag = c.scratch;
-        int* sinfo = info ? info : c.scratch + 256;
-        int* piv = c.scratch + 256 + batch;  // ends at n*batch + batch + 256 <= capacity
+        char* base = (char*)c.scratch;
+        int* flag = (int*)base;
+        int* sinfo = info ? info : (int*)(base + 1024);
+        int* piv = (int*)(base + 1024 + 4 * (int64_t)batch);
+        const int64_t goff = (1024 + 4 * (int64_t)batch + 4 * (int64_t)n * batch + 255) & ~(int64_t)255;
+        double2* G = (double2*)(base + goff);  // ends within noalloc_bytes(n, batch)
         fill_int(c.stream, flag, 1, 0);  // the array keeps the last call's contents
         InvArgs a;
         a.n = n;
@@ -1589,6 +1603,8 @@ int qt_invert_batched_noalloc(int32_t n, int32_t batch, void* A, void* X, int32_
         a.sx = nn;
         a.piv = piv;
         a.info = sinfo;  // cleared by the inversion itself
+        a.G = G;
+        a.sg = (int64_t)n * kInvPanelMax;
         zinv_single(c.stream, a);
         hipLaunchKernelGGL(info_any_kernel, dim3((batch + 255) / 256), dim3(256), 0, c.stream, sinfo, batch,
                            flag);
@@ -1946,7 +1962,8 @@ struct WSolve {
         const size_t nn = (size_t)nm * nm;
         const size_t blk = sizeof(double2) * nn * B;
         const size_t per = (blk + 256) * (size_t)(19 * nx) + (blk + 256) * 5;
-        return per + sizeof(int) * (size_t)B * (nm + 2) + 8192;
+        return per + sizeof(int) * (size_t)B * (nm + 2) + sizeof(double2) * (size_t)B * nm * kInvPanelMax +
+               8192;
     }
     void carve(Arena& a) {
         auto slabs = [&](int count) { return a.take<double2>((size_t)count * B * nn); };
@@ -1963,6 +1980,7 @@ struct WSolve {
             for (int d = 0; d < 5; ++d) Z[s][d] = slabs(nx);
         for (auto& t : T) t = slabs(1);
         w.piv = a.take<int>((size_t)B * nm);
+        w.ginv = a.take<double2>((size_t)B * nm * kInvPanelMax);
         w.info = a.take<int>((size_t)B);
         w.infoacc = a.take<int>((size_t)B);
     }

@@ -34,7 +34,7 @@ QT_INFO_SINGULAR = 0x1
 QT_INFO_LEAD_NMAX = 0x2
 QT_ERR_SINGULAR = 5
 QT_ERR_TOO_LARGE = 8
-QT_NOALLOC_SCRATCH_INTS = 1 << 20
+QT_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham_destroy",
             "qt_rgf_solve", "qt_sancho", "qt_device_alloc", "qt_device_free", "qt_memcpy_h2d",
