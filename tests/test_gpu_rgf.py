@@ -1,10 +1,30 @@
 """GPU parity of the HIP RGF path against the CPU oracle, through the C ABI.
 
-Tolerances: the plain-C and vendor LAPACK restatements of the oracle already
-differ by up to ~1e-10 (relative to the block scale) on these devices,
-because the Schur-complement inversions amplify rounding; the FP64 device
-path is held to the same floor: 1e-9 elementwise relative to each point's
-largest element, 1e-10 median normwise."""
+The bar is the north star's: 1e-10 relative on LDOS, density (ndos, pdos),
+the contact currents and the diagonal G^R, G^<, G^> entries.  The device has
+no broadening (z = E + 0i, rgf_mod.f90:62), so at some energies z - H - Sigma
+is nearly singular and two correct FP64 operation orders differ by far more
+than 1e-10.  Each point is therefore classified by measurement:
+
+* well conditioned: the oracle's plain and vendor-LAPACK flavours agree to
+  1e-11 (or, where an extended-precision result exists, both are within
+  1e-11 of it): the GPU must be within 1e-10 there;
+* ill conditioned (the rest): the GPU must be as accurate as another correct
+  FP64 operation order.  Against the extended-precision result
+  (tests/_xprec.py, fixtures in tests/golden) its error must not exceed
+  EXEMPT_POINT_FACTOR times the larger of the two flavours' errors at any
+  point, their median ratio must stay <= EXEMPT_MEDIAN and its worst error
+  over the grid within EXEMPT_WORST of the flavours' worst; without an
+  extended-precision result the same ratios are taken against the two
+  flavours' disagreement.
+
+The factors come from measurement at these ill-conditioned points: the
+errors of correct FP64 sweeps (vendor LU, plain LU, numpy LU, textbook
+Gauss-Jordan) against the extended-precision result differ from each other
+by up to ~10x per point, and by up to ~2.4x in their worst value over the
+nb=512 grid, almost all of it from the lead decimation at eta = 1e-5 (a
+sweep fed with another implementation's surface Green's functions inherits
+that implementation's error)."""
 import os
 import time
 
@@ -16,6 +36,12 @@ from quantum_transport_amd import hamiltonian as hm
 pytestmark = pytest.mark.gpu
 
 BOLTZ = 8.61734e-05
+WELL = 1e-11                # flavour agreement that marks a point well conditioned
+BAR = 1e-10                 # parity bar at well-conditioned points
+EXEMPT_POINT_FACTOR = 8.0   # per point, ill-conditioned: GPU error / LAPACK-order error
+EXEMPT_MEDIAN = 2.0         # median of that ratio over the ill-conditioned points
+EXEMPT_WORST = 3.0          # worst GPU error / worst LAPACK-order error over the grid
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 @pytest.fixture(scope="module")
@@ -26,59 +52,79 @@ def qt():
     return rgf
 
 
-def _rel_err(a, b):
-    """elementwise error relative to each point's largest |b|, and the
-    per-point normwise error."""
-    a = np.atleast_2d(a)
-    b = np.atleast_2d(b)
-    scale = np.abs(b).max(axis=1, keepdims=True)
-    scale[scale == 0] = 1.0
-    ew = (np.abs(a - b) / scale).max()
-    nw = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
-    return ew, nw
+def spectra(r, nm):
+    """LDOS, ndos, pdos per slab (negf_mod.f90:141-149: -2 Im Tr G^R,
+    Im Tr G^<, -Im Tr G^> over each slab's orbitals) and tr, tre."""
+    off = np.concatenate([[0], np.cumsum(nm)])
+    red = lambda d, c: np.stack([c * np.atleast_2d(d)[:, off[s]:off[s + 1]].sum(1).imag
+                                 for s in range(len(nm))], 1)
+    return dict(ldos=red(r["gr_diag"], -2.0), ndos=red(r["gl_diag"], 1.0), pdos=red(r["gg_diag"], -1.0),
+                tr=np.atleast_1d(r["tr"])[:, None], tre=np.atleast_1d(r["tre"])[:, None])
 
 
-def _diag_errors(a, b, key):
-    """per-point max elementwise error (relative to the point's scale) and
-    normwise error of diagonal `key` of result a against result b."""
-    if key == "gr_diag":
-        scale = np.abs(b[key]).max(axis=1)
-    else:
-        # G^< (G^>) cancels to rounding noise where f ~ 0 (1): measure them
-        # on the scale of the spectral function G - G^H
-        scale = np.maximum(np.abs(b["gr_diag"]).max(axis=1), np.abs(b[key]).max(axis=1))
+def _obs_err(a, b, ref_spec):
+    """per-point error of each observable of spectra a against b: densities
+    relative to the point's largest |LDOS|, currents to the largest |tr|
+    over the grid (the spectrum's scale; tr itself vanishes in gaps)."""
+    pt_scale = np.maximum(np.abs(ref_spec["ldos"]).max(axis=1), 1e-300)
+    cur_scale = max(np.abs(ref_spec["tr"]).max(), 1e-300)
+    out = {}
+    for k in ("ldos", "ndos", "pdos"):
+        out[k] = np.abs(a[k] - b[k]).max(axis=1) / pt_scale
+    for k in ("tr", "tre"):
+        out[k] = np.abs(a[k] - b[k]).max(axis=1) / cur_scale
+    return out
+
+
+def _diag_err(a, b, key):
+    """per-point max elementwise error of diagonal `key`, relative to the
+    point's scale (G^<, G^> cancel to rounding noise where f ~ 0 or 1: they
+    are measured on the scale of the spectral function)."""
+    scale = np.abs(b["gr_diag"]).max(axis=1)
+    if key != "gr_diag":
+        scale = np.maximum(scale, np.abs(b[key]).max(axis=1))
     scale = np.where(scale == 0, 1.0, scale)[:, None]
-    d = np.abs(np.atleast_2d(a[key]) - np.atleast_2d(b[key])) / scale
-    return d.max(axis=1), np.sqrt((d ** 2).mean(axis=1))
+    return (np.abs(np.atleast_2d(a[key]) - np.atleast_2d(b[key])) / scale).max(axis=1)
 
 
-def _check_diags(gpu, ref, what, ew_tol=1e-9, nw_tol=1e-10, alt=None):
-    """GPU diagonals against the oracle.  With `alt` (the oracle in its other
-    LAPACK flavour) the bar of each point is widened to 20x the disagreement
-    of the two oracle flavours there: at energies where z - H is close to
-    singular (z = E + 0i in the device) two correct FP64 operation orders
-    already differ by far more than 1e-9."""
+def _classify(what, e_gpu, e_a, e_b, against):
+    """e_gpu: GPU error per point; e_a, e_b: the two oracle flavours' errors
+    (against the extended-precision result), or e_a = e_b = their
+    disagreement when there is none."""
+    lap = np.maximum(e_a, e_b)
+    good = lap <= WELL
+    n_ill = int((~good).sum())
+    msg = f"{what}: {good.sum()} well conditioned (GPU max {e_gpu[good].max() if good.any() else 0:.1e})"
+    assert (e_gpu[good] <= BAR).all(), (what, "well conditioned", e_gpu[good].max())
+    if n_ill:
+        ratio = e_gpu[~good] / lap[~good]
+        msg += (f", {n_ill} ill conditioned: GPU/{against} median {np.median(ratio):.2f}, "
+                f"max {ratio.max():.2f}; worst GPU {e_gpu.max():.1e} vs LAPACK {lap.max():.1e}")
+        assert (ratio <= EXEMPT_POINT_FACTOR).all(), (what, "point ratio", ratio.max())
+        assert np.median(ratio) <= EXEMPT_MEDIAN, (what, "median ratio", np.median(ratio))
+        if against == "xprec":
+            assert e_gpu.max() <= EXEMPT_WORST * lap.max(), (what, "worst error", e_gpu.max(), lap.max())
+    print(msg)
+    return n_ill
+
+
+def check_parity(gpu, mkl, plain, nm, what, truth=None):
+    """Observables and diagonal entries of the GPU result against the oracle
+    (both flavours) and, where given, the extended-precision spectra."""
+    sg, sm, sp = spectra(gpu, nm), spectra(mkl, nm), spectra(plain, nm)
+    if truth is not None:
+        st = {k: (truth[k] if truth[k].ndim == 2 else truth[k][:, None]) for k in sg}
+        eg, em, ep = _obs_err(sg, st, st), _obs_err(sm, st, st), _obs_err(sp, st, st)
+        for k in eg:
+            _classify(f"{what} {k}", eg[k], em[k], ep[k], "xprec")
+    else:
+        eg, espread = _obs_err(sg, sm, sm), _obs_err(sp, sm, sm)
+        for k in eg:
+            _classify(f"{what} {k}", eg[k], espread[k], espread[k], "flavour spread")
     for key in ("gr_diag", "gl_diag", "gg_diag"):
-        ew, nw = _diag_errors(gpu, ref, key)
-        tol_p = np.full_like(ew, ew_tol)
-        nw_bar = nw_tol
-        if alt is not None:
-            fe, fn = _diag_errors(alt, ref, key)
-            tol_p = np.maximum(tol_p, 20.0 * fe)
-            nw_bar = max(nw_tol, 20.0 * np.median(fn))
-        worst = int(np.argmax(ew / tol_p))
-        print(f"{what} {key}: max elementwise {ew.max():.2e} (point {worst}: {ew[worst]:.2e} "
-              f"vs bar {tol_p[worst]:.2e}), median normwise {np.median(nw):.2e} (bar {nw_bar:.1e})")
-        assert (ew <= tol_p).all(), (what, key, worst, ew[worst], tol_p[worst])
-        assert np.median(nw) < nw_bar, (what, key, np.median(nw))
-
-
-def _check_currents(gpu, ref, what, tol=1e-8):
-    for key in ("tr", "tre"):
-        scale = max(np.abs(ref[key]).max(), 1e-300)
-        err = np.abs(gpu[key] - ref[key]).max() / scale
-        print(f"{what} {key}: max err / max |{key}| = {err:.2e}")
-        assert err < tol, (what, key, err)
+        e = _diag_err(gpu, mkl, key)
+        spread = _diag_err(plain, mkl, key)
+        _classify(f"{what} {key}", e, spread, spread, "flavour spread")
 
 
 # ----------------------------------------------------------------- Sancho
@@ -141,52 +187,59 @@ def test_forward_cnt_golden(qt, golden_dir):
 # ----------------------------------------------------------------- parity vs oracle
 
 
-def test_forward_parity_small_configs(qt, oracle_mod, flavour):
-    """nb=64 x 32 identical slabs, 256 energies in [-3,3] (eta=1e-5 leads,
-    z = E in the device): diagonals of G^R, G^<, G^> and tr/tre."""
+def test_forward_parity_c1_full_grid(qt, oracle_mod):
+    """nb=64 x 32 identical slabs, all 256 energies in [-3, 3] eV (eta=1e-5
+    leads, z = E in the device), mul = 0, mur = -0.1 eV: LDOS / ndos / pdos
+    per slab and tr / tre against the extended-precision result
+    (tests/golden/c1_xprec.npz), diagonal entries against the oracle."""
+    if not oracle_mod.available("mkl"):
+        pytest.skip("needs the vendor-LAPACK oracle flavour")
     _, dev = hm.synthetic_device(64, 32, seed=1001)
     E = hm.energy_grid(-3.0, 3.0, 256)
-    if flavour != "mkl":
-        E = E[::16]
+    truth = np.load(os.path.join(GOLDEN, "c1_xprec.npz"))
+    assert np.array_equal(truth["E"], E)
     t0 = time.time()
-    ref = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, flavour=flavour)
-    alt = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, flavour="plain") if flavour == "mkl" else None
+    mkl = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, flavour="mkl")
+    plain = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, flavour="plain")
     t1 = time.time()
     H = qt.Hamiltonian.from_device(dev)
     gpu = qt.solve_batch(H, E, None, mul=0.0, mur=-0.1)
-    t2 = time.time()
     H.close()
-    print(f"oracle {t1 - t0:.1f}s, gpu {t2 - t1:.2f}s")
+    print(f"oracle (both flavours) {t1 - t0:.1f}s, gpu {time.time() - t1:.2f}s")
     assert (gpu["info"] == 0).all()
-    assert np.array_equal(gpu["lead_iters"], ref["sancho_iters"])
-    _check_diags(gpu, ref, "C1", alt=alt)
-    _check_currents(gpu, ref, "C1")
+    assert np.array_equal(gpu["lead_iters"], mkl["sancho_iters"])
+    check_parity(gpu, mkl, plain, dev.nm, "C1", truth=truth)
 
 
-def test_forward_parity_kz_batch(qt, oracle_mod, flavour):
+def test_forward_parity_kz_batch(qt, oracle_mod):
     """several kz sets in one batch: H_ii(kz) = H_ii + T_z e^{ikz} + h.c."""
+    if not oracle_mod.available("mkl"):
+        pytest.skip("needs the vendor-LAPACK oracle flavour")
     kz = [0.0, np.pi / 4, np.pi / 2, 3 * np.pi / 4]
     _, dev = hm.synthetic_device(48, 10, seed=1002, kz=kz, tz_density=0.05)
     E = np.repeat(hm.energy_grid(-2.0, 2.0, 12), 4)
     k = np.tile(np.arange(4), 12)
-    ref = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour=flavour)
-    alt = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour="plain") if flavour == "mkl" else None
+    mkl = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour="mkl")
+    plain = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour="plain")
     H = qt.Hamiltonian.from_device(dev)
     gpu = qt.solve_batch(H, E, k, mul=0.0, mur=-0.1)
     H.close()
-    _check_diags(gpu, ref, "kz", alt=alt)
-    _check_currents(gpu, ref, "kz")
+    check_parity(gpu, mkl, plain, dev.nm, "kz")
 
 
-def test_backward_parity(qt, oracle_mod, flavour):
+def test_backward_parity(qt, oracle_mod):
+    """backward variant (rgf_mod.f90:272-472) at unequal mu and T"""
+    if not oracle_mod.available("mkl"):
+        pytest.skip("needs the vendor-LAPACK oracle flavour")
     _, dev = hm.synthetic_device(32, 8, seed=7)
     E = hm.energy_grid(-2.5, 2.5, 24)
-    ref = oracle_mod.rgf_batch(dev, E, None, 0.05, -0.15, variant="backward", flavour=flavour)
+    args = dict(variant="backward", templ=250.0, tempr=400.0)
+    mkl = oracle_mod.rgf_batch(dev, E, None, 0.05, -0.15, flavour="mkl", **args)
+    plain = oracle_mod.rgf_batch(dev, E, None, 0.05, -0.15, flavour="plain", **args)
     H = qt.Hamiltonian.from_device(dev)
-    gpu = qt.solve_batch(H, E, None, variant="backward", mul=0.05, mur=-0.15)
+    gpu = qt.solve_batch(H, E, None, mul=0.05, mur=-0.15, **args)
     H.close()
-    _check_diags(gpu, ref, "backward")
-    _check_currents(gpu, ref, "backward")
+    check_parity(gpu, mkl, plain, dev.nm, "backward")
 
 
 @pytest.mark.parametrize("variant", ["forward", "backward"])
@@ -211,15 +264,17 @@ def test_full_block_lists(qt, oracle_mod, variant):
 
 
 def test_uneven_slabs_coupling_fix(qt, oracle_mod):
+    if not oracle_mod.available("mkl"):
+        pytest.skip("needs the vendor-LAPACK oracle flavour")
     sizes = [24, 40, 16, 32, 24]
     dev = hm.uneven_device(sizes, seed=5, coupling_density=0.2)
     E = hm.energy_grid(-2.0, 2.0, 8)
     H = qt.Hamiltonian.from_device(dev)
     gpu = qt.solve_batch(H, E, None, mul=0.0, mur=-0.1, coupling_fix=True)
     H.close()
-    ref = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, coupling_fix=True)
-    _check_diags(gpu, ref, "uneven")
-    _check_currents(gpu, ref, "uneven")
+    mkl = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, coupling_fix=True, flavour="mkl")
+    plain = oracle_mod.rgf_batch(dev, E, None, 0.0, -0.1, coupling_fix=True, flavour="plain")
+    check_parity(gpu, mkl, plain, dev.nm, "uneven")
 
 
 def test_uneven_slabs_need_fix(qt):
@@ -292,24 +347,29 @@ def test_singular_block_is_zeroed(qt, oracle_mod):
 # ----------------------------------------------------------------- production shape
 
 
-def test_production_block_size_parity(qt, oracle_mod, flavour):
-    """nb=512 blocks (the production slab size) on a short device, 2 kz:
-    full oracle comparison."""
-    if flavour != "mkl":
+def test_production_block_size_parity(qt, oracle_mod):
+    """nb=512 blocks (the production slab size) on a short device, 2 kz,
+    16 energies: spectra against the extended-precision result
+    (tests/golden/n512_xprec.npz) when present, else against the flavour
+    spread; diagonal entries against the oracle."""
+    if not oracle_mod.available("mkl"):
         pytest.skip("needs the vendor-LAPACK oracle for n=512")
     kz = [0.0, np.pi / 2]
     _, dev = hm.synthetic_device(512, 4, seed=1002, kz=kz, tz_density=0.02)
-    E = np.array([-1.5, -0.05, 0.6, 1.9])
-    k = np.array([0, 1, 1, 0])
+    E = np.linspace(-1.9, 1.9, 16)
+    k = np.arange(16) % 2
     t0 = time.time()
-    ref = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour=flavour)
-    alt = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour="plain", nthreads=4)
+    mkl = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour="mkl")
+    plain = oracle_mod.rgf_batch(dev, E, k, 0.0, -0.1, flavour="plain", nthreads=16)
     print(f"oracle n=512 (both flavours): {time.time() - t0:.1f}s")
     H = qt.Hamiltonian.from_device(dev)
     gpu = qt.solve_batch(H, E, k, mul=0.0, mur=-0.1)
     H.close()
-    _check_diags(gpu, ref, "n512", alt=alt)
-    _check_currents(gpu, ref, "n512")
+    path = os.path.join(GOLDEN, "n512_xprec.npz")
+    truth = np.load(path) if os.path.exists(path) else None
+    if truth is not None:
+        assert np.array_equal(truth["E"], E) and np.array_equal(truth["k"], k)
+    check_parity(gpu, mkl, plain, dev.nm, "n512", truth=truth)
 
 
 def test_production_shape_properties(qt):
