@@ -163,6 +163,90 @@ int qt_sancho(int32_t nm, int32_t npts, const double* En, double eta, int32_t nm
               const void* const* H10, void* const* G00, void* const* GBB, int32_t* iters,
               int32_t* converged);
 
+/* ---- learned lead warm start ------------------------------------------ */
+
+/* No reference counterpart (the reference runs plain decimation, sancho in
+ * rgf_mod.f90:482-568, for every point).  A lead (principal layer H00,
+ * coupling H10 per kz) gets a small network (E, kz) -> g0 = diag(d) + U V^H,
+ * rank R, trained on the device by the library's own kernels on Sancho-Rubio
+ * surface Green's functions of the run's first energies; later points start
+ * Newton's method on g = (z - H00 - H01 g H10)^-1 (H01 = H10^H, z = E + i eta)
+ * from g0, each Newton correction solving its Stein equation by Smith
+ * doubling, and fall back to plain Sancho-Rubio when Newton has not reached
+ * the residual |(z - H00 - H01 g H10)^-1 - g|_F <= newton_tol |g|_F within
+ * newton_max steps. */
+typedef struct qt_lead qt_lead;
+
+typedef struct {
+    int32_t rank;        /* R, columns of U and V (32) */
+    int32_t hidden;      /* width of the 3 hidden GELU layers (256) */
+    int32_t batch;       /* training minibatch, multiple of 32 (128) */
+    int32_t epochs;      /* passes over the training points (30) */
+    float lr;            /* Adam step size (1e-3) */
+    int32_t newton_max;  /* Newton steps before the fallback (4) */
+    double newton_tol;   /* accepted surface-equation residual (1e-10) */
+    int32_t smith_max;   /* doublings per Newton correction (30) */
+    int32_t sancho_nmax; /* decimation cap (50, rgf_mod.f90:507) */
+    uint64_t seed;       /* weight initialisation and minibatch order */
+} qt_lead_opts;
+
+#define QT_LEAD_SANCHO 0        /* plain Sancho-Rubio (the reference's method) */
+#define QT_LEAD_NEWTON_MLP 1    /* Newton from the network's g0, Sancho fallback */
+#define QT_LEAD_NEWTON_BARE 2   /* Newton from g0 = (z - H00)^-1, Sancho fallback
+                                   (the method without the network) */
+#define QT_LEAD_NEWTON_GIVEN 3  /* Newton from the caller's blocks G00 (e.g. a
+                                   neighbouring energy's result), Sancho fallback */
+#define QT_LEAD_F_COMPARE 0x1   /* also run plain Sancho-Rubio and report the
+                                   difference (qt_lead_stat.sancho_err) */
+
+typedef struct {
+    int32_t newton_steps;    /* Newton corrections applied */
+    int32_t smith_doublings; /* doublings over all corrections */
+    int32_t fallback;        /* 1: the result is plain Sancho-Rubio's */
+    int32_t sancho_iters;    /* decimation steps run for this point (fallback,
+                                QT_LEAD_SANCHO, or the comparison) */
+    double residual;         /* |Phi(g) - g|_F / |Phi(g)|_F of the returned g
+                                (Newton points; 0 for Sancho results) */
+    double warm_err;         /* |g0 - g|_F / |g|_F (Newton modes) */
+    double sancho_err;       /* |g - g_Sancho|_F / |g_Sancho|_F (QT_LEAD_F_COMPARE) */
+} qt_lead_stat;
+
+typedef struct {
+    double total_ms;         /* the mode's lead work: network + Newton + fallback */
+    double network_ms;       /* features, forward pass, g0 assembly */
+    double newton_ms;        /* Newton steps */
+    double fallback_ms;      /* Sancho-Rubio on the points Newton left */
+    double compare_ms;       /* the comparison run (QT_LEAD_F_COMPARE) */
+    int64_t points, fallbacks, newton_steps, smith_doublings, sancho_iterations;
+} qt_lead_report;
+
+/* Uploads the lead of nk k-points: H00[ik], H10[ik] (nm x nm host blocks);
+ * kz[ik] and the energy window [emin, emax] scale the network's inputs. */
+int qt_lead_create(int32_t nm, int32_t nk, const void* const* H00, const void* const* H10, const double* kz,
+                   double eta, double emin, double emax, const qt_lead_opts* opts, qt_lead** out);
+int qt_lead_destroy(qt_lead* l);
+/* Runs plain Sancho-Rubio on the npts training points (En[i], kidx[i]) and
+ * trains the network on the results for opts->epochs passes; loss[e] (may be
+ * NULL) receives the mean relative squared error of epoch e. */
+int qt_lead_train(qt_lead* l, int32_t npts, const double* En, const int32_t* kidx, double* loss);
+/* Surface Green's functions of npts points by `mode` (QT_LEAD_*); G00 (npts
+ * host blocks, may be NULL; required, and read as the warm start, with
+ * QT_LEAD_NEWTON_GIVEN), stats (npts, may be NULL), report (may be NULL). */
+int qt_lead_surface(qt_lead* l, int32_t mode, int32_t flags, int32_t npts, const double* En,
+                    const int32_t* kidx, void* const* G00, qt_lead_stat* stats, qt_lead_report* report);
+/* Network parameters as FP32 arrays in layer order (W0, b0, W1, b1, W2, b2,
+ * W3, b3; W_l row-major in_l x out_l): the count, and copies out / in (the
+ * BF16 weights used by the products are the rounded masters). */
+int qt_lead_param_count(const qt_lead* l, int64_t* count);
+int qt_lead_get_params(const qt_lead* l, float* params);
+int qt_lead_set_params(qt_lead* l, const float* params);
+/* One forward and backward pass on `batch` points (a multiple of 32) with
+ * host targets G (batch complex128 nm x nm blocks): the network output rows
+ * Y (batch x (2 nm + 4 nm R) floats), the loss and the parameter gradients
+ * (same order as qt_lead_get_params).  Any of Y, loss, grad may be NULL. */
+int qt_lead_eval(qt_lead* l, int32_t batch, const double* En, const int32_t* kidx, const void* G, float* Y,
+                 double* loss, float* grad);
+
 /* ---- block primitives ------------------------------------------------ */
 
 /* Batched complex128 product R[b] = alpha*op(A[b])*op(B[b]) + beta*R[b],

@@ -211,6 +211,27 @@ void count_active(hipStream_t s, const int* active, int batch, int* count);
 // fill ints
 void fill_int(hipStream_t s, int* p, int n, int v);
 
+// ---- multilayer perceptron of the lead warm start (qt_mlp.hip) ----
+// BF16 row-major operands, FP32 results:
+//   C[m][n] = sum_k opA(m, k) opB(k, n) (+ bias[n]),  K % 32 == 0, ld % 8 == 0;
+//   transA = 0: A is M x K (ld lda); 1: A is K x M.  transB = 0: B is K x N; 1: N x K.
+void mlp_gemm(hipStream_t s, int transA, int transB, int M, int N, int K, const uint16_t* A, int lda,
+              const uint16_t* B, int ldb, float* C, int ldc, const float* bias);
+void mlp_gelu_fwd(hipStream_t s, int64_t n, const float* Z, uint16_t* H);            // H = bf16(gelu(Z))
+void mlp_gelu_bwd(hipStream_t s, int64_t n, const float* dA, const float* Z, float* dZ);  // dZ = dA gelu'(Z)
+void mlp_to_bf16(hipStream_t s, int64_t n, const float* X, uint16_t* Y);
+void mlp_colsum(hipStream_t s, int rows, int cols, const float* dZ, float* db);
+void mlp_adam(hipStream_t s, int64_t n, float* w, const float* g, float* m, float* v, uint16_t* wbf, float lr,
+              float b1, float b2, float eps, int step);
+// output row of stride ldy (d re | d im | U re | U im | V re | V im, U(i, k)
+// at i R + k, 2n + 4nR values) ->
+// complex128 d [b][n], U, V [b][n x R] column-major
+void mlp_out_to_lowrank(hipStream_t s, int batch, int n, int R, const float* Y, int ldy, double2* d, double2* U,
+                        double2* V);
+// scale[b] * (diag(E_b), dU_b, dV_b) -> output gradient rows
+void mlp_lowrank_to_out(hipStream_t s, int batch, int n, int R, const double2* E, int64_t sE, const double2* dU,
+                        const double2* dV, const double* scale, float* dY, int ldy);
+
 // Per-launch timing accumulator (HIP events on the library stream)
 struct Timer {
     bool enabled = false;

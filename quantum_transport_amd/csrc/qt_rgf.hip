@@ -22,53 +22,11 @@
 #include <vector>
 
 #include "../../include/qt_rgf.h"
+#include "qt_ctx.h"
 #include "qt_internal.h"
 
-struct qt_ham {
-    int nx = 0, nk = 0;
-    std::vector<int> nm;
-    std::vector<int64_t> hoff;  // offset of slab s inside one k set (elements)
-    std::vector<int64_t> coff;  // offset of coupling s (0..nx)
-    int64_t hset = 0, cset = 0;
-    double2* H = nullptr;       // [nk][hset]
-    double2* C = nullptr;       // [nk][cset]
-    double2* S = nullptr;       // [nk][hset] or null (identity)
-    bool uniform = true;
-    bool leads_same = false;    // both contacts are the same semi-infinite lead
-    int nmax = 0;
-};
 
 namespace qt {
-
-static const double BOLTZ = 8.61734e-05;  // eV/K, rgf_mod.f90:38
-
-static double ferm(double a) { return 1.0 / (1.0 + std::exp(a)); }  // rgf_mod.f90:475-479
-
-// ------------------------------------------------------------ device arena
-struct Arena {
-    char* base = nullptr;
-    size_t cap = 0, used = 0;
-    void reserve(size_t bytes) {
-        if (bytes <= cap) return;
-        release();
-        QT_HIP(hipMalloc((void**)&base, bytes));
-        cap = bytes;
-    }
-    void reset() { used = 0; }
-    template <class T>
-    T* take(size_t count) {
-        const size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
-        if (used + bytes > cap) throw hip_error("workspace arena exhausted", 4);
-        T* p = (T*)(base + used);
-        used += bytes;
-        return p;
-    }
-    void release() {
-        if (base) (void)hipFree(base);
-        base = nullptr;
-        cap = used = 0;
-    }
-};
 
 // Fixed scratch of qt_invert_batched_noalloc: status word, info flags, pivot
 // lists and the inversion's panel multipliers, QT_NOALLOC_SCRATCH_BYTES per
@@ -76,122 +34,11 @@ struct Arena {
 // that qt_init takes with hipGetSymbolAddress (Ctx::scratch), never this name.
 static __device__ __align__(256) int scratch[QT_NOALLOC_SCRATCH_BYTES / sizeof(int)];
 
-struct Ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int* scratch = nullptr;     // device address of `scratch` on this device
-    Timer timer;
-    Arena arena;
-    qt_timing last{};
-    std::mutex mu;
-    int* host_count = nullptr;  // pinned
-    hipEvent_t ev[4];
-    char* staging = nullptr;    // pinned host staging for block outputs (grown on demand)
-    size_t staging_cap = 0;
-    char* host_staging(size_t bytes) {
-        if (bytes > staging_cap) {
-            if (staging) QT_HIP(hipHostFree(staging));
-            staging = nullptr;
-            staging_cap = 0;
-            QT_HIP(hipHostMalloc((void**)&staging, bytes));
-            staging_cap = bytes;
-        }
-        return staging;
-    }
-};
-
-static Ctx* g_ctx = nullptr;
-static std::mutex g_init_mu;
+Ctx* g_ctx = nullptr;
+std::mutex g_init_mu;
 static thread_local std::string g_last_error;
 
-// ------------------------------------------------------------ operand helpers
-static ZOp op_member(const double2* p, int64_t ld, int64_t sb) {
-    ZOp o;
-    o.p = p;
-    o.ld = ld;
-    o.sb = sb;
-    return o;
-}
-static ZOp op_k(const double2* p, int64_t ld, int64_t sk) {
-    ZOp o;
-    o.p = p;
-    o.ld = ld;
-    o.sk = sk;
-    return o;
-}
-
-struct Gemm {
-    hipStream_t s;
-    Timer* t;
-    int batch;
-    const int* kidx;
-    const int* active = nullptr;
-    int flop_batch = -1;
-    int prec = 0;  // 1: int8 digit-slice products (QT_PREC_MIXED)
-    // C = alpha op(A) op(B) + beta D
-    void operator()(int opa, int opb, int M, int N, int K, ZOp A, ZOp B, double2* C, int64_t ldc,
-                    int64_t sbc, double alpha = 1.0, ZOp D = ZOp(), double beta = 0.0) const {
-        ZGemmArgs g;
-        g.M = M;
-        g.N = N;
-        g.K = K;
-        g.batch = batch;
-        g.A = A;
-        g.B = B;
-        g.C = C;
-        g.ldc = ldc;
-        g.sbc = sbc;
-        g.D = D;
-        g.alpha = alpha;
-        g.beta = D.p ? beta : 0.0;
-        g.kidx = kidx;
-        g.active = active;
-        g.flop_batch = flop_batch;
-        g.prec = prec;
-        zgemm(s, opa, opb, g, t);
-    }
-};
-
-// ------------------------------------------------------------ workspace
-constexpr int NTEMP = 15;
-
-struct Work {
-    int B = 0;               // points in the batch
-    int nmax = 0;
-    int64_t nn = 0;          // nmax^2
-    int64_t ndiag = 0;       // sum(nm)
-    double2* Gl = nullptr;   // slab-major: slab s at hoff[s]*B, member stride nm[s]^2
-    double2* Gln = nullptr;
-    double2* H00 = nullptr;  // Hii + sigma_r S per point (only with sigma), same layout
-    double2* SLS = nullptr;  // sigma_lesser S per point (only with sigma), same layout
-    double2* T[NTEMP];       // temps, 2B * nn each
-    double2* sigF = nullptr; // first-contact self-energy [B][nn]
-    double2* sigL = nullptr; // last-contact self-energy  [B][nn]
-    int* piv = nullptr;      // 2B * nmax
-    double2* ginv = nullptr; // 2B * nmax * kInvPanelMax inversion multipliers
-    int* info = nullptr;     // 2B
-    double* zre = nullptr;   // 2B energies (the batch twice, for the lead members)
-    int* kidx = nullptr;     // 2B
-    double* fF = nullptr;    // B  occupation injected at the first contact
-    double* fFc = nullptr;   // B  occupation of the first contact's current
-    double* fL = nullptr;    // B  occupation of the last contact
-    double* nfF = nullptr;   // B  -fF
-    double* nfL = nullptr;   // B  -fL
-    int* active = nullptr;   // 2B
-    int* iters = nullptr;    // 2B
-    int* lead_it = nullptr;  // 2B  (first contact, then last contact)
-    int* act1 = nullptr;     // 2B  decimation flags of the two lead surfaces
-    int* act2 = nullptr;     // 2B
-    int* iters2 = nullptr;   // 2B
-    int* count = nullptr;    // 1
-    double* partial = nullptr;  // 2B * max(64, tiles^2)
-    int* infoacc = nullptr;     // B accumulated QT_INFO bits
-    double* tr = nullptr;       // B
-    double* tre = nullptr;      // B
-    double2* diag = nullptr;    // 3 * B * ndiag staging
-};
-
-static size_t work_bytes(const qt_ham& h, int B, bool sigma) {
+size_t work_bytes(const qt_ham& h, int B, bool sigma) {
     const int64_t nn = (int64_t)h.nmax * h.nmax;
     const int64_t tiles = (h.nmax + 31) / 32;
     size_t b = 0;
@@ -209,7 +56,7 @@ static size_t work_bytes(const qt_ham& h, int B, bool sigma) {
     return b + (1 << 20);
 }
 
-static void work_carve(Arena& a, Work& w, const qt_ham& h, int B, bool sigma) {
+void work_carve(Arena& a, Work& w, const qt_ham& h, int B, bool sigma) {
     a.reset();
     w.B = B;
     w.nmax = h.nmax;
@@ -248,8 +95,8 @@ static void work_carve(Arena& a, Work& w, const qt_ham& h, int B, bool sigma) {
     w.diag = a.take<double2>((size_t)3 * B * w.ndiag);
 }
 
-static void inv_batch(Ctx& c, Work& w, int n, int batch, double2* A, int64_t sa, double2* X,
-                      int64_t sx, const int* active, int flop_batch = -1) {
+void inv_batch(Ctx& c, Work& w, int n, int batch, double2* A, int64_t sa, double2* X,
+               int64_t sx, const int* active, int flop_batch) {
     InvArgs a;
     a.n = n;
     a.batch = batch;
@@ -272,7 +119,7 @@ __global__ void info_accum_kernel(const int* info, int batch, int* acc, int peri
     if (b < batch && info[b] != 0) atomicOr(&acc[b % period], (int)QT_INFO_SINGULAR);
 }
 
-static void note_inv_info(Ctx& c, Work& w, int batch, int period) {
+void note_inv_info(Ctx& c, Work& w, int batch, int period) {
     hipLaunchKernelGGL(info_accum_kernel, dim3((batch + 255) / 256), dim3(256), 0, c.stream, w.info,
                        batch, w.infoacc, period);
     QT_HIP(hipGetLastError());
@@ -284,34 +131,7 @@ __global__ void lead_info_kernel(const int* active, int members, int* acc, int p
     if (m < members && active[m]) atomicOr(&acc[m % period], (int)QT_INFO_LEAD_NMAX);
 }
 
-// ------------------------------------------------------------ Sancho-Rubio
-// Members m with z_m = zre[m] + i eta (rgf_mod.f90:508).  H00 addressed
-// per member or per k; H10 per member (overwritten).  Outputs G00 (and
-// GBB) per member with stride nn.  rgf_mod.f90:482-568.
-struct SanchoIO {
-    int n = 0, members = 0;
-    const double* zre = nullptr;
-    const int* kidx = nullptr;
-    double eta = 1e-5;
-    int nmax = 50;
-    double tol = 1e-100;
-    bool reset = false;
-    ZOp S;           // null = identity
-    ZOp H00;
-    double2* H10 = nullptr;
-    double2* G00 = nullptr;
-    double2* GBB = nullptr;
-    double2* G00b = nullptr;      // opposite surface of the same lead (run with
-                                  // H_01 and H_10 exchanged); null = not wanted
-    int64_t nn = 0;
-    int* iters_out = nullptr;     // members (device)
-    int* iters2_out = nullptr;    // opposite surface
-    int* info_bits = nullptr;     // per point accumulator (device), may be null
-    int period = 1;               // member m belongs to point m % period
-    int64_t total_iters = 0;      // host: sum of iterations over members
-};
-
-static void sancho_run(Ctx& c, Work& w, SanchoIO& io) {
+void sancho_run(Ctx& c, Work& w, SanchoIO& io) {
     const int n = io.n, M = io.members;
     const int64_t nn = io.nn;
     hipStream_t s = c.stream;
@@ -947,7 +767,7 @@ struct Solve {
     }
 };
 
-static int status_of(const hip_error& e) {
+int status_of(const hip_error& e) {
     g_last_error = e.what();
     return e.code;
 }

@@ -41,7 +41,9 @@ EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham
             "qt_memcpy_d2h", "qt_synchronize", "qt_set_kernel_timing", "qt_last_timing",
             "qt_zgemm_batched", "qt_zgemm_batched_prec", "qt_invert_batched", "qt_corr1d2_batched",
             "qt_conv1d2_batched", "qt_gw_polarization", "qt_gw_selfenergy", "qt_gw_mix",
-            "qt_gw_wsolve_batched", "qt_gw_sigma_ephoton", "qt_invert_batched_noalloc"]
+            "qt_gw_wsolve_batched", "qt_gw_sigma_ephoton", "qt_invert_batched_noalloc", "qt_lead_create",
+            "qt_lead_destroy", "qt_lead_train", "qt_lead_surface", "qt_lead_param_count", "qt_lead_get_params",
+            "qt_lead_set_params", "qt_lead_eval"]
 
 
 class QTError(RuntimeError):
@@ -78,6 +80,34 @@ class Timing(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LeadOpts(C.Structure):
+    _fields_ = [("rank", C.c_int32), ("hidden", C.c_int32), ("batch", C.c_int32), ("epochs", C.c_int32),
+                ("lr", C.c_float), ("newton_max", C.c_int32), ("newton_tol", C.c_double),
+                ("smith_max", C.c_int32), ("sancho_nmax", C.c_int32), ("seed", C.c_uint64)]
+
+
+class LeadStat(C.Structure):
+    _fields_ = [("newton_steps", C.c_int32), ("smith_doublings", C.c_int32), ("fallback", C.c_int32),
+                ("sancho_iters", C.c_int32), ("residual", C.c_double), ("warm_err", C.c_double),
+                ("sancho_err", C.c_double)]
+
+
+class LeadReport(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("network_ms", C.c_double), ("newton_ms", C.c_double),
+                ("fallback_ms", C.c_double), ("compare_ms", C.c_double), ("points", C.c_int64),
+                ("fallbacks", C.c_int64), ("newton_steps", C.c_int64), ("smith_doublings", C.c_int64),
+                ("sancho_iterations", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+QT_LEAD_SANCHO = 0
+QT_LEAD_NEWTON_MLP = 1
+QT_LEAD_NEWTON_BARE = 2
+QT_LEAD_NEWTON_GIVEN = 3
+QT_LEAD_F_COMPARE = 0x1
+
 _lib: Optional[C.CDLL] = None
 _inited = False
 
@@ -108,6 +138,17 @@ def lib() -> C.CDLL:
         L.qt_gw_mix.argtypes = [C.c_int64, C.c_double, C.c_void_p, C.c_void_p, C.c_int32]
         L.qt_invert_batched_noalloc.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                                 C.c_void_p]
+        L.qt_lead_create.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                     C.c_double, C.c_double, C.POINTER(LeadOpts), C.POINTER(C.c_void_p)]
+        L.qt_lead_destroy.argtypes = [C.c_void_p]
+        L.qt_lead_train.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.qt_lead_surface.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
+        L.qt_lead_param_count.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.qt_lead_get_params.argtypes = [C.c_void_p, C.c_void_p]
+        L.qt_lead_set_params.argtypes = [C.c_void_p, C.c_void_p]
+        L.qt_lead_eval.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]
         L.qt_gw_wsolve_batched.argtypes = [C.c_int32] * 3 + [C.c_void_p] * 8 + [C.c_int32]
         L.qt_gw_sigma_ephoton.argtypes = [C.c_int32] * 4 + [C.c_double] * 3 + [C.c_void_p] * 6 \
             + [C.c_int32]
@@ -631,3 +672,102 @@ def green_rgf_rs(TEMP, nm: int, nx: int, E: float, mu, Hii, H1i, sigma_lesser_ph
     out = {key: np.stack(v, axis=2) for key, v in res.items()}
     out.update(tr=float(tr[0]), tre=float(tre[0]), info=int(info[0]))
     return out
+
+
+# ----------------------------------------------------------------- learned lead warm start
+
+
+class Lead:
+    """A lead with the learned warm start of its surface Green's function
+    (qt_lead_*): H00[ik], H10[ik] per kz (the principal layer and its
+    coupling), z = E + i eta.  Modes: "sancho" (plain decimation,
+    rgf_mod.f90:482-568), "newton" (Newton from the trained network's g0) and
+    "bare" (Newton from (z - H00)^-1, the method without the network); both
+    Newton modes fall back to decimation."""
+
+    MODES = {"sancho": QT_LEAD_SANCHO, "newton": QT_LEAD_NEWTON_MLP, "bare": QT_LEAD_NEWTON_BARE,
+             "given": QT_LEAD_NEWTON_GIVEN}
+
+    def __init__(self, H00s, H10s, kz, eta: float, emin: float, emax: float, *, rank: int = 32,
+                 hidden: int = 256, batch: int = 128, epochs: int = 30, lr: float = 1e-3,
+                 newton_max: int = 4, newton_tol: float = 1e-10, smith_max: int = 30,
+                 sancho_nmax: int = 50, seed: int = 7):
+        init()
+        self.n = int(H00s[0].shape[0])
+        self.nk = len(H00s)
+        keep = [_f(h) for h in H00s] + [_f(h) for h in H10s]
+        kz = np.ascontiguousarray(kz, dtype=np.float64)
+        self.opts = LeadOpts(rank, hidden, batch, epochs, lr, newton_max, newton_tol, smith_max, sancho_nmax, seed)
+        h = C.c_void_p()
+        check(lib().qt_lead_create(C.c_int32(self.n), C.c_int32(self.nk), _ptrs(keep[:self.nk]),
+                                   _ptrs(keep[self.nk:]), kz.ctypes.data_as(C.c_void_p), C.c_double(eta),
+                                   C.c_double(emin), C.c_double(emax), C.byref(self.opts), C.byref(h)))
+        self.handle = h
+        self.rank, self.hidden = rank, hidden
+        self.width = (2 * self.n + 4 * self.n * rank + 31) // 32 * 32
+
+    def train(self, E, kidx) -> np.ndarray:
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        k = np.ascontiguousarray(kidx, dtype=np.int32)
+        loss = np.zeros(self.opts.epochs)
+        check(lib().qt_lead_train(self.handle, C.c_int32(len(E)), E.ctypes.data_as(C.c_void_p),
+                                  k.ctypes.data_as(C.c_void_p), loss.ctypes.data_as(C.c_void_p)))
+        return loss
+
+    def surface(self, mode: str, E, kidx, compare: bool = False, blocks: bool = False, start=None) -> dict:
+        """start: warm-start blocks for mode "given" (returned updated)."""
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        k = np.ascontiguousarray(kidx, dtype=np.int32)
+        npts = len(E)
+        stats = (LeadStat * max(npts, 1))()
+        rep = LeadReport()
+        if start is not None:
+            blocks = True
+            G = [np.array(b, np.complex128, order="F") for b in start]
+        else:
+            G = [np.zeros((self.n, self.n), np.complex128, order="F") for _ in range(npts)] if blocks else None
+        check(lib().qt_lead_surface(self.handle, C.c_int32(self.MODES[mode]),
+                                    C.c_int32(QT_LEAD_F_COMPARE if compare else 0), C.c_int32(npts),
+                                    E.ctypes.data_as(C.c_void_p), k.ctypes.data_as(C.c_void_p),
+                                    _ptrs(G) if blocks else None, stats, C.byref(rep)))
+        out = {f: np.array([getattr(stats[i], f) for i in range(npts)]) for f, _ in LeadStat._fields_}
+        out["report"] = rep.as_dict()
+        if blocks:
+            out["G00"] = G
+        return out
+
+    def params(self) -> np.ndarray:
+        cnt = C.c_int64()
+        check(lib().qt_lead_param_count(self.handle, C.byref(cnt)))
+        p = np.zeros(cnt.value, np.float32)
+        check(lib().qt_lead_get_params(self.handle, p.ctypes.data_as(C.c_void_p)))
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().qt_lead_set_params(self.handle, p.ctypes.data_as(C.c_void_p)))
+
+    def eval(self, E, kidx, G):
+        """One forward + backward pass: (output rows, loss, parameter gradient)."""
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        k = np.ascontiguousarray(kidx, dtype=np.int32)
+        B = len(E)
+        Gf = np.ascontiguousarray(np.transpose(np.asarray(G, np.complex128), (0, 2, 1)))  # column-major blocks
+        Y = np.zeros((B, self.width), np.float32)
+        loss = C.c_double()
+        grad = np.zeros_like(self.params())
+        check(lib().qt_lead_eval(self.handle, C.c_int32(B), E.ctypes.data_as(C.c_void_p),
+                                 k.ctypes.data_as(C.c_void_p), Gf.ctypes.data_as(C.c_void_p),
+                                 Y.ctypes.data_as(C.c_void_p), C.byref(loss), grad.ctypes.data_as(C.c_void_p)))
+        return Y, loss.value, grad
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().qt_lead_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
