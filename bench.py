@@ -7,9 +7,12 @@ on the 1024-point grid in [-2, 2] eV, Fermi contacts mu_L=0, mu_R=-0.1 eV,
 T=300 K; both Sancho-Rubio leads and the full forward + backward sweep
 (G^R, G^<, G^> diagonals, contact currents) per energy point, complex128.
 
-A step is one device batch of --batch energy points (the (E, kz) pairs are
-taken in order from the 1024 x 4 grid).  Inputs (Hamiltonian blocks) are
-resident in HBM before the timed region; outputs stay on the device.
+A step is one device batch of --batch energy points.  The (E, kz) pairs are
+taken in the order of a fixed stride-2053 permutation of the 1024 x 4 grid,
+so every batch (warm-up and timed) spreads over the whole energy window and
+the mix of lead decimation depths is the grid's.  Inputs (Hamiltonian
+blocks) are resident in HBM before the timed region; outputs stay on the
+device.
 
 The JSON line carries the roofline of the dominant kernel (the batched
 complex GEMM on the FP64 matrix cores: algorithmic 8mnk flops per launch over
@@ -37,13 +40,40 @@ PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 # once, C written once), measured with rocprofv3 FETCH_SIZE / WRITE_SIZE and a
 # copy-kernel calibration; a launch moves this per batch member
 GEMM_BYTES_PER_MEMBER_512 = 12 * 2 ** 20
+STRIDE = 2053  # odd stride of the permutation that orders the (E, kz) points
 
 
-def cpu_baseline(nb: int, slabs: int, threads: int, seed: int) -> dict:
+def host_facts() -> dict:
+    """nproc, CPU model and the BLAS/LAPACK library of the CPU baseline."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = "plain C restatement (no vendor BLAS)"
+    try:
+        import oracle
+
+        if oracle.available("mkl"):
+            mk = C.CDLL(os.path.join("/opt/conda/lib", "libmkl_rt.so"))
+            buf = C.create_string_buffer(256)
+            mk.MKL_Get_Version_String(buf, C.c_int(256))
+            blas = buf.value.decode(errors="replace").strip()
+    except (OSError, AttributeError):
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "blas": blas}
+
+
+def cpu_baseline(nb: int, slabs: int, threads: int, seed: int, validate: bool = False) -> dict:
     """Times the oracle (vendor LAPACK flavour, one point per OpenMP thread)
     on a bounded sample and extrapolates to the full slab count: the cost of
     a point is lead + slabs * slab, with slab and lead separated from runs at
-    2 and 6 slabs."""
+    2 and 6 slabs.  With ``validate`` the same points are also timed at the
+    full slab count (about 50 s on 16 threads)."""
     import oracle
     from quantum_transport_amd import hamiltonian as hm
 
@@ -61,7 +91,7 @@ def cpu_baseline(nb: int, slabs: int, threads: int, seed: int) -> dict:
     slab = max(times[6] - times[2], 0.0) / 4.0
     lead = max(times[2] - 2 * slab, 0.0)
     t_full = lead + slabs * slab  # wall time of `threads` points in parallel
-    return {
+    out = {
         "value": threads / t_full,
         "unit": "energy points/s",
         "cores": threads,
@@ -70,6 +100,16 @@ def cpu_baseline(nb: int, slabs: int, threads: int, seed: int) -> dict:
                    f"nb={nb}, timed at 2 and 6 slabs ({times[2]:.1f}s, {times[6]:.1f}s) and "
                    f"extrapolated to {slabs} slabs (lead {lead:.1f}s + {slabs} x {slab:.2f}s)"),
     }
+    out.update(host_facts())
+    if validate:
+        # the same points at the full slab count, timed directly
+        dev = hm.device_from_wannier(w, slabs, 1, [(0.0, q) for q in kz])
+        t0 = time.perf_counter()
+        oracle.rgf_batch(dev, E, k, 0.0, -0.1, nthreads=threads, flavour=flavour)
+        t_direct = time.perf_counter() - t0
+        out["validation"] = {"direct_s": round(t_direct, 2), "extrapolated_s": round(t_full, 2),
+                             "ratio": round(t_direct / t_full, 3)}
+    return out
 
 
 def run_gw(args) -> dict:
@@ -320,11 +360,14 @@ def time_batches(rgf, H, Eall, Kall, B: int, nd: int, steps: int, warmup: int,
     for key in ("gr_diag", "gl_diag", "gg_diag"):
         bufs[key] = rgf.DeviceBuffer(16 * B * nd)
     cursor = [0]
+    N = len(Eall)
+    perm = (np.arange(N) * STRIDE) % N  # STRIDE is odd and N a power of two: a permutation
+    sampled = []
 
     def step():
-        i0 = cursor[0] % len(Eall)
-        idx = (np.arange(B) + i0) % len(Eall)
+        idx = perm[(cursor[0] + np.arange(B)) % N]
         cursor[0] += B
+        sampled.append(idx)
         rgf.solve_batch_device(H, Eall[idx], Kall[idx], bufs, mul=0.0, mur=-0.1,
                                coupling_fix=coupling_fix, precision=precision)
 
@@ -346,6 +389,11 @@ def time_batches(rgf, H, Eall, Kall, B: int, nd: int, steps: int, warmup: int,
     rgf.set_kernel_timing(False)
     for b in bufs.values():
         b.free()
+    timed = np.concatenate(sampled[warmup:]) if steps else np.zeros(0, int)
+    acc["sample"] = {"points": int(len(timed)), "E_min": float(Eall[timed].min()) if len(timed) else None,
+                     "E_max": float(Eall[timed].max()) if len(timed) else None,
+                     "E_mean": float(Eall[timed].mean()) if len(timed) else None,
+                     "kz_counts": np.bincount(Kall[timed], minlength=int(Kall.max()) + 1).tolist()}
     return elapsed, acc
 
 
@@ -478,6 +526,67 @@ def run_leads(args, hm, rgf) -> dict:
     return res
 
 
+def run_warmstart(args, hm, rgf) -> dict:
+    """Learned lead warm start at the lead shape of the leads workload (nb =
+    256, 4096 energies in [-3, 3] eV x 16 kz, eta = 1e-6).  The run solves
+    its first 10% of points with plain Sancho-Rubio (every 10th energy, all
+    kz: a stride through the grid, so the network interpolates), trains the
+    network on the device on them, and then solves a strided sample of
+    --warm-points of the remaining points three ways: plain Sancho-Rubio,
+    Newton from the network's g0, Newton from the bare layer's (z - H00)^-1
+    (both Newton modes with the Sancho fallback).  value = lead points/s of
+    the warm-started mode (network + Newton + fallback, device time)."""
+    nb = 256
+    kz = np.arange(16) * np.pi / 16
+    w = hm.synthetic_wannier(nb, 1003, tz_density=0.02)
+    H00s, H10s = zip(*[w.mat_def(0.0, 0.0, q, 1) for q in kz])
+    E = hm.energy_grid(-3.0, 3.0, 4096)
+    rgf.init(0)
+    lead = rgf.Lead(list(H00s), list(H10s), kz, 1e-6, -3.0, 3.0, batch=128, epochs=args.warm_epochs)
+    train = np.zeros(len(E), bool)
+    train[::10] = True
+    Et, kt = np.repeat(E[train], 16), np.tile(np.arange(16), int(train.sum()))
+    t0 = time.perf_counter()
+    loss = lead.train(Et, kt)
+    t_train = time.perf_counter() - t0
+    rest = np.where(~train)[0]
+    pts = (np.arange(args.warm_points) * STRIDE) % (len(rest) * 16)
+    Ee, ke = E[rest[pts // 16]], (pts % 16).astype(np.int32)
+    modes = {}
+    for mode in ("sancho", "newton", "bare"):
+        r = lead.surface(mode, Ee, ke, compare=(mode != "sancho"))
+        rep = r["report"]
+        conv = r["fallback"] == 0
+        modes[mode] = {
+            "points_per_s": round(rep["points"] / (rep["total_ms"] * 1e-3), 2),
+            "ms_per_point": round(rep["total_ms"] / rep["points"], 4),
+            "network_ms": round(rep["network_ms"], 2), "newton_ms": round(rep["newton_ms"], 2),
+            "fallback_ms": round(rep["fallback_ms"], 2),
+            "fallback_rate": round(rep["fallbacks"] / rep["points"], 4) if mode != "sancho" else None,
+            "newton_steps_per_point": round(rep["newton_steps"] / rep["points"], 3),
+            "smith_doublings_per_point": round(rep["smith_doublings"] / rep["points"], 3),
+            "sancho_iterations_per_point": round(rep["sancho_iterations"] / rep["points"], 2),
+            "warm_start_err_median": round(float(np.median(r["warm_err"])), 4) if mode != "sancho" else None,
+            "g00_err_vs_sancho_max": float(r["sancho_err"].max()) if mode != "sancho" else 0.0,
+            "converged_points": int(conv.sum()) if mode != "sancho" else None,
+        }
+    lead.close()
+    ws = modes["newton"]
+    return {"metric": "lead surface Green's functions/s @nb=256, eta=1e-6: learned warm start vs Sancho-Rubio",
+            "value": ws["points_per_s"], "unit": "lead points/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": round(ws["ms_per_point"] * args.warm_points, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64 (network bf16, f32 accumulation)",
+            "data": "synthetic (seeded SplitMix64 generator)",
+            "config": {"workload": "learned lead warm start, nb=256, 4096 E in [-3,3] eV x 16 kz, eta=1e-6",
+                       "nb": nb, "energies": 4096, "kz": 16, "train_points": int(len(Et)),
+                       "eval_points": int(args.warm_points)},
+            "training": {"seconds": round(t_train, 2), "epochs": int(args.warm_epochs),
+                         "loss_first": round(float(loss[0]), 4), "loss_last": round(float(loss[-1]), 5)},
+            "ablation": modes,
+            "speed_vs_sancho": round(ws["points_per_s"] / modes["sancho"]["points_per_s"], 3),
+            "cpu_baseline": None}
+
+
 def run_uneven(args, hm, rgf) -> dict:
     """Uneven slabs: 80 slabs with sizes drawn uniformly from {128, 192, ..., 768}
     (direct block construction), 2048 energies in [-2, 2] eV x 8 kz on one
@@ -517,6 +626,7 @@ def run_uneven(args, hm, rgf) -> dict:
                      "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4), "traffic": None,
                      "kernel": "zgemm_mfma_kernel (batched complex128 GEMM)",
                      "launches": acc["gemm_launches"]},
+        "sample": acc["sample"],
         "path": {"lead_iterations_avg": round(K, 2),
                  "algorithmic_tflops_approx": round(points * (sweep + lead) / elapsed / 1e12, 3),
                  "inversion_tflops": round(inv_tf, 3),
@@ -537,15 +647,20 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1002)
-    ap.add_argument("--workload", choices=("rgf", "uneven", "leads", "gw", "corrconv", "ephoton",
+    ap.add_argument("--workload", choices=("rgf", "uneven", "leads", "warmstart", "gw", "corrconv", "ephoton",
                                                "vendor-zgemm"),
                     default="rgf",
                     help="rgf: the headline RGF sweep; uneven: 80 slabs of 128..768 orbitals; "
                          "leads: Sancho-Rubio surface Green's functions at nb=256, eta=1e-6; "
+                         "warmstart: the learned lead warm start against Sancho-Rubio at that shape; "
                          "gw: the GW energy-axis stage; corrconv: batched corr1d2/conv1d2; "
                          "vendor-zgemm: rocBLAS batched ZGEMM at the sweep's shape, for comparison")
     ap.add_argument("--gw-slabs", type=int, default=2)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--warm-points", type=int, default=1024, help="warmstart: evaluated points")
+    ap.add_argument("--warm-epochs", type=int, default=30, help="warmstart: training epochs")
+    ap.add_argument("--cpu-validate", action="store_true",
+                    help="also time the CPU baseline's points at the full slab count")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the RGF path is measured per GPU "
@@ -574,6 +689,9 @@ def main() -> int:
         return 0
     if args.workload == "leads":
         print(json.dumps(run_leads(args, hm, rgf)), flush=True)
+        return 0
+    if args.workload == "warmstart":
+        print(json.dumps(run_warmstart(args, hm, rgf)), flush=True)
         return 0
 
     nb, nx, B = args.nb, args.slabs, args.batch
@@ -619,6 +737,7 @@ def main() -> int:
                      "traffic": (GEMM_BYTES_PER_MEMBER_512 * B) if nb == 512 else None,
                      "kernel": "zgemm_mfma_kernel (batched complex128 GEMM)",
                      "launches": acc["gemm_launches"]},
+        "sample": acc["sample"],
         "path": {"lead_iterations_avg": round(K, 2),
                  "algorithmic_tflops": round(points * path_flops / elapsed / 1e12, 3),
                  "inversion_tflops": round(inv_tf, 3),
@@ -633,7 +752,7 @@ def main() -> int:
     H.close()
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        res["cpu_baseline"] = cpu_baseline(nb, nx, threads, args.seed)
+        res["cpu_baseline"] = cpu_baseline(nb, nx, threads, args.seed, validate=args.cpu_validate)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -227,7 +227,10 @@ int qt_lead_create(int32_t nm, int32_t nk, const void* const* H00, const void* c
 int qt_lead_destroy(qt_lead* l);
 /* Runs plain Sancho-Rubio on the npts training points (En[i], kidx[i]) and
  * trains the network on the results for opts->epochs passes; loss[e] (may be
- * NULL) receives the mean relative squared error of epoch e. */
+ * NULL) receives the mean relative squared error of epoch e.  Points whose
+ * decimation has not converged within opts->sancho_nmax steps are left out
+ * of the training set (their result is no solution of the surface equation);
+ * QT_ERR_INVALID when none is left. */
 int qt_lead_train(qt_lead* l, int32_t npts, const double* En, const int32_t* kidx, double* loss);
 /* Surface Green's functions of npts points by `mode` (QT_LEAD_*); G00 (npts
  * host blocks, may be NULL; required, and read as the warm start, with

@@ -574,27 +574,37 @@ int qt_lead_train(qt_lead* l, int32_t npts, const double* En, const int32_t* kid
         const int Ms = 128;
         LeadWork lw;
         carve(c, L, Ms, false, lw);
-        std::vector<int> it(Ms);
+        std::vector<int> it(Ms), unconverged(Ms), valid;
+        valid.reserve(npts);
         for (int p0 = 0; p0 < npts; p0 += Ms) {
             const int m = std::min(Ms, npts - p0);
             QT_HIP(hipMemcpyAsync(lw.zre, En + p0, sizeof(double) * m, hipMemcpyHostToDevice, c.stream));
             QT_HIP(hipMemcpyAsync(lw.kidx, kidx + p0, sizeof(int) * m, hipMemcpyHostToDevice, c.stream));
             lead_sancho(c, L, lw, m, lw.zre, lw.kidx, Gtrain + (int64_t)p0 * L.nn, it.data());
+            // a member still active after the decimation loop missed the stop test
+            QT_HIP(hipMemcpy(unconverged.data(), lw.w.active, sizeof(int) * m, hipMemcpyDeviceToHost));
+            for (int b = 0; b < m; ++b)
+                if (!unconverged[b]) valid.push_back(p0 + b);
+        }
+        const int nvalid = (int)valid.size();
+        if (nvalid == 0) {
+            QT_HIP(hipFree(Gtrain));
+            return QT_ERR_INVALID;
         }
         // features of every training point, and minibatches of a seeded
-        // permutation refreshed every epoch (both gathered on the device)
+        // permutation of the converged ones, refreshed every epoch (both
+        // gathered on the device)
         const int Bt = L.o.batch;
-        const int steps = (npts + Bt - 1) / Bt;
+        const int steps = (nvalid + Bt - 1) / Bt;
         QT_HIP(hipMalloc((void**)&feat, sizeof(uint16_t) * (size_t)npts * L.nin));
         QT_HIP(hipMalloc((void**)&order, sizeof(int) * (size_t)steps * Bt));
         for (int p0 = 0; p0 < npts; p0 += 4096)
             upload_features(c, L, std::min(4096, npts - p0), En + p0, kidx + p0, feat + (size_t)p0 * L.nin);
         std::mt19937_64 rng(L.o.seed ^ 0x9e3779b97f4a7c15ull);
-        std::vector<int> perm(npts), ord((size_t)steps * Bt);
-        std::iota(perm.begin(), perm.end(), 0);
+        std::vector<int> perm(valid), ord((size_t)steps * Bt);
         for (int ep = 0; ep < L.o.epochs; ++ep) {
             std::shuffle(perm.begin(), perm.end(), rng);
-            for (size_t q = 0; q < ord.size(); ++q) ord[q] = perm[q % npts];  // the last batch wraps around
+            for (size_t q = 0; q < ord.size(); ++q) ord[q] = perm[q % nvalid];  // the last batch wraps around
             QT_HIP(hipStreamSynchronize(c.stream));
             QT_HIP(hipMemcpy(order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
             const double zero = 0.0;

@@ -22,11 +22,15 @@ def qt():
     return rgf
 
 
-def _lead(qt, nb, eta=1e-6, **kw):
+def _lead_blocks(nb=64):
+    """(H00, H10) per kz of the first contact (its decimation input)"""
     _, dev = hm.synthetic_device(nb, 3, seed=2004, kz=KZ, tz_density=0.05)
-    H00 = [dev.Hii[k][0] for k in range(len(KZ))]
-    H10 = [dev.H1i[k][0].conj().T for k in range(len(KZ))]  # the first contact's decimation input
-    return qt.Lead(H00, H10, KZ, eta, -3.0, 3.0, **kw)
+    return [(dev.Hii[k][0], dev.H1i[k][0].conj().T) for k in range(len(KZ))]
+
+
+def _lead(qt, nb, eta=1e-6, **kw):
+    blocks = _lead_blocks(nb)
+    return qt.Lead([b[0] for b in blocks], [b[1] for b in blocks], KZ, eta, -3.0, 3.0, **kw)
 
 
 def test_network_matches_cpu_oracle(qt):
@@ -126,3 +130,32 @@ def test_warm_start_ablation(qt):
               f"{np.median(r['warm_err']) if mode != 'sancho' else 0:.2f}")
     assert np.median(rows["newton"]["warm_err"]) < 0.7 * np.median(rows["bare"]["warm_err"])
     lead.close()
+
+
+def test_training_leaves_out_unconverged_decimations(qt):
+    """With the decimation capped below what in-band points need at eta =
+    1e-6 (out-of-band points converge in 7-11 steps), training on a mix of
+    converged and unconverged points must equal,
+    bit for bit, training on the converged ones alone (the minibatch order
+    depends only on the seed and the number of points kept)."""
+    nmax = 24
+    E = np.linspace(-90.0, 60.0, 96)  # the bands span about -56..24 eV: 56 points lie outside
+    k = np.arange(96) % len(KZ)
+    lead = _lead(qt, 64, batch=32, epochs=3, sancho_nmax=nmax)
+    blocks = _lead_blocks(64)
+    H00, H10 = [b[0] for b in blocks], [b[1] for b in blocks]
+    _, _, it, cv = qt.sancho([H00[i] for i in k], [H10[i] for i in k], E, eta=1e-6, nmax=nmax, want_gbb=False)
+    print(f"{cv.sum()} of {len(E)} points converge within {nmax} steps")
+    assert 32 <= cv.sum() < len(E)
+    p0 = lead.params()
+    loss_mix = lead.train(E, k)
+    p_mix = lead.params()
+    lead2 = _lead(qt, 64, batch=32, epochs=3, sancho_nmax=nmax)
+    lead2.set_params(p0)
+    loss_conv = lead2.train(E[cv], k[cv])
+    print("loss, mixed set:", loss_mix, "converged only:", loss_conv)
+    assert np.isfinite(loss_mix).all()
+    assert np.array_equal(loss_mix, loss_conv)
+    assert np.array_equal(p_mix, lead2.params())
+    lead.close()
+    lead2.close()
