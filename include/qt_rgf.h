@@ -326,7 +326,10 @@ int qt_conv1d2_batched(int32_t n, int64_t nseries, const void* X, const void* Y,
  * ((ij*nx + ix)*nk + ik)*nen with ij = row + col*nm (0-based), nk = nky*nkz.
  * Entries with |row - col| > ndiag are written as zero.  G enters as
  * i*Im(G) (gw_mod.f90:444-446).  dEn = En(2) - En(1).  nen <= 2048.
- * flags: QT_F_DEVICE_OUTPUTS -> every array is a device pointer. */
+ * flags: QT_F_DEVICE_OUTPUTS -> every array is a device pointer.  With host
+ * arrays the slabs are processed in chunks whose device copies stay within
+ * 60% of the free device memory, or within QT_WORKSET_BYTES bytes when that
+ * environment variable is set. */
 
 /* Replaces the P loop of gw_ephoton_3d_ijs (gw_mod.f90:444-550):
  *   P^<_ij(q) = dE sum_k corr(G^<_ij(k), G^>_ji(k')),
@@ -366,7 +369,11 @@ int qt_gw_wsolve_batched(int32_t nm, int32_t nx, int32_t nw, const void* Vii, co
  * Fortran layouts: Mii (nm, nm, length); Gl, Gg (nm, nm, length, nen + 2 nop),
  * the energy axis extended by nop points (hw = nop dE) on each side, so
  * output energy ie reads G at ie and ie + 2 nop; Sr, Sl, Sg (nm, nm, length, nen).
- * hw = 0 is QT_ERR_INVALID.  flags: QT_F_DEVICE_OUTPUTS -> device pointers. */
+ * hw = 0 is QT_ERR_INVALID.  flags: QT_F_DEVICE_OUTPUTS -> device pointers.
+ * Energies are processed in chunks that keep the device working set (host
+ * path: G^<, G^> and the three Sigma of a chunk, plus a product scratch) within
+ * 60% of the free device memory, or within the byte count in the environment
+ * variable QT_WORKSET_BYTES when it is set. */
 int qt_gw_sigma_ephoton(int32_t nm, int32_t length, int32_t nen, int32_t nop, double pre_fact,
                         double intensity, double hw, const void* Mii, const void* Gl, const void* Gg,
                         void* Sr, void* Sl, void* Sg, int32_t flags);

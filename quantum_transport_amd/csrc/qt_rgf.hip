@@ -1498,6 +1498,39 @@ static std::vector<int> band_elements(int nm, int ndiag, bool upper_only) {
     return v;
 }
 
+// device working set allowed to a chunked entry point: 60% of the free
+// memory plus what the arena already holds, or QT_WORKSET_BYTES when set
+static size_t workset_budget(const Ctx& c) {
+    const char* env = getenv("QT_WORKSET_BYTES");
+    if (env && atoll(env) > 0) return (size_t)atoll(env);
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) freeb = (size_t)8 << 30;
+    return (size_t)(0.6 * (double)(freeb + c.arena.cap));
+}
+
+// element-major (nm*nm, nx, nk, nen) arrays: slabs [x0, x0 + cx) of every
+// element between the caller's host array and a compact device chunk
+static void slabs_h2d(Ctx& c, double2* dst, const void* src, int64_t nij, int nx, int x0, int cx,
+                      int64_t per_slab) {
+    const size_t w = sizeof(double2) * (size_t)(per_slab * cx), sp = sizeof(double2) * (size_t)(per_slab * nx);
+    QT_HIP(hipMemcpy2DAsync(dst, w, (const double2*)src + per_slab * x0, sp, w, (size_t)nij, hipMemcpyHostToDevice,
+                            c.stream));
+}
+
+static void slabs_d2h(Ctx& c, void* dst, const double2* src, int64_t nij, int nx, int x0, int cx, int64_t per_slab) {
+    const size_t w = sizeof(double2) * (size_t)(per_slab * cx), dp = sizeof(double2) * (size_t)(per_slab * nx);
+    QT_HIP(hipMemcpy2DAsync((double2*)dst + per_slab * x0, dp, src, w, w, (size_t)nij, hipMemcpyDeviceToHost,
+                            c.stream));
+}
+
+// slabs per chunk of a host-path GW call holding `arrays` chunk arrays
+static int gw_slab_chunk(const Ctx& c, int64_t nij, int nx, int64_t per_slab, int arrays, size_t fixed) {
+    const size_t per = sizeof(double2) * (size_t)(nij * per_slab) * (size_t)arrays + 256 * (size_t)arrays;
+    const size_t budget = workset_budget(c);
+    const int64_t cx = budget > fixed + per ? (int64_t)((budget - fixed) / per) : 1;
+    return (int)std::min<int64_t>(nx, std::max<int64_t>(1, cx));
+}
+
 static bool gw_shape_ok(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t nm, int32_t ndiag) {
     if (nen <= 0 || nky <= 0 || nkz <= 0 || nx <= 0 || nm <= 0 || ndiag < 0) return false;
     const int64_t nij = (int64_t)nm * nm;
@@ -1516,10 +1549,14 @@ int qt_gw_polarization(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_
     try {
         QT_HIP(hipSetDevice(c.device));
         const std::vector<int> el = band_elements(nm, ndiag, false);
-        const int64_t total = (int64_t)nm * nm * nx * nky * nkz * nen;
-        const size_t bytes = sizeof(double2) * (size_t)total;
+        const int64_t nij = (int64_t)nm * nm;
+        const int64_t per_slab = (int64_t)nky * nkz * nen;
         const bool dev = (flags & QT_F_DEVICE_OUTPUTS) != 0;
-        c.arena.reserve(sizeof(int) * el.size() + (dev ? 0 : 6 * bytes) + 8192);
+        const size_t fixed = sizeof(int) * el.size() + 8192;
+        // host path: slabs in chunks (every element of a slab range: 3 G and 3 P)
+        const int cx = dev ? nx : gw_slab_chunk(c, nij, nx, per_slab, 6, fixed);
+        const int64_t chunk = nij * cx * per_slab;
+        c.arena.reserve(fixed + (dev ? 0 : 6 * (sizeof(double2) * (size_t)chunk + 256)));
         c.arena.reset();
         int* del = c.arena.take<int>((int64_t)el.size());
         QT_HIP(hipMemcpyAsync(del, el.data(), sizeof(int) * el.size(), hipMemcpyHostToDevice, c.stream));
@@ -1527,37 +1564,46 @@ int qt_gw_polarization(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_
         a.nen = nen;
         a.nky = nky;
         a.nkz = nkz;
-        a.nx = nx;
         a.nm = nm;
         a.dEn = dEn;
         a.spindeg = spindeg;
         if (dev) {
+            a.nx = nx;
             a.Gr = (const double2*)Gr;
             a.Gl = (const double2*)Gl;
             a.Gg = (const double2*)Gg;
             a.Pr = (double2*)Pr;
             a.Pl = (double2*)Pl;
             a.Pg = (double2*)Pg;
-        } else {
-            double2* g[3];
-            for (auto& p : g) p = c.arena.take<double2>(total);
-            a.Pr = c.arena.take<double2>(total);
-            a.Pl = c.arena.take<double2>(total);
-            a.Pg = c.arena.take<double2>(total);
-            QT_HIP(hipMemcpyAsync(g[0], Gr, bytes, hipMemcpyHostToDevice, c.stream));
-            QT_HIP(hipMemcpyAsync(g[1], Gl, bytes, hipMemcpyHostToDevice, c.stream));
-            QT_HIP(hipMemcpyAsync(g[2], Gg, bytes, hipMemcpyHostToDevice, c.stream));
+            const size_t bytes = sizeof(double2) * (size_t)(nij * nx * per_slab);
+            for (double2* p : {a.Pr, a.Pl, a.Pg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
+            gw_polarization(c.stream, a, del, (int)el.size());
+            QT_HIP(hipStreamSynchronize(c.stream));
+            return QT_OK;
+        }
+        double2* g[3];
+        double2* pc[3];
+        for (auto& p : g) p = c.arena.take<double2>(chunk);
+        for (auto& p : pc) p = c.arena.take<double2>(chunk);
+        const void* src[3] = {Gr, Gl, Gg};
+        void* dst[3] = {Pr, Pl, Pg};
+        for (int x0 = 0; x0 < nx; x0 += cx) {
+            const int cnt = std::min(cx, nx - x0);
+            const size_t bytes = sizeof(double2) * (size_t)(nij * cnt * per_slab);
+            for (int i = 0; i < 3; ++i) {
+                slabs_h2d(c, g[i], src[i], nij, nx, x0, cnt, per_slab);
+                QT_HIP(hipMemsetAsync(pc[i], 0, bytes, c.stream));
+            }
+            a.nx = cnt;
             a.Gr = g[0];
             a.Gl = g[1];
             a.Gg = g[2];
-        }
-        for (double2* p : {a.Pr, a.Pl, a.Pg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
-        gw_polarization(c.stream, a, del, (int)el.size());
-        QT_HIP(hipStreamSynchronize(c.stream));
-        if (!dev) {
-            QT_HIP(hipMemcpy(Pr, a.Pr, bytes, hipMemcpyDeviceToHost));
-            QT_HIP(hipMemcpy(Pl, a.Pl, bytes, hipMemcpyDeviceToHost));
-            QT_HIP(hipMemcpy(Pg, a.Pg, bytes, hipMemcpyDeviceToHost));
+            a.Pr = pc[0];
+            a.Pl = pc[1];
+            a.Pg = pc[2];
+            gw_polarization(c.stream, a, del, (int)el.size());
+            for (int i = 0; i < 3; ++i) slabs_d2h(c, dst[i], pc[i], nij, nx, x0, cnt, per_slab);
+            QT_HIP(hipStreamSynchronize(c.stream));
         }
         return QT_OK;
     } catch (const hip_error& e) {
@@ -1578,10 +1624,14 @@ int qt_gw_selfenergy(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t 
     try {
         QT_HIP(hipSetDevice(c.device));
         const std::vector<int> pr = band_elements(nm, ndiag, true);
-        const int64_t total = (int64_t)nm * nm * nx * nky * nkz * nen;
-        const size_t bytes = sizeof(double2) * (size_t)total;
+        const int64_t nij = (int64_t)nm * nm;
+        const int64_t per_slab = (int64_t)nky * nkz * nen;
         const bool dev = (flags & QT_F_DEVICE_OUTPUTS) != 0;
-        c.arena.reserve(sizeof(int) * pr.size() + (dev ? 0 : 9 * bytes) + 8192);
+        const size_t fixed = sizeof(int) * pr.size() + 8192;
+        // host path: slabs in chunks (6 inputs and 3 outputs of a slab range)
+        const int cx = dev ? nx : gw_slab_chunk(c, nij, nx, per_slab, 9, fixed);
+        const int64_t chunk = nij * cx * per_slab;
+        c.arena.reserve(fixed + (dev ? 0 : 9 * (sizeof(double2) * (size_t)chunk + 256)));
         c.arena.reset();
         int* dpr = c.arena.take<int>((int64_t)pr.size());
         QT_HIP(hipMemcpyAsync(dpr, pr.data(), sizeof(int) * pr.size(), hipMemcpyHostToDevice, c.stream));
@@ -1589,10 +1639,10 @@ int qt_gw_selfenergy(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t 
         a.nen = nen;
         a.nky = nky;
         a.nkz = nkz;
-        a.nx = nx;
         a.nm = nm;
         a.dEn = dEn;
         if (dev) {
+            a.nx = nx;
             a.Gr = (const double2*)Gr;
             a.Gl = (const double2*)Gl;
             a.Gg = (const double2*)Gg;
@@ -1602,30 +1652,36 @@ int qt_gw_selfenergy(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t 
             a.Sr = (double2*)Sr;
             a.Sl = (double2*)Sl;
             a.Sg = (double2*)Sg;
-        } else {
-            const void* src[6] = {Gr, Gl, Gg, Wr, Wl, Wg};
-            double2* in[6];
-            for (int i = 0; i < 6; ++i) {
-                in[i] = c.arena.take<double2>(total);
-                QT_HIP(hipMemcpyAsync(in[i], src[i], bytes, hipMemcpyHostToDevice, c.stream));
-            }
+            const size_t bytes = sizeof(double2) * (size_t)(nij * nx * per_slab);
+            for (double2* p : {a.Sr, a.Sl, a.Sg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
+            gw_selfenergy(c.stream, a, dpr, (int)pr.size());
+            QT_HIP(hipStreamSynchronize(c.stream));
+            return QT_OK;
+        }
+        double2* in[6];
+        double2* sc[3];
+        for (auto& p : in) p = c.arena.take<double2>(chunk);
+        for (auto& p : sc) p = c.arena.take<double2>(chunk);
+        const void* src[6] = {Gr, Gl, Gg, Wr, Wl, Wg};
+        void* dst[3] = {Sr, Sl, Sg};
+        for (int x0 = 0; x0 < nx; x0 += cx) {
+            const int cnt = std::min(cx, nx - x0);
+            const size_t bytes = sizeof(double2) * (size_t)(nij * cnt * per_slab);
+            for (int i = 0; i < 6; ++i) slabs_h2d(c, in[i], src[i], nij, nx, x0, cnt, per_slab);
+            for (double2* p : sc) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
+            a.nx = cnt;
             a.Gr = in[0];
             a.Gl = in[1];
             a.Gg = in[2];
             a.Wr = in[3];
             a.Wl = in[4];
             a.Wg = in[5];
-            a.Sr = c.arena.take<double2>(total);
-            a.Sl = c.arena.take<double2>(total);
-            a.Sg = c.arena.take<double2>(total);
-        }
-        for (double2* p : {a.Sr, a.Sl, a.Sg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
-        gw_selfenergy(c.stream, a, dpr, (int)pr.size());
-        QT_HIP(hipStreamSynchronize(c.stream));
-        if (!dev) {
-            QT_HIP(hipMemcpy(Sr, a.Sr, bytes, hipMemcpyDeviceToHost));
-            QT_HIP(hipMemcpy(Sl, a.Sl, bytes, hipMemcpyDeviceToHost));
-            QT_HIP(hipMemcpy(Sg, a.Sg, bytes, hipMemcpyDeviceToHost));
+            a.Sr = sc[0];
+            a.Sl = sc[1];
+            a.Sg = sc[2];
+            gw_selfenergy(c.stream, a, dpr, (int)pr.size());
+            for (int i = 0; i < 3; ++i) slabs_d2h(c, dst[i], sc[i], nij, nx, x0, cnt, per_slab);
+            QT_HIP(hipStreamSynchronize(c.stream));
         }
         return QT_OK;
     } catch (const hip_error& e) {
@@ -1683,65 +1739,91 @@ int qt_gw_sigma_ephoton(int32_t nm, int32_t length, int32_t nen, int32_t nop, do
         QT_HIP(hipSetDevice(c.device));
         const bool dev = (flags & QT_F_DEVICE_OUTPUTS) != 0;
         const int64_t nn = (int64_t)nm * nm;
-        const int batch = length * nen;
-        const int64_t ns = nn * batch;                          // samples of one Sigma
-        const int64_t ng = nn * length * ((int64_t)nen + 2 * nop);  // samples of one G
-        const size_t sb = sizeof(double2) * (size_t)ns, gb = sizeof(double2) * (size_t)ng;
-        const size_t mb = sizeof(double2) * (size_t)(nn * length);
-        c.arena.reserve(sb + sizeof(int) * (size_t)batch + (dev ? 0 : 2 * gb + mb + 3 * sb) + 8192);
+        const int64_t per_e = nn * length;  // samples of one array at one energy
+        const size_t z = sizeof(double2);
+        // members (slab ix, energy e) are independent: energies go in chunks
+        // sized to the working-set budget.  Per output energy the device holds
+        // T, and on the host path also G^<, G^> and the three Sigma; fixed are
+        // the 2 nop extra energies of each G and M.
+        const size_t per = z * (size_t)per_e * (dev ? 1 : 6) + sizeof(int) * (size_t)length + 1024;
+        const size_t fixed = (dev ? 0 : z * (size_t)(per_e * 4 * (int64_t)nop + nn * length)) + 8192;
+        const size_t budget = workset_budget(c);
+        const int ce = (int)std::min<int64_t>(nen, std::max<int64_t>(1, budget > fixed + per
+                                                                             ? (int64_t)((budget - fixed) / per)
+                                                                             : 1));
+        c.arena.reserve(fixed + per * (size_t)ce);
         c.arena.reset();
-        double2* T = c.arena.take<double2>(ns);
-        int* kidx = c.arena.take<int>(batch);
-        const double2 *dM = (const double2*)Mii, *dGl = (const double2*)Gl, *dGg = (const double2*)Gg;
-        double2 *dSr = (double2*)Sr, *dSl = (double2*)Sl, *dSg = (double2*)Sg;
+        const int64_t ns_max = per_e * ce;
+        double2* T = c.arena.take<double2>(ns_max);
+        int* kidx = c.arena.take<int>((int64_t)length * ce);
+        const double2* dM = (const double2*)Mii;
+        double2 *gl = nullptr, *gg = nullptr, *hs[3] = {nullptr, nullptr, nullptr};
         if (!dev) {
             double2* m = c.arena.take<double2>(nn * length);
-            double2* gl = c.arena.take<double2>(ng);
-            double2* gg = c.arena.take<double2>(ng);
-            QT_HIP(hipMemcpyAsync(m, Mii, mb, hipMemcpyHostToDevice, c.stream));
-            QT_HIP(hipMemcpyAsync(gl, Gl, gb, hipMemcpyHostToDevice, c.stream));
-            QT_HIP(hipMemcpyAsync(gg, Gg, gb, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(m, Mii, z * (size_t)(nn * length), hipMemcpyHostToDevice, c.stream));
             dM = m;
-            dGl = gl;
-            dGg = gg;
-            dSr = c.arena.take<double2>(ns);
-            dSl = c.arena.take<double2>(ns);
-            dSg = c.arena.take<double2>(ns);
+            gl = c.arena.take<double2>(per_e * (ce + 2 * (int64_t)nop));
+            gg = c.arena.take<double2>(per_e * (ce + 2 * (int64_t)nop));
+            for (auto& q : hs) q = c.arena.take<double2>(ns_max);
         }
         {
-            std::vector<int> k((size_t)batch);
-            for (int b = 0; b < batch; ++b) k[(size_t)b] = b % length;
-            QT_HIP(hipMemcpyAsync(kidx, k.data(), sizeof(int) * (size_t)batch, hipMemcpyHostToDevice, c.stream));
+            std::vector<int> k((size_t)length * ce);
+            for (size_t b = 0; b < k.size(); ++b) k[b] = (int)(b % length);
+            QT_HIP(hipMemcpyAsync(kidx, k.data(), sizeof(int) * k.size(), hipMemcpyHostToDevice, c.stream));
             QT_HIP(hipStreamSynchronize(c.stream));
         }
         const double f = pre_fact * intensity / (hw * hw);
-        const int64_t shift = 2 * (int64_t)nop * length * nn;
-        const double2* G[2] = {dGl, dGg};
-        double2* S[2] = {dSl, dSg};
-        for (int x = 0; x < 2; ++x) {
-            eph_sum(c.stream, ns, shift, G[x], S[x]);  // S = G(E - hw) + G(E + hw)
-            ZGemmArgs g;                                // T = M_ix S
-            g.M = g.N = g.K = nm;
-            g.batch = batch;
-            g.A = ZOp{dM, nm, 0, nn};
-            g.B = ZOp{S[x], nm, nn, 0};
-            g.C = T;
-            g.ldc = nm;
-            g.sbc = nn;
-            g.kidx = kidx;
-            zgemm(c.stream, 0, 0, g, &c.timer);
-            g.A = ZOp{T, nm, nn, 0};                    // Sigma = f T M_ix
-            g.B = ZOp{dM, nm, 0, nn};
-            g.C = S[x];
-            g.alpha = f;
-            zgemm(c.stream, 0, 0, g, &c.timer);
-        }
-        eph_retarded(c.stream, ns, dSl, dSg, dSr);
-        QT_HIP(hipStreamSynchronize(c.stream));
-        if (!dev) {
-            QT_HIP(hipMemcpy(Sr, dSr, sb, hipMemcpyDeviceToHost));
-            QT_HIP(hipMemcpy(Sl, dSl, sb, hipMemcpyDeviceToHost));
-            QT_HIP(hipMemcpy(Sg, dSg, sb, hipMemcpyDeviceToHost));
+        const int64_t shift = 2 * (int64_t)nop * per_e;
+        for (int e0 = 0; e0 < nen; e0 += ce) {
+            const int cnt = std::min(ce, nen - e0);
+            const int64_t ns = per_e * cnt;
+            const int64_t off = per_e * e0;  // of energy e0 in G (padded axis) and in Sigma
+            const double2 *cGl, *cGg;
+            double2 *cSr, *cSl, *cSg;
+            if (dev) {
+                cGl = (const double2*)Gl + off;
+                cGg = (const double2*)Gg + off;
+                cSr = (double2*)Sr + off;
+                cSl = (double2*)Sl + off;
+                cSg = (double2*)Sg + off;
+            } else {
+                const size_t gbytes = z * (size_t)(ns + shift);
+                QT_HIP(hipMemcpyAsync(gl, (const double2*)Gl + off, gbytes, hipMemcpyHostToDevice, c.stream));
+                QT_HIP(hipMemcpyAsync(gg, (const double2*)Gg + off, gbytes, hipMemcpyHostToDevice, c.stream));
+                cGl = gl;
+                cGg = gg;
+                cSr = hs[0];
+                cSl = hs[1];
+                cSg = hs[2];
+            }
+            const double2* G[2] = {cGl, cGg};
+            double2* S[2] = {cSl, cSg};
+            for (int x = 0; x < 2; ++x) {
+                eph_sum(c.stream, ns, shift, G[x], S[x]);  // S = G(E - hw) + G(E + hw)
+                ZGemmArgs g;                                // T = M_ix S
+                g.M = g.N = g.K = nm;
+                g.batch = length * cnt;
+                g.A = ZOp{dM, nm, 0, nn};
+                g.B = ZOp{S[x], nm, nn, 0};
+                g.C = T;
+                g.ldc = nm;
+                g.sbc = nn;
+                g.kidx = kidx;
+                zgemm(c.stream, 0, 0, g, &c.timer);
+                g.A = ZOp{T, nm, nn, 0};                    // Sigma = f T M_ix
+                g.B = ZOp{dM, nm, 0, nn};
+                g.C = S[x];
+                g.alpha = f;
+                zgemm(c.stream, 0, 0, g, &c.timer);
+            }
+            eph_retarded(c.stream, ns, cSl, cSg, cSr);
+            if (!dev) {
+                const size_t sbytes = z * (size_t)ns;
+                QT_HIP(hipMemcpyAsync((double2*)Sr + off, cSr, sbytes, hipMemcpyDeviceToHost, c.stream));
+                QT_HIP(hipMemcpyAsync((double2*)Sl + off, cSl, sbytes, hipMemcpyDeviceToHost, c.stream));
+                QT_HIP(hipMemcpyAsync((double2*)Sg + off, cSg, sbytes, hipMemcpyDeviceToHost, c.stream));
+            }
+            QT_HIP(hipStreamSynchronize(c.stream));
         }
         return QT_OK;
     } catch (const hip_error& e) {

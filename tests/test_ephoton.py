@@ -137,6 +137,39 @@ def test_gpu_ephoton_device_pointers(qt, oracle_mod):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("budget", ["1", "420000", "3000000"])
+def test_gpu_ephoton_energy_chunks(qt, oracle_mod, monkeypatch, budget):
+    """The working set capped through QT_WORKSET_BYTES: 1 byte gives one
+    energy per chunk, 420 kB about 2 energies on the host path (the last
+    chunk shorter), 3 MB the whole axis.  Host and device paths give the
+    same bits as the uncapped call and match the oracle."""
+    import ctypes as C
+    nm, length, nen, nop = 20, 3, 7, 2
+    rng = np.random.default_rng(77)
+    M, Gl, Gg = (np.asfortranarray(a) for a in _inputs(rng, nm, length, nen, nop))
+    args = (nop, 0.6, 1.7, 0.45)
+    free = qt.gw_sigma_ephoton(M, Gl, Gg, *args)
+    monkeypatch.setenv("QT_WORKSET_BYTES", budget)
+    print("ephoton chunked, budget", budget, flush=True)
+    host = qt.gw_sigma_ephoton(M, Gl, Gg, *args)
+    ins = []
+    for a in (M, Gl, Gg):
+        b = qt.DeviceBuffer(a.nbytes)
+        qt.check(qt.lib().qt_memcpy_h2d(b.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes)))
+        ins.append(b)
+    outs = [qt.DeviceBuffer(16 * nm * nm * length * nen) for _ in range(3)]
+    qt.check(qt.lib().qt_gw_sigma_ephoton(nm, length, nen, nop, *args[1:], *(b.ptr for b in ins),
+                                          *(b.ptr for b in outs), qt.QT_F_DEVICE_OUTPUTS))
+    dev = [b.to_host(np.complex128, (nen, length, nm, nm)).transpose(3, 2, 1, 0) for b in outs]
+    for b in ins + outs:
+        b.free()
+    ref = oracle_mod.gw_sigma_ephoton(M, Gl, Gg, *args)
+    for f, h, d, r in zip(free, host, dev, ref):
+        assert np.array_equal(f, h) and np.array_equal(f, d)
+        assert np.abs(h - r).max() <= 1e-12 * np.abs(r).max()
+
+
+@pytest.mark.gpu
 def test_gpu_ephoton_rejects_bad_arguments(qt):
     rng = np.random.default_rng(3)
     M, Gl, Gg = _inputs(rng, 2, 1, 2, 1)

@@ -67,6 +67,30 @@ def test_selfenergy_matches_oracle(qt, oracle_mod, case):
         assert np.array_equal(g4, g4.transpose(1, 0, 2, 3, 4))
 
 
+@pytest.mark.parametrize("budget", ["1", "230000"])
+def test_slab_chunks_match_whole_call(qt, oracle_mod, monkeypatch, budget):
+    """Host-path calls with the device working set capped (QT_WORKSET_BYTES):
+    1 byte processes one slab per chunk, 230 kB three slabs for P and two for
+    Sigma (the last chunk shorter) at nm = 3, 5 slabs, 2 k points, 40
+    energies.  Same bits as the uncapped
+    call, and the oracle's values."""
+    nm, nx, nky, nkz, nen, ndiag = 3, 5, 1, 2, 40, 1
+    G = _arrays(91, nm, nx, nky * nkz, nen, 3)
+    W = _arrays(92, nm, nx, nky * nkz, nen, 3)
+    P0 = qt.gw_polarization(*G, nky, nkz, ndiag, 0.02, 2.0)
+    S0 = qt.gw_selfenergy(*G, *W, nky, nkz, ndiag, 0.02)
+    monkeypatch.setenv("QT_WORKSET_BYTES", budget)
+    print("gw chunked, budget", budget, flush=True)
+    P1 = qt.gw_polarization(*G, nky, nkz, ndiag, 0.02, 2.0)
+    S1 = qt.gw_selfenergy(*G, *W, nky, nkz, ndiag, 0.02)
+    for a, b in zip(P0 + S0, P1 + S1):
+        assert np.array_equal(a, b)
+    for g, r in zip(P1, oracle_mod.gw_polarization(*G, nky, nkz, ndiag, 0.02, 2.0)):
+        _close(g, r, "P")
+    for g, r in zip(S1, oracle_mod.gw_selfenergy(*G, *W, nky, nkz, ndiag, 0.02)):
+        _close(g, r, "Sigma")
+
+
 def test_mix_matches_oracle(qt, oracle_mod):
     rng = np.random.default_rng(2)
     S = rng.standard_normal(5000) + 1j * rng.standard_normal(5000)
