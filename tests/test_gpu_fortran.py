@@ -206,3 +206,105 @@ def test_qt_negf_coo_route(tmp_path, oracle_mod, flavour):
     for got, key in zip(rec[1:], ("tr", "tre")):
         want = np.sum(ref[key]) * dE * e0 / tpi / hbar * e0
         assert abs(float(got) - want) <= 1e-5 * abs(want), (key, got, want)
+
+
+POINT = os.path.join(ROOT, "tests", "fortran", "_build", "point_check")
+SENTINEL = 1234.5 - 678.25j  # point_check's fill of every output block before a call
+
+
+def _point_run(tmp_path, nm, Hii, H1i, En, mul, mur, tl, tr_, mode=0):
+    """runs tests/fortran/point_check on one device and energy; returns the
+    completed process and, when it succeeded, the parsed outputs"""
+    nx = len(nm)
+    with open(tmp_path / "point_in.bin", "wb") as fh:
+        fh.write(np.array([nx, *nm, mode], np.int32).tobytes())
+        fh.write(np.array([En, mul, mur, tl, tr_], np.float64).tobytes())
+        for b in list(Hii) + list(H1i):
+            fh.write(np.asfortranarray(b, np.complex128).tobytes(order="F"))
+    r = subprocess.run([POINT], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        return r, None
+    raw = np.fromfile(tmp_path / "point_out.bin", dtype=np.uint8)
+    pos = 0
+
+    def take(count, dtype):
+        nonlocal pos
+        a = raw[pos:pos + count * np.dtype(dtype).itemsize].view(dtype)
+        pos += count * np.dtype(dtype).itemsize
+        return a
+
+    def lists(count):
+        out = []
+        for _ in range(count):
+            out.append([take(n * n, np.complex128).reshape(n, n, order="F") for n in nm])
+        t = take(2, np.float64)
+        return out, float(t[0]), float(t[1])
+
+    res = {"forward": lists(6), "backward": lists(6), "const": lists(4)}
+    n0 = nm[0]
+    res["sancho"] = (take(n0 * n0, np.complex128).reshape(n0, n0, order="F"),
+                     take(n0 * n0, np.complex128).reshape(n0, n0, order="F"))
+    assert pos == raw.size
+    return r, res
+
+
+@pytest.mark.gpu
+def test_fortran_per_point_entries(tmp_path, oracle_mod):
+    """rgf_mod::rgf_variableblock_forward / _backward (rgf_mod.f90:46-54,
+    272-280), cuda_rgf_mod::cuda_rgf_constblocksize (cuda_rgf_mod.f90:53-59)
+    and selfenergy_mod::sancho (rgf_mod.f90:482) called from a Fortran
+    program with type_matrix_complex lists, against the oracle's full
+    blocks at unequal contact potentials and temperatures; blocks a routine
+    does not write (Gln of the sweep's last slab) must be left as they were.  Every block list
+    must be within 1e-10 of the oracle where its two flavours agree to
+    1e-11, else within 8x their disagreement."""
+    if not os.path.exists(POINT):
+        pytest.fail("point_check not built (make -C tests/fortran)")
+    from quantum_transport_amd import hamiltonian as hm
+
+    nb, nx = 12, 5
+    _, dev = hm.synthetic_device(nb, nx, seed=31, coupling_density=0.3)
+    Hii, H1i = dev.Hii[0], dev.H1i[0]
+    mul, mur, tl, tr_ = 0.1, -0.15, 300.0, 250.0
+    names = ("G_r", "G_lesser", "G_greater", "Jdens", "Gl", "Gln")
+    currents = []  # (got, mkl, plain) of tr and tre, judged on the scale of the whole grid
+    for En in (-1.3, 0.2, 1.1):
+        print("fortran per-point entries, E =", En, flush=True)
+        r, got = _point_run(tmp_path, [nb] * nx, Hii, H1i, En, mul, mur, tl, tr_)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        for variant, key in (("forward", "forward"), ("backward", "backward"), ("forward", "const")):
+            refs = [oracle_mod.rgf(Hii, H1i, En, mul, mur, tl, tr_, variant=variant, flavour=f)
+                    for f in ("mkl", "plain")]
+            blocks, t, te = got[key]
+            for name, gl in zip(names, blocks):
+                # blocks the reference routine never writes (all zero in the
+                # oracle's fresh lists) must come back as the driver's sentinel
+                a = [SENTINEL * np.ones_like(x) if not x.any() else x for x in refs[0][name]]
+                b = [SENTINEL * np.ones_like(x) if not x.any() else x for x in refs[1][name]]
+                scale = max(np.abs(x).max() for x in a)
+                err = max(np.abs(x - y).max() for x, y in zip(gl, a)) / scale
+                spread = max(np.abs(x - y).max() for x, y in zip(a, b)) / scale
+                assert err <= max(1e-10, 8 * spread), (En, key, name, err, spread)
+            currents += [(t, refs[0]["tr"], refs[1]["tr"], En, key), (te, refs[0]["tre"], refs[1]["tre"], En, key)]
+        g00, gbb, _, _ = oracle_mod.sancho(None, Hii[0], H1i[0], En, flavour="mkl")
+        for x, y in zip(got["sancho"], (g00, gbb)):
+            assert np.abs(x - y).max() <= 1e-10 * np.abs(y).max()
+    # tr vanishes in gaps (rounding noise there): relative to its largest value
+    cur = max(abs(c[1]) for c in currents)
+    assert cur > 0
+    for v, a, b, En, key in currents:
+        assert abs(v - a) / cur <= max(1e-10, 8 * abs(a - b) / cur), (En, key, v, a)
+
+
+@pytest.mark.gpu
+def test_fortran_per_point_refuses_nonuniform_potential(tmp_path):
+    """A mul matrix that is not uniform stops the program with a message
+    (the device path takes one potential and temperature per contact)."""
+    if not os.path.exists(POINT):
+        pytest.fail("point_check not built (make -C tests/fortran)")
+    from quantum_transport_amd import hamiltonian as hm
+
+    _, dev = hm.synthetic_device(6, 3, seed=32)
+    r, got = _point_run(tmp_path, [6] * 3, dev.Hii[0], dev.H1i[0], 0.3, 0.0, -0.1, 300.0, 300.0, mode=1)
+    assert r.returncode != 0 and got is None
+    assert "mul must be a uniform matrix" in r.stdout + r.stderr
