@@ -636,6 +636,31 @@ def run_uneven(args, hm, rgf) -> dict:
     }
 
 
+def run_copy(args) -> dict:
+    """Device-to-device copies of 256 MiB (torch's copy kernel): the plain
+    copy of known size against which FETCH_SIZE / WRITE_SIZE are scaled, and
+    the copy bandwidth the HBM actually sustains."""
+    import torch
+
+    n = 256 << 20
+    dev = torch.device("cuda", 0)
+    a = torch.ones(n, dtype=torch.uint8, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    reps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"metric": "device-to-device copy bandwidth (read + write)", "value": round(2 * n / dt / 1e9, 1),
+            "unit": "GB/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": round(1e3 * dt, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+            "data": "synthetic (ones)", "config": {"workload": "256 MiB device-to-device copy", "bytes": n},
+            "cpu_baseline": None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -648,13 +673,14 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--workload", choices=("rgf", "uneven", "leads", "warmstart", "gw", "corrconv", "ephoton",
-                                               "vendor-zgemm"),
+                                               "vendor-zgemm", "copy"),
                     default="rgf",
                     help="rgf: the headline RGF sweep; uneven: 80 slabs of 128..768 orbitals; "
                          "leads: Sancho-Rubio surface Green's functions at nb=256, eta=1e-6; "
                          "warmstart: the learned lead warm start against Sancho-Rubio at that shape; "
                          "gw: the GW energy-axis stage; corrconv: batched corr1d2/conv1d2; "
-                         "vendor-zgemm: rocBLAS batched ZGEMM at the sweep's shape, for comparison")
+                         "vendor-zgemm: rocBLAS batched ZGEMM at the sweep's shape, for comparison; "
+                         "copy: 256 MiB device-to-device copies, the HBM counter calibration")
     ap.add_argument("--gw-slabs", type=int, default=2)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--warm-points", type=int, default=1024, help="warmstart: evaluated points")
@@ -679,6 +705,9 @@ def main() -> int:
         return 0
     if args.workload == "vendor-zgemm":
         print(json.dumps(run_vendor_zgemm(args)), flush=True)
+        return 0
+    if args.workload == "copy":
+        print(json.dumps(run_copy(args)), flush=True)
         return 0
 
     from quantum_transport_amd import hamiltonian as hm
