@@ -412,6 +412,27 @@ typedef struct {
 int qt_set_kernel_timing(int enable);
 int qt_last_timing(qt_timing* t);
 
+/* ---- device slicing ------------------------------------------------------ */
+
+/* Cuts a device graph into slices with connections only between neighbouring
+ * slices (the block-tridiagonal structure of the sweeps), replaces
+ * graph_partition::slice (graph_partition.f90:426-541) and the COO-to-graph
+ * step its caller needs (convert_fromCOO, :587-595, has an empty body).
+ * Host only: needs no qt_init and no device.
+ *   npts points 0..npts-1; nedge undirected edges, point pairs
+ *   edges[2e], edges[2e+1] (self pairs and repeats ignored) -- for a matrix,
+ *   its nonzero off-diagonal entries;
+ *   left[nleft], right[nright]: contact points; nright = 0 slices by
+ *   breadth-first distance from the left contact (slice_1contact, :426-450),
+ *   else by recursive bisection between the contacts (slice_2contacts,
+ *   :452-541), a part of at most nmax points (or no more than its outer
+ *   contact) kept as one slice.
+ * Out: slice_of[npts] = slice of each point (0-based, slices in order from
+ * the left contact), *nslices.  QT_ERR_INVALID for bad ids or when the
+ * graph is not connected (a point the left contact cannot reach). */
+int qt_graph_slice(int32_t npts, int64_t nedge, const int32_t* edges, int32_t nleft, const int32_t* left,
+                   int32_t nright, const int32_t* right, int32_t nmax, int32_t* slice_of, int32_t* nslices);
+
 #ifdef __cplusplus
 }
 #endif

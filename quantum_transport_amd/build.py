@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libqtrgf.so")
 SOURCES = ["qt_gemm.hip", "qt_gemm_i8.hip", "qt_inv.hip", "qt_elem.hip", "qt_fft.hip", "qt_gw.hip", "qt_rgf.hip",
-           "qt_mlp.hip", "qt_lead.hip"]
+           "qt_mlp.hip", "qt_lead.hip", "qt_graph.hip"]
 ARCH = os.environ.get("QT_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -43,7 +43,7 @@ EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham
             "qt_conv1d2_batched", "qt_gw_polarization", "qt_gw_selfenergy", "qt_gw_mix",
             "qt_gw_wsolve_batched", "qt_gw_sigma_ephoton", "qt_invert_batched_noalloc", "qt_lead_create",
             "qt_lead_destroy", "qt_lead_train", "qt_lead_surface", "qt_lead_param_count", "qt_lead_get_params",
-            "qt_lead_set_params", "qt_lead_eval"]
+            "qt_lead_set_params", "qt_lead_eval", "qt_graph_slice"]
 
 
 class QTError(RuntimeError):
@@ -152,6 +152,8 @@ def lib() -> C.CDLL:
         L.qt_gw_wsolve_batched.argtypes = [C.c_int32] * 3 + [C.c_void_p] * 8 + [C.c_int32]
         L.qt_gw_sigma_ephoton.argtypes = [C.c_int32] * 4 + [C.c_double] * 3 + [C.c_void_p] * 6 \
             + [C.c_int32]
+        L.qt_graph_slice.argtypes = [C.c_int32, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                                     C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]
         _lib = L
     return _lib
 
