@@ -435,10 +435,14 @@ __global__ void gw_mix_kernel(int64_t n, double alpha, double2* S, const double2
     }
 }
 
-// one radix-8 butterfly per thread where L allows (512 threads at L = 4096)
+// one butterfly per thread where L allows (512 radix-8 threads at L = 4096),
+// at most QT_GW_NTMAX threads
+#ifndef QT_GW_NTMAX
+#define QT_GW_NTMAX 512
+#endif
 template <int L>
 struct GwCfg {
-    static constexpr int NT = L / 8 >= 512 ? 512 : (L / 8 >= 64 ? L / 8 : 64);
+    static constexpr int NT = L / RADIX >= QT_GW_NTMAX ? QT_GW_NTMAX : (L / RADIX >= 64 ? L / RADIX : 64);
 };
 
 template <int L>
