@@ -636,6 +636,49 @@ def run_uneven(args, hm, rgf) -> dict:
     }
 
 
+def run_scba(args) -> dict:
+    """One GW-SCBA iteration on the device (qt_gw_scba_step: G of every energy
+    by the green_RGF_RS recursion with the current Sigma, P, W, Sigma, mixing;
+    every array device resident) at the GW shape's block size and energy
+    axis: nm = 256 orbitals per slab, 2048 energies in [-4, 4] eV, one kz, on
+    --scba-slabs slabs (default 6: the element-major arrays of the 50-slab
+    device would need 15 x 107 GB), bare Coulomb blocks with eps = 4 and
+    r0 = 0.3 A, P and Sigma on the diagonal blocks (ndiag = nm), mixing 0.5.
+    A step is one iteration; the first (warm-up) starts from Sigma = 0."""
+    from quantum_transport_amd import hamiltonian as hm
+    from quantum_transport_amd import rgf
+
+    nm, nx, nen = 256, args.scba_slabs, 2048
+    w = hm.synthetic_wannier(nm, args.seed)
+    dev = hm.device_from_wannier(w, nx, 1, [(0.0, 0.0)])
+    Vd, V1 = w.coulomb_blocks(0.0, 0.0, 0.0, 4.0, 0.3)
+    Vii = np.repeat(np.asarray(Vd)[:, :, None], nx, axis=2)
+    V1i = np.repeat(np.asarray(V1)[:, :, None], nx, axis=2)
+    E = hm.energy_grid(-4.0, 4.0, nen)
+    rgf.init(0)
+    H = rgf.Hamiltonian.from_device(dev)
+    st = rgf.GWState(nm, nx, nen, Vii, V1i)
+    mu, T = [0.0, -0.1], [300.0, 300.0]
+    for _ in range(args.warmup):
+        rgf.gw_scba_step(H, st, E, mu, T, nm, 2.0, 0.5)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tr, tre, status = rgf.gw_scba_step(H, st, E, mu, T, nm, 2.0, 0.5)
+    el = (time.perf_counter() - t0) / max(1, args.steps)
+    st.free()
+    H.close()
+    return {"metric": "GW-SCBA iterations: energy x slab points per second (G, P, W, Sigma, mixing)",
+            "value": round(nen * nx / el, 1), "unit": "energy-slab points/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el, 1),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded SplitMix64 blocks, bare Coulomb on the orbital centres)",
+            "config": {"workload": f"GW-SCBA iteration, nm=256, {nx} slabs, 2048 E in [-4,4] eV, 1 kz, "
+                                   "ndiag=nm, eps=4, r0=0.3 A, mixing 0.5",
+                       "nm": nm, "slabs": nx, "energies": nen},
+            "iteration_s_per_slab": round(el / nx, 3),
+            "status": int(status), "current_tr_sum": float(np.sum(tr)), "cpu_baseline": None}
+
+
 def run_copy(args) -> dict:
     """Device-to-device copies of 256 MiB (torch's copy kernel): the plain
     copy of known size against which FETCH_SIZE / WRITE_SIZE are scaled, and
@@ -673,15 +716,17 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--workload", choices=("rgf", "uneven", "leads", "warmstart", "gw", "corrconv", "ephoton",
-                                               "vendor-zgemm", "copy"),
+                                               "vendor-zgemm", "copy", "scba"),
                     default="rgf",
                     help="rgf: the headline RGF sweep; uneven: 80 slabs of 128..768 orbitals; "
                          "leads: Sancho-Rubio surface Green's functions at nb=256, eta=1e-6; "
                          "warmstart: the learned lead warm start against Sancho-Rubio at that shape; "
                          "gw: the GW energy-axis stage; corrconv: batched corr1d2/conv1d2; "
                          "vendor-zgemm: rocBLAS batched ZGEMM at the sweep's shape, for comparison; "
-                         "copy: 256 MiB device-to-device copies, the HBM counter calibration")
+                         "copy: 256 MiB device-to-device copies, the HBM counter calibration; "
+                         "scba: one GW-SCBA iteration on the device at nm=256, 2048 E")
     ap.add_argument("--gw-slabs", type=int, default=2)
+    ap.add_argument("--scba-slabs", type=int, default=6, help="scba: slabs of the device")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--warm-points", type=int, default=1024, help="warmstart: evaluated points")
     ap.add_argument("--warm-epochs", type=int, default=30, help="warmstart: training epochs")
@@ -708,6 +753,9 @@ def main() -> int:
         return 0
     if args.workload == "copy":
         print(json.dumps(run_copy(args)), flush=True)
+        return 0
+    if args.workload == "scba":
+        print(json.dumps(run_scba(args)), flush=True)
         return 0
 
     from quantum_transport_amd import hamiltonian as hm

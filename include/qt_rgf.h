@@ -362,6 +362,26 @@ int qt_gw_wsolve_batched(int32_t nm, int32_t nx, int32_t nw, const void* Vii, co
                          const void* PR, const void* PL, const void* PG, void* WR, void* WL, void* WG,
                          int32_t flags);
 
+/* One GW-SCBA iteration with every array on the device: the loop body of
+ * gw_ephoton_3d_ijs (gw_mod.f90:372-749) for one k point and nx slabs of equal
+ * size nm (h, no overlap):
+ *   G^R, G^<, G^> of the nen energies En (host, equally spaced) by the
+ *     green_RGF_RS recursion (gw_mod.f90:1131-1345; p: variant QT_RGF_GW, the
+ *     contacts' mu / T and the lead options, as for qt_rgf_solve) with
+ *     Sigma^R, Sigma^< of S as the scattering self-energies;
+ *   P from G (qt_gw_polarization, ndiag, spindeg), W from P and the Coulomb
+ *     blocks Vii, V1i (qt_gw_wsolve_batched; device, (nm, nm, nx)),
+ *     Sigma_new from G and W (qt_gw_selfenergy), and
+ *     S <- S + alpha (Sigma_new - S) (qt_gw_mix).
+ * S[3], G[3], P[3]: element-major (nm*nm, nx, 1, nen) complex128 device
+ * arrays (R, <, >): S read and updated, G and P written.  tr, tre (host, nen
+ * each, may be NULL): the contact currents of the G pass.  Needs device
+ * scratch of six such arrays.  QT_ERR_SINGULAR when a block inversion met a
+ * zero pivot (the iteration still completes). */
+int qt_gw_scba_step(qt_ham* h, const qt_rgf_params* p, int32_t nen, const double* En, int32_t ndiag,
+                    double spindeg, double alpha, const void* Vii, const void* V1i, void* const* S,
+                    void* const* G, void* const* P, double* tr, double* tre);
+
 /* e-photon self-energies in the monochromatic assumption, replaces
  * calc_sigma_ephoton_monochromatic_ext (gw_mod.f90:1075-1122):
  *   Sigma^<>(E) = f M_ix [G^<>(E - hw) + G^<>(E + hw)] M_ix,  f = pre_fact intensity / hw^2,

@@ -273,7 +273,41 @@ struct FullOut {
     void* const* GGi1 = nullptr;
     int nx = 0;
     int p0 = 0;  // first point of the batch
+    // device element-major (nm*nm, nx, 1, nen) destinations of G^R, G^<, G^>
+    // (point p0 + b at energy index p0 + b; members b < nb only)
+    double2* em[3] = {nullptr, nullptr, nullptr};
+    int nen = 0;
+    int nb = 0;
 };
+
+// out[q*out_sq + x*out_sx + p] = in[p*in_sp + x*in_sx + q], p < P, q < Q, x < X:
+// the transposes between element-major series and member-major blocks
+__global__ __launch_bounds__(256) void tr3_kernel(const double2* __restrict__ in, int64_t in_sp, int64_t in_sx,
+                                                  double2* __restrict__ out, int64_t out_sq, int64_t out_sx,
+                                                  int P, int Q) {
+    __shared__ double2 t[32][33];
+    const int q0 = blockIdx.x * 32, p0 = blockIdx.y * 32;
+    const double2* ib = in + (int64_t)blockIdx.z * in_sx;
+    double2* ob = out + (int64_t)blockIdx.z * out_sx;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int pp = p0 + k, qq = q0 + threadIdx.x;
+        if (pp < P && qq < Q) t[k][threadIdx.x] = ib[(int64_t)pp * in_sp + qq];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int qq = q0 + k, pp = p0 + threadIdx.x;
+        if (pp < P && qq < Q) ob[(int64_t)qq * out_sq + pp] = t[threadIdx.x][k];
+    }
+}
+
+static void transpose3(hipStream_t s, const double2* in, int64_t in_sp, int64_t in_sx, double2* out,
+                       int64_t out_sq, int64_t out_sx, int P, int Q, int X) {
+    if (P <= 0 || Q <= 0 || X <= 0) return;
+    if ((P + 31) / 32 > 65535 || X > 65535) throw hip_error("transpose3: grid too large", 1);
+    hipLaunchKernelGGL(tr3_kernel, dim3((Q + 31) / 32, (P + 31) / 32, X), dim3(32, 8), 0, s, in, in_sp, in_sx,
+                       out, out_sq, out_sx, P, Q);
+    QT_HIP(hipGetLastError());
+}
 
 static void d2h_real_blocks(Ctx& c, void* const* dst, int nx, int p0, int B, int s, int n,
                             const double* src, int64_t sb) {
@@ -358,7 +392,16 @@ struct Solve {
         const int n = h.nm[s];
         d2h_blocks(c, full.G_r, full.nx, full.p0, B, s, n, G, w.nn);
         d2h_blocks(c, full.G_lesser, full.nx, full.p0, B, s, n, L, w.nn);
-        if (full.G_greater) {
+        // element-major destinations: member b's block -> series (ij, s) at energy p0 + b
+        const int64_t sser = (int64_t)full.nx * full.nen;
+        auto scatter = [&](double2* em, const double2* src) {
+            if (em)
+                transpose3(c.stream, src, w.nn, 0, em + (int64_t)s * full.nen + full.p0, sser, 0, full.nb,
+                           n * n, 1);
+        };
+        scatter(full.em[0], G);
+        scatter(full.em[1], L);
+        if (full.G_greater || full.em[2]) {
             AntiArgs an;
             an.n = n;
             an.batch = B;
@@ -370,6 +413,7 @@ struct Solve {
             an.c1 = 1.0;
             antiherm(c.stream, an);  // G^> = G^< + (G - G^H)
             d2h_blocks(c, full.G_greater, full.nx, full.p0, B, s, n, w.T[12], w.nn);
+            scatter(full.em[2], w.T[12]);
         }
         if (jd) d2h_blocks(c, full.Jdens, full.nx, full.p0, B, s, n, jd, w.nn);
     }
@@ -2105,6 +2149,215 @@ int qt_gw_wsolve_batched(int32_t nm, int32_t nx, int32_t nw, const void* Vii, co
         }
         return st;
     } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+// ---------------------------------------------------------------- GW-SCBA iteration
+// One iteration of the loop of gw_ephoton_3d_ijs (gw_mod.f90:372-749) with
+// every array on the device: G by the green_RGF_RS recursion with the
+// current Sigma (gathered per batch from the element-major arrays, G
+// scattered back into them), then P, W, Sigma_new and the mixing, each
+// stage the kernels of its own entry point.
+int qt_gw_scba_step(qt_ham* h, const qt_rgf_params* p, int32_t nen, const double* En, int32_t ndiag,
+                    double spindeg, double alpha, const void* Vii, const void* V1i, void* const* S,
+                    void* const* G, void* const* P, double* tr, double* tre) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (!h || !p || !En || !Vii || !S || !G || !P || nen < 2 || ndiag < 0) return QT_ERR_INVALID;
+    if (p->variant != QT_RGF_GW || p->precision != QT_PREC_FP64) return QT_ERR_UNSUPPORTED;
+    for (int i = 0; i < 3; ++i)
+        if (!S[i] || !G[i] || !P[i]) return QT_ERR_INVALID;
+    if (h->nk != 1 || h->S) return QT_ERR_UNSUPPORTED;
+    const int nm = h->nm[0], nx = h->nx;
+    for (int s = 0; s < nx; ++s)
+        if (h->nm[s] != nm) return QT_ERR_UNSUPPORTED;
+    if (nx > 1 && !V1i) return QT_ERR_INVALID;
+    if (nen > 2048 || !gw_shape_ok(nen, 1, 1, nx, nm, ndiag)) return QT_ERR_UNSUPPORTED;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    void* tmp = nullptr;
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        const int64_t nn = (int64_t)nm * nm;
+        const int64_t total = nn * nx * nen;  // samples of one array
+        double2* const* Sd = (double2* const*)S;
+        double2* const* Gd = (double2* const*)G;
+        double2* const* Pd = (double2* const*)P;
+        int status = QT_OK;
+
+        // ---- G of every energy (qt_rgf_solve's batch loop, QT_RGF_GW with sigma)
+        {
+            const int B = choose_batch(*h, nen, true);
+            c.arena.reserve(work_bytes(*h, B, true));
+            Work w;
+            work_carve(c.arena, w, *h, B, true);
+            qt_timing tm{};
+            std::vector<double> hz(2 * B), hf(5 * B), htr(B), htre(B);
+            std::vector<int> hk(2 * B, 0), hinfo(B);
+            std::vector<int64_t> doff(nx);
+            for (int s = 0; s < nx; ++s) doff[s] = (int64_t)s * nm;
+            for (int p0 = 0; p0 < nen; p0 += B) {
+                const int nb = std::min(B, nen - p0);
+                for (int b = 0; b < B; ++b) {
+                    const int ip = p0 + std::min(b, nb - 1);
+                    hz[b] = hz[B + b] = En[ip];
+                    // gw_mod.f90:1185,1232,1333, as in qt_rgf_solve
+                    const double fF = ferm((En[ip] - p->mul) / (BOLTZ * p->templ));
+                    const double fFc = ferm((En[ip] - p->mul) / (BOLTZ * p->tempr));
+                    const double fL = ferm((En[ip] - p->mur) / (BOLTZ * p->templ));
+                    hf[b] = fF;
+                    hf[B + b] = fL;
+                    hf[2 * B + b] = -fF;
+                    hf[3 * B + b] = -fL;
+                    hf[4 * B + b] = fFc;
+                }
+                QT_HIP(hipMemcpyAsync(w.zre, hz.data(), sizeof(double) * 2 * B, hipMemcpyHostToDevice, c.stream));
+                QT_HIP(hipMemcpyAsync(w.kidx, hk.data(), sizeof(int) * 2 * B, hipMemcpyHostToDevice, c.stream));
+                QT_HIP(hipMemcpyAsync(w.fF, hf.data(), sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+                QT_HIP(hipMemcpyAsync(w.fL, hf.data() + B, sizeof(double) * B, hipMemcpyHostToDevice, c.stream));
+                QT_HIP(hipMemcpyAsync(w.nfF, hf.data() + 2 * B, sizeof(double) * B, hipMemcpyHostToDevice,
+                                      c.stream));
+                QT_HIP(hipMemcpyAsync(w.nfL, hf.data() + 3 * B, sizeof(double) * B, hipMemcpyHostToDevice,
+                                      c.stream));
+                QT_HIP(hipMemcpyAsync(w.fFc, hf.data() + 4 * B, sizeof(double) * B, hipMemcpyHostToDevice,
+                                      c.stream));
+                fill_int(c.stream, w.infoacc, B, 0);
+                // H00 = Hii + Sigma^R, SLS = Sigma^< of each member's energy (no overlap):
+                // series (ij, s) of energies p0.. gathered into member blocks, the padding
+                // members repeating the last one
+                for (int s = 0; s < nx; ++s) {
+                    const int64_t off = (int64_t)s * nen + p0;
+                    transpose3(c.stream, Sd[0] + off, (int64_t)nx * nen, 0, w.T[0], w.nn, 0, (int)nn, nb, 1);
+                    transpose3(c.stream, Sd[1] + off, (int64_t)nx * nen, 0, w.T[1], w.nn, 0, (int)nn, nb, 1);
+                    for (int b = nb; b < B; ++b)
+                        for (int q = 0; q < 2; ++q)
+                            QT_HIP(hipMemcpyAsync(w.T[q] + b * w.nn, w.T[q] + (nb - 1) * w.nn,
+                                                  sizeof(double2) * nn, hipMemcpyDeviceToDevice, c.stream));
+                    double2* hr = w.H00 + h->hoff[s] * B;
+                    double2* hl = w.SLS + h->hoff[s] * B;
+                    copy_blocks(c.stream, nm, B, hl, nn, op_member(w.T[1], nm, w.nn), nullptr, 0, false);
+                    ShiftArgs sh;
+                    sh.n = nm;
+                    sh.batch = B;
+                    sh.out = hr;
+                    sh.sout = nn;
+                    sh.M1 = op_k(h->H + h->hoff[s], nm, h->hset);
+                    sh.c1 = 1.0;
+                    sh.M2 = op_member(w.T[0], nm, w.nn);
+                    sh.c2 = 1.0;
+                    sh.kidx = w.kidx;
+                    shifted(c.stream, sh);
+                }
+                Solve sv{c, *h, *p, w, B, true, (p->flags & QT_F_COUPLING_FIX) != 0, true, true};
+                sv.doff = doff;
+                sv.full.any = true;
+                sv.full.nx = nx;
+                sv.full.p0 = p0;
+                sv.full.nb = nb;
+                sv.full.nen = nen;
+                for (int i = 0; i < 3; ++i) sv.full.em[i] = Gd[i];
+                sv.run(tm);
+                QT_HIP(hipMemcpyAsync(htr.data(), w.tr, sizeof(double) * B, hipMemcpyDeviceToHost, c.stream));
+                QT_HIP(hipMemcpyAsync(htre.data(), w.tre, sizeof(double) * B, hipMemcpyDeviceToHost, c.stream));
+                QT_HIP(hipMemcpyAsync(hinfo.data(), w.infoacc, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+                QT_HIP(hipStreamSynchronize(c.stream));
+                for (int b = 0; b < nb; ++b) {
+                    if (tr) tr[p0 + b] = htr[b];
+                    if (tre) tre[p0 + b] = htre[b];
+                    if (hinfo[b] & QT_INFO_SINGULAR) status = QT_ERR_SINGULAR;
+                }
+            }
+        }
+
+        // ---- P, W, Sigma_new, mixing.  Scratch: A (3 arrays: P as blocks, then W
+        // element-major), Bm (3: W as blocks, then Sigma_new), the element lists, I.
+        const std::vector<int> el = band_elements(nm, ndiag, false);
+        const std::vector<int> pr = band_elements(nm, ndiag, true);
+        const size_t abytes = sizeof(double2) * (size_t)total;
+        const size_t lists = sizeof(int) * (el.size() + pr.size()) + sizeof(double2) * (size_t)nn + 1024;
+        QT_HIP(hipMalloc(&tmp, 6 * abytes + lists));
+        double2* A[3];
+        double2* Bm[3];
+        for (int i = 0; i < 3; ++i) {
+            A[i] = (double2*)tmp + (size_t)i * total;
+            Bm[i] = (double2*)tmp + (size_t)(3 + i) * total;
+        }
+        double2* dI = (double2*)tmp + (size_t)6 * total;
+        int* del = (int*)(dI + nn);
+        int* dpr = del + el.size();
+        QT_HIP(hipMemcpyAsync(del, el.data(), sizeof(int) * el.size(), hipMemcpyHostToDevice, c.stream));
+        QT_HIP(hipMemcpyAsync(dpr, pr.data(), sizeof(int) * pr.size(), hipMemcpyHostToDevice, c.stream));
+        {
+            std::vector<double2> id((size_t)nn, make_double2(0.0, 0.0));
+            for (int d = 0; d < nm; ++d) id[(size_t)d * nm + d] = make_double2(1.0, 0.0);
+            QT_HIP(hipMemcpyAsync(dI, id.data(), sizeof(double2) * nn, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipStreamSynchronize(c.stream));
+        }
+        const double dEn = En[1] - En[0];
+        GwArgs a;
+        a.nen = nen;
+        a.nky = 1;
+        a.nkz = 1;
+        a.nx = nx;
+        a.nm = nm;
+        a.dEn = dEn;
+        a.spindeg = spindeg;
+        a.Gr = Gd[0];
+        a.Gl = Gd[1];
+        a.Gg = Gd[2];
+        a.Pr = Pd[0];
+        a.Pl = Pd[1];
+        a.Pg = Pd[2];
+        for (int i = 0; i < 3; ++i) QT_HIP(hipMemsetAsync(Pd[i], 0, abytes, c.stream));
+        gw_polarization(c.stream, a, del, (int)el.size());
+        // P element-major (ij, x, e) -> blocks (e, x, ij), the W solve's member-major layout
+        for (int i = 0; i < 3; ++i)
+            transpose3(c.stream, Pd[i], (int64_t)nx * nen, nen, A[i], (int64_t)nx * nn, nn, (int)nn, nen, nx);
+        {
+            size_t fr = 0, tot = 0;
+            QT_HIP(hipMemGetInfo(&fr, &tot));
+            const size_t per = WSolve::bytes(nm, nx, 1);
+            int Bw = (int)std::max<int64_t>(1, std::min<int64_t>(nen, (int64_t)(0.7 * (double)fr) / (int64_t)per));
+            Bw = std::min(Bw, 256);
+            c.arena.reserve(WSolve::bytes(nm, nx, Bw) + 16384);
+            c.arena.reset();
+            for (int w0 = 0; w0 < nen; w0 += Bw) {
+                const int cnt = std::min(Bw, nen - w0);
+                WSolve ws(c, nm, nx, cnt, (const double2*)Vii, (const double2*)V1i, dI);
+                const size_t mark = c.arena.used;
+                ws.carve(c.arena);
+                const size_t off = (size_t)w0 * nn * nx;
+                const double2* px[2] = {A[1] + off, A[2] + off};
+                double2* wx[2] = {Bm[1] + off, Bm[2] + off};
+                ws.run(A[0] + off, px, Bm[0] + off, wx);
+                QT_HIP(hipStreamSynchronize(c.stream));
+                std::vector<int> info(cnt);
+                QT_HIP(hipMemcpy(info.data(), ws.w.infoacc, sizeof(int) * cnt, hipMemcpyDeviceToHost));
+                for (int v : info)
+                    if (v) status = QT_ERR_SINGULAR;
+                c.arena.used = mark;
+            }
+        }
+        // W blocks (e, x, ij) -> element-major (ij, x, e) into A
+        for (int i = 0; i < 3; ++i)
+            transpose3(c.stream, Bm[i], (int64_t)nx * nn, nn, A[i], (int64_t)nx * nen, nen, nen, (int)nn, nx);
+        a.Wr = A[0];
+        a.Wl = A[1];
+        a.Wg = A[2];
+        a.Sr = Bm[0];
+        a.Sl = Bm[1];
+        a.Sg = Bm[2];
+        for (int i = 0; i < 3; ++i) QT_HIP(hipMemsetAsync(Bm[i], 0, abytes, c.stream));
+        gw_selfenergy(c.stream, a, dpr, (int)pr.size());
+        for (int i = 0; i < 3; ++i) gw_mix(c.stream, total, alpha, Sd[i], Bm[i]);
+        QT_HIP(hipStreamSynchronize(c.stream));
+        QT_HIP(hipFree(tmp));
+        return status;
+    } catch (const hip_error& e) {
+        if (tmp) {
+            (void)hipStreamSynchronize(c.stream);
+            (void)hipFree(tmp);
+        }
         return status_of(e);
     }
 }

@@ -140,6 +140,42 @@ class Wannier:
                                 self.Hr[:, :, x1 - self.xmin, j - self.ymin, l - self.zmin] * ph)
         return Hii, H1i
 
+    def bare_coulomb(self, a1: int, a2: int, a3: int, eps: float, r0: float, ldiag: bool) -> np.ndarray:
+        """V_ij between orbital i of cell (a1, a2, a3) and orbital j of cell 0
+        (wannierHam3d.f90, bare_coulomb): e / (4 pi eps0 eps |r|) tanh(|r| / r0)
+        in eV with |r| in Angstrom, r = a1 alpha + a2 beta + a3 gamma + c_i -
+        c_j; at r = 0 the self-interaction e / (4 pi eps0 eps r0) when ldiag,
+        else 0 (the constants e = 1.6e-19 C, eps0 = 8.85e-12 F/m as there)."""
+        alpha, beta, gamm = self.cell[:, 0], self.cell[:, 1], self.cell[:, 2]
+        pref = 1.6e-19 / (4.0 * 3.14159265359 * 8.85e-12 * eps * 1e-10)
+        r = (a1 * alpha + a2 * beta + a3 * gamm)[:, None, None] + self.centres[:, :, None] - self.centres[:, None, :]
+        d = np.linalg.norm(r, axis=0)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            V = np.where(d > 0, pref / d * np.tanh(d / r0), pref / r0 if ldiag else 0.0)
+        return V
+
+    def coulomb_blocks(self, kx: float, ky: float, kz: float, eps: float, r0: float, ns: int = 1,
+                       ldiag: bool = True) -> Tuple[np.ndarray, np.ndarray]:
+        """Diagonal and sub-diagonal blocks V(I,I), V(I+1,I) of the bare
+        Coulomb operator (w90_bare_coulomb_blocks, wannierHam3d.f90:363-390):
+        block (i,k) sums bare_coulomb(i-k, j, l) (and i-k+ns for the
+        coupling) over the transverse cells with the phase of mat_def."""
+        alpha, beta, gamm, xhat, yhat, zhat = self._axes()
+        nb = self.nb
+        kv = kx * xhat + ky * yhat + kz * zhat
+        Vii = np.zeros((nb * ns, nb * ns), dtype=np.complex128, order="F")
+        V1i = np.zeros((nb * ns, nb * ns), dtype=np.complex128, order="F")
+        for i in range(ns):
+            for k in range(ns):
+                for j in range(self.ymin, self.ymax + 1):
+                    for l in range(self.zmin, self.zmax + 1):
+                        ph = np.exp(-1j * np.dot((i - k) * alpha + j * beta + l * gamm, kv))
+                        Vii[i * nb:(i + 1) * nb, k * nb:(k + 1) * nb] += self.bare_coulomb(i - k, j, l, eps, r0,
+                                                                                            ldiag) * ph
+                        V1i[i * nb:(i + 1) * nb, k * nb:(k + 1) * nb] += self.bare_coulomb(i - k + ns, j, l, eps,
+                                                                                            r0, ldiag) * ph
+        return Vii, V1i
+
 
 def band_edges(w: Wannier, chunk: int = 512) -> dict:
     """Band-structure scan for the band edges, wannierHam3d.f90:153-201: the bulk

@@ -43,7 +43,7 @@ EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham
             "qt_conv1d2_batched", "qt_gw_polarization", "qt_gw_selfenergy", "qt_gw_mix",
             "qt_gw_wsolve_batched", "qt_gw_sigma_ephoton", "qt_invert_batched_noalloc", "qt_lead_create",
             "qt_lead_destroy", "qt_lead_train", "qt_lead_surface", "qt_lead_param_count", "qt_lead_get_params",
-            "qt_lead_set_params", "qt_lead_eval", "qt_graph_slice"]
+            "qt_lead_set_params", "qt_lead_eval", "qt_graph_slice", "qt_gw_scba_step"]
 
 
 class QTError(RuntimeError):
@@ -154,6 +154,9 @@ def lib() -> C.CDLL:
             + [C.c_int32]
         L.qt_graph_slice.argtypes = [C.c_int32, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                      C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]
+        L.qt_gw_scba_step.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_void_p, C.c_int32,
+                                      C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -674,6 +677,56 @@ def green_rgf_rs(TEMP, nm: int, nx: int, E: float, mu, Hii, H1i, sigma_lesser_ph
     out = {key: np.stack(v, axis=2) for key, v in res.items()}
     out.update(tr=float(tr[0]), tre=float(tre[0]), info=int(info[0]))
     return out
+
+
+class GWState:
+    """Device-resident arrays of a GW-SCBA run (qt_gw_scba_step): S (Sigma^R,
+    Sigma^<, Sigma^>, updated by every step), G and P (written), element-major
+    (nm*nm, nx, 1, nen) complex128, and the Coulomb blocks Vii, V1i."""
+
+    def __init__(self, nm: int, nx: int, nen: int, Vii, V1i, S0=None):
+        init()
+        self.nm, self.nx, self.nen = nm, nx, nen
+        self.nbytes = 16 * nm * nm * nx * nen
+        self.S = [DeviceBuffer(self.nbytes) for _ in range(3)]
+        self.G = [DeviceBuffer(self.nbytes) for _ in range(3)]
+        self.P = [DeviceBuffer(self.nbytes) for _ in range(3)]
+        V = [np.asfortranarray(v, dtype=np.complex128) for v in (Vii, V1i)]
+        self.V = [DeviceBuffer(v.nbytes) for v in V]
+        for b, v in zip(self.V, V):
+            check(lib().qt_memcpy_h2d(b.ptr, v.ctypes.data_as(C.c_void_p), C.c_size_t(v.nbytes)))
+        for i, b in enumerate(self.S):
+            a = np.zeros(nm * nm * nx * nen, np.complex128) if S0 is None else \
+                np.ascontiguousarray(S0[i], dtype=np.complex128).ravel()
+            check(lib().qt_memcpy_h2d(b.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes)))
+
+    def host(self, which: str):
+        shape = (self.nm * self.nm, self.nx, 1, self.nen)
+        return [b.to_host(np.complex128, shape) for b in getattr(self, which)]
+
+    def free(self):
+        for b in self.S + self.G + self.P + self.V:
+            b.free()
+
+
+def gw_scba_step(H: "Hamiltonian", state: GWState, En, mu, TEMP, ndiag: int, spindeg: float = 2.0,
+                 alpha: float = 0.5, lead_eta: float = 1e-3, lead_nmax: int = 100):
+    """One GW-SCBA iteration on the device (qt_gw_scba_step): G by the
+    green_RGF_RS recursion with the state's Sigma, P, W, Sigma_new, mixing.
+    Returns (tr, tre, status)."""
+    En = np.ascontiguousarray(En, dtype=np.float64)
+    p = make_params("gw", mu[0], mu[1], TEMP[0], TEMP[1], lead_eta=lead_eta, lead_nmax=lead_nmax,
+                    lead_reset=True)
+    tr = np.zeros(len(En))
+    tre = np.zeros(len(En))
+    ptrs = lambda bufs: (C.c_void_p * 3)(*(b.ptr.value for b in bufs))
+    S, G, P = ptrs(state.S), ptrs(state.G), ptrs(state.P)
+    st = lib().qt_gw_scba_step(H.handle, C.byref(p), len(En), En.ctypes.data_as(C.c_void_p), ndiag, spindeg,
+                               alpha, state.V[0].ptr, state.V[1].ptr, S, G, P, tr.ctypes.data_as(C.c_void_p),
+                               tre.ctypes.data_as(C.c_void_p))
+    if st != QT_ERR_SINGULAR:
+        check(st)
+    return tr, tre, st
 
 
 # ----------------------------------------------------------------- learned lead warm start

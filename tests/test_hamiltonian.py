@@ -160,3 +160,23 @@ def test_band_edges_golden_cnt_gap_is_positive():
     assert r["nk"] == (24, 1, 1)
     assert 0.1 < r["Eg"] < 2.0
     assert np.all(np.diff(r["EN"], axis=0) >= -1e-12)
+
+
+def test_bare_coulomb_closed_form():
+    """Two orbitals 3 A apart along x in a cubic 6 A cell: V between them,
+    their self-interaction, the next cell's pair at 3 A and 9 A (the
+    coupling block), against e / (4 pi eps0 eps r) tanh(r / r0) by hand."""
+    from quantum_transport_amd.hamiltonian import Wannier
+
+    cell = np.diag([6.0, 6.0, 6.0])
+    Hr = np.zeros((2, 2, 3, 1, 1), complex)
+    centres = np.array([[0.0, 3.0], [0.0, 0.0], [0.0, 0.0]])
+    w = Wannier(2, 1, cell, Hr, -1, 0, 0, centres)
+    eps, r0 = 4.0, 0.3
+    k = 1.6e-19 / (4 * 3.14159265359 * 8.85e-12 * eps * 1e-10)  # eV A
+    Vii, V1i = w.coulomb_blocks(0.0, 0.0, 0.0, eps, r0)
+    assert np.allclose(Vii, [[k / r0, k / 3 * np.tanh(10)], [k / 3 * np.tanh(10), k / r0]], rtol=1e-13)
+    # V(I+1, I): orbital i in cell +1 (x + 6 A) against orbital j in cell 0
+    want = np.array([[k / 6 * np.tanh(20), k / 3 * np.tanh(10)], [k / 9 * np.tanh(30), k / 6 * np.tanh(20)]])
+    assert np.allclose(V1i, want, rtol=1e-13)
+    assert np.allclose(w.bare_coulomb(0, 0, 0, eps, r0, False).diagonal(), 0.0)
