@@ -159,9 +159,10 @@ def run_gw(args) -> dict:
         torch.cuda.synchronize()
         times[name] = (time.perf_counter() - t0) / args.steps
     series = nm * nm * nx
-    # algorithmic HBM bytes: P reads 5 series and writes 3 per element and slab,
-    # Sigma reads 6 and writes 3 (16 B per complex sample)
-    bytes_p = series * nen * 16 * 8
+    # algorithmic HBM bytes: P reads G^R, G^<, G^> and writes P^R, P^<, P^>
+    # once per element and slab (the element pair ij/ji shares its reads),
+    # Sigma reads 6 series and writes 3 (16 B per complex sample)
+    bytes_p = series * nen * 16 * 6
     bytes_s = series * nen * 16 * 9
     bw_p = bytes_p / times["polarization"] / 1e9
     bw_s = bytes_s / times["selfenergy"] / 1e9
@@ -177,7 +178,7 @@ def run_gw(args) -> dict:
                                f"2048 energies, 1 kz", "nm": nm, "slabs": nx, "energies": nen},
         "roofline": {"bound": "hbm", "achieved": round(bw_p, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(bw_p / 8000.0, 4), "traffic": None,
-                     "kernel": "gw_pol_kernel (P assembly)"},
+                     "kernel": "gw_pol_pair_kernel (P assembly)"},
         "stages": {"polarization_ms": round(1e3 * times["polarization"], 2),
                    "selfenergy_ms": round(1e3 * times["selfenergy"], 2),
                    "selfenergy_GBps": round(bw_s, 1)},

@@ -8,8 +8,8 @@
 //
 // The reference evaluates every correlation / convolution with its own
 // three DFTs.  Here the k-sums are taken in the frequency domain: one
-// workgroup owns one element (P) or one element pair ij/ji (Sigma) at one
-// slab, transforms each input series once per term with the LDS FFT of
+// workgroup owns one element pair ij/ji (Sigma; P at one k point) or one
+// element (P over several k points) at one slab, transforms each input series once per term with the LDS FFT of
 // qt_fft.h (length L = power of two >= 2 nen - 1, which gives the same
 // linear products), accumulates the products over k in registers and
 // inverse-transforms once per output series:
@@ -248,6 +248,110 @@ __global__ __launch_bounds__(NT) void gw_pol_kernel(GwDims d, const int* __restr
     }
 }
 
+// -x* : the spectrum at -k of i times a real series, from its value x at k
+__device__ __forceinline__ double2 at_minus(double2 x) { return make_double2(-x.x, x.y); }
+
+// P of one element pair (ij, ji), row <= col, at one k point (nk == 1, so
+// k' = k): the six series of the pair in three packed transforms,
+//   (G^<_ij, G^>_ij), (G^<_ji, G^>_ji), (G^R_ij, G^R_ji),
+// and four inverse transforms (P^< + i P^> and P^R of each element), where
+// the element kernel above takes six transforms per element.  The spectrum
+// of i y at -k is -conj of its value at k (y real), bit for bit the value
+// the unpacking gives at -k, so a thread needs only its own samples k.
+// A diagonal pair (ij == ji) takes (G^<, G^>) and (G^R, 0) and two inverses.
+template <int L, int NT>
+__global__ __launch_bounds__(NT) void gw_pol_pair_kernel(GwDims d, const int* __restrict__ pairs, int npair,
+                                                         const double2* __restrict__ Gr,
+                                                         const double2* __restrict__ Gl,
+                                                         const double2* __restrict__ Gg, double2* Pr,
+                                                         double2* Pl, double2* Pg, double2 dE,
+                                                         const double2* __restrict__ tw) {
+    using S = Spec<L, NT>;
+    constexpr int PER = S::PER;
+    extern __shared__ double2 s[];
+    const int w = blockIdx.x;
+    if (w >= npair * d.nx) return;
+    tw = stage_twiddles<L, NT>(s, tw);
+    const int p = w / d.nx, ix = w % d.nx;
+    const int ij = pairs[p];
+    const int row = ij % d.nm, col = ij / d.nm;
+    const int ji = col + row * d.nm;
+    const bool diag = ij == ji;
+    const int n = d.nen;
+    // P^< (real part) and P^> (imaginary part) of c from one inverse transform;
+    // sample m is lag m, i.e. output i = m + n/2 (mod L)
+    auto lesser_greater = [&](const S& c, int e) {
+        const double inv = 1.0 / (double)L;
+        const int h = n / 2;
+        double2* zl = series(Pl, d, e, ix, 0);
+        double2* zg = series(Pg, d, e, ix, 0);
+        fft::transform_io<L, NT, true, RADIX>(
+            s, tw, [&](int, int q) { return c.v[q]; },
+            [&](int m, int, double2 v) {
+                const int i = (m + h) & (L - 1);
+                if (i < n) {
+                    zl[i] = make_double2(dE.x * v.x * inv, dE.y * v.x * inv);
+                    zg[i] = make_double2(dE.x * v.y * inv, dE.y * v.y * inv);
+                }
+            });
+    };
+    // spectra at k of i Im G^<_ij, i Im G^<_ji
+    double2 lij[PER], lji[PER];
+    {
+        S cij, cji;  // P^< + i P^> of ij and ji
+        double2 gij[PER];
+        fwd_pack<L, NT>(s, tw, series(Gl, d, ij, ix, 0), series(Gg, d, ij, ix, 0), n);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            if (!S::ok(threadIdx.x + q * NT)) continue;
+            double2 zk, zm;
+            zpair<L, NT>(s, q, zk, zm);
+            lij[q] = spec1(zk, zm);
+            gij[q] = spec2(zk, zm);
+        }
+        if (!diag) fwd_pack<L, NT>(s, tw, series(Gl, d, ji, ix, 0), series(Gg, d, ji, ix, 0), n);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            if (!S::ok(threadIdx.x + q * NT)) continue;
+            double2 l = lij[q], g = gij[q];
+            if (!diag) {
+                double2 zk, zm;
+                zpair<L, NT>(s, q, zk, zm);
+                l = spec1(zk, zm);
+                g = spec2(zk, zm);
+            }
+            lji[q] = l;
+            // P^<_ij ~ G^<_ij(k) G^>_ji(-k), P^>_ij ~ G^>_ij(k) G^<_ji(-k)
+            double2 a = cmul(lij[q], at_minus(g)), b = cmul(gij[q], at_minus(l));
+            cij.v[q] = make_double2(a.x - b.y, a.y + b.x);
+            // P^<_ji ~ G^<_ji(k) G^>_ij(-k), P^>_ji ~ G^>_ji(k) G^<_ij(-k)
+            a = cmul(l, at_minus(gij[q]));
+            b = cmul(g, at_minus(lij[q]));
+            cji.v[q] = make_double2(a.x - b.y, a.y + b.x);
+        }
+        lesser_greater(cij, ij);
+        if (!diag) lesser_greater(cji, ji);
+    }
+    // P^R_ij ~ -G^<_ij(k) G^R_ij(-k) + G^R_ij(k) G^<_ji(-k), and ji likewise
+    S rij, rji;
+    fwd_pack<L, NT>(s, tw, series(Gr, d, ij, ix, 0), diag ? nullptr : series(Gr, d, ji, ix, 0), n);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        if (!S::ok(threadIdx.x + q * NT)) continue;
+        double2 zk, zm;
+        zpair<L, NT>(s, q, zk, zm);
+        const double2 r1 = spec1(zk, zm);
+        double2 t1 = cmul(lij[q], at_minus(r1)), t2 = cmul(r1, at_minus(lji[q]));
+        rij.v[q] = make_double2(t2.x - t1.x, t2.y - t1.y);
+        const double2 r2 = spec2(zk, zm);
+        t1 = cmul(lji[q], at_minus(r2));
+        t2 = cmul(r2, at_minus(lij[q]));
+        rji.v[q] = make_double2(t2.x - t1.x, t2.y - t1.y);
+    }
+    finish<L, NT>(s, tw, rij, dE, n, true, series(Pr, d, ij, ix, 0));
+    if (!diag) finish<L, NT>(s, tw, rji, dE, n, true, series(Pr, d, ji, ix, 0));
+}
+
 // ---------------------------------------------------------------- Sigma
 // Sigma of element e at slab ix and point k, the q-sums in two passes:
 //   1. i Im G^< and i Im G^R at k' share one packed transform; with the
@@ -445,8 +549,10 @@ struct GwCfg {
     static constexpr int NT = L / RADIX >= QT_GW_NTMAX ? QT_GW_NTMAX : (L / RADIX >= 64 ? L / RADIX : 64);
 };
 
+// elems: every in-band element (any nk), or, with pairs == true, the
+// in-band elements with row <= col (nk == 1)
 template <int L>
-void launch_pol(hipStream_t st, const GwDims& d, const int* elems, int nelem, const double2* Gr,
+void launch_pol(hipStream_t st, const GwDims& d, const int* elems, int nelem, bool pairs, const double2* Gr,
                 const double2* Gl, const double2* Gg, double2* Pr, double2* Pl, double2* Pg, double2 dE) {
     constexpr int NT = GwCfg<L>::NT;
     const size_t lds = sizeof(double2) * gw_lds_elems<L>();
@@ -454,12 +560,20 @@ void launch_pol(hipStream_t st, const GwDims& d, const int* elems, int nelem, co
     if (!attr) {
         QT_HIP(hipFuncSetAttribute((const void*)gw_pol_kernel<L, NT>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        QT_HIP(hipFuncSetAttribute((const void*)gw_pol_pair_kernel<L, NT>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         attr = true;
     }
     const int64_t blocks = (int64_t)nelem * d.nx;
     if (blocks > 0x7fffffff) throw hip_error("gw polarization: grid too large", 1);
-    hipLaunchKernelGGL((gw_pol_kernel<L, NT>), dim3((unsigned)blocks), dim3(NT), lds, st, d, elems, nelem,
-                       Gr, Gl, Gg, Pr, Pl, Pg, dE, fft_twiddles(L));
+    if (pairs) {
+        if (d.nk != 1) throw hip_error("gw polarization: element pairs need nk == 1", 1);
+        hipLaunchKernelGGL((gw_pol_pair_kernel<L, NT>), dim3((unsigned)blocks), dim3(NT), lds, st, d, elems,
+                           nelem, Gr, Gl, Gg, Pr, Pl, Pg, dE, fft_twiddles(L));
+    } else {
+        hipLaunchKernelGGL((gw_pol_kernel<L, NT>), dim3((unsigned)blocks), dim3(NT), lds, st, d, elems, nelem,
+                           Gr, Gl, Gg, Pr, Pl, Pg, dE, fft_twiddles(L));
+    }
     QT_HIP(hipGetLastError());
 }
 
@@ -490,12 +604,16 @@ int fft_len(int nen) {
 
 }  // namespace
 
-void gw_polarization(hipStream_t st, const GwArgs& a, const int* elems, int nelem) {
+void gw_polarization(hipStream_t st, const GwArgs& a, const int* elems, int nelem, const int* pairs, int npair) {
     GwDims d{a.nen, a.nky, a.nkz, a.nky * a.nkz, a.nx, a.nm};
     // dE = -i dEn/(2 pi) spindeg/nk  (gw_mod.f90:516)
     const double2 dE = make_double2(0.0, -a.dEn / 2.0 / kPi * a.spindeg / (double)d.nk);
     const int L = fft_len(a.nen);
-#define QT_POL(LL) launch_pol<LL>(st, d, elems, nelem, a.Gr, a.Gl, a.Gg, a.Pr, a.Pl, a.Pg, dE)
+    // one k point: element pairs share their transforms
+    const bool use_pairs = d.nk == 1 && pairs != nullptr;
+    const int* list = use_pairs ? pairs : elems;
+    const int cnt = use_pairs ? npair : nelem;
+#define QT_POL(LL) launch_pol<LL>(st, d, list, cnt, use_pairs, a.Gr, a.Gl, a.Gg, a.Pr, a.Pl, a.Pg, dE)
     switch (L) {
         case 2: QT_POL(2); break;
         case 4: QT_POL(4); break;

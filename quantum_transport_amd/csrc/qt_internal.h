@@ -181,7 +181,8 @@ const double2* fft_twiddles(int L);
 // GW assembly on element-major arrays (qt_gw.hip): series (ij, ix, ik) at
 // ((ij*nx + ix)*nk + ik)*nen, ij = row + col*nm.  elems: in-band ij to
 // assemble (polarisation); pairs: in-band ij with row <= col (self-energy,
-// which writes both ij and ji).  Out-of-band outputs are left untouched.
+// which writes both ij and ji; polarisation takes them instead of elems at
+// nk == 1 when given).  Out-of-band outputs are left untouched.
 struct GwArgs {
     int nen = 0, nky = 1, nkz = 1, nx = 0, nm = 0;
     double dEn = 0.0, spindeg = 1.0;
@@ -190,7 +191,8 @@ struct GwArgs {
     double2 *Pr = nullptr, *Pl = nullptr, *Pg = nullptr;
     double2 *Sr = nullptr, *Sl = nullptr, *Sg = nullptr;
 };
-void gw_polarization(hipStream_t s, const GwArgs& a, const int* elems, int nelem);
+void gw_polarization(hipStream_t s, const GwArgs& a, const int* elems, int nelem, const int* pairs = nullptr,
+                     int npair = 0);
 void gw_selfenergy(hipStream_t s, const GwArgs& a, const int* pairs, int npair);
 void gw_mix(hipStream_t s, int64_t n, double alpha, double2* S, const double2* Snew);
 // S[t] = G[t] + G[t + shift] for t < n (the two photon-shifted energies)

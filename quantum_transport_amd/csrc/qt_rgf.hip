@@ -1593,17 +1593,20 @@ int qt_gw_polarization(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_
     try {
         QT_HIP(hipSetDevice(c.device));
         const std::vector<int> el = band_elements(nm, ndiag, false);
+        const std::vector<int> pr = band_elements(nm, ndiag, true);
         const int64_t nij = (int64_t)nm * nm;
         const int64_t per_slab = (int64_t)nky * nkz * nen;
         const bool dev = (flags & QT_F_DEVICE_OUTPUTS) != 0;
-        const size_t fixed = sizeof(int) * el.size() + 8192;
+        const size_t fixed = sizeof(int) * (el.size() + pr.size()) + 8192;
         // host path: slabs in chunks (every element of a slab range: 3 G and 3 P)
         const int cx = dev ? nx : gw_slab_chunk(c, nij, nx, per_slab, 6, fixed);
         const int64_t chunk = nij * cx * per_slab;
         c.arena.reserve(fixed + (dev ? 0 : 6 * (sizeof(double2) * (size_t)chunk + 256)));
         c.arena.reset();
         int* del = c.arena.take<int>((int64_t)el.size());
+        int* dpr = c.arena.take<int>((int64_t)pr.size());
         QT_HIP(hipMemcpyAsync(del, el.data(), sizeof(int) * el.size(), hipMemcpyHostToDevice, c.stream));
+        QT_HIP(hipMemcpyAsync(dpr, pr.data(), sizeof(int) * pr.size(), hipMemcpyHostToDevice, c.stream));
         GwArgs a;
         a.nen = nen;
         a.nky = nky;
@@ -1621,7 +1624,7 @@ int qt_gw_polarization(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_
             a.Pg = (double2*)Pg;
             const size_t bytes = sizeof(double2) * (size_t)(nij * nx * per_slab);
             for (double2* p : {a.Pr, a.Pl, a.Pg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
-            gw_polarization(c.stream, a, del, (int)el.size());
+            gw_polarization(c.stream, a, del, (int)el.size(), dpr, (int)pr.size());
             QT_HIP(hipStreamSynchronize(c.stream));
             return QT_OK;
         }
@@ -1645,7 +1648,7 @@ int qt_gw_polarization(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_
             a.Pr = pc[0];
             a.Pl = pc[1];
             a.Pg = pc[2];
-            gw_polarization(c.stream, a, del, (int)el.size());
+            gw_polarization(c.stream, a, del, (int)el.size(), dpr, (int)pr.size());
             for (int i = 0; i < 3; ++i) slabs_d2h(c, dst[i], pc[i], nij, nx, x0, cnt, per_slab);
             QT_HIP(hipStreamSynchronize(c.stream));
         }
@@ -2309,7 +2312,7 @@ int qt_gw_scba_step(qt_ham* h, const qt_rgf_params* p, int32_t nen, const double
         a.Pl = Pd[1];
         a.Pg = Pd[2];
         for (int i = 0; i < 3; ++i) QT_HIP(hipMemsetAsync(Pd[i], 0, abytes, c.stream));
-        gw_polarization(c.stream, a, del, (int)el.size());
+        gw_polarization(c.stream, a, del, (int)el.size(), dpr, (int)pr.size());
         // P element-major (ij, x, e) -> blocks (e, x, ij), the W solve's member-major layout
         for (int i = 0; i < 3; ++i)
             transpose3(c.stream, Pd[i], (int64_t)nx * nen, nen, A[i], (int64_t)nx * nn, nn, (int)nn, nen, nx);

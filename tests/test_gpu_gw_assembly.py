@@ -13,6 +13,7 @@ CASES = [
     (2, 2, 2, 2, 24, 1),
     (2, 1, 1, 1, 1000, 1),
     (2, 1, 1, 1, 2048, 0),
+    (6, 3, 1, 1, 200, 5),
 ]
 
 
