@@ -637,6 +637,65 @@ def run_uneven(args, hm, rgf) -> dict:
     }
 
 
+def run_small(args, hm, rgf) -> dict:
+    """The reference's CPU-case shape: nb=64 x 32 identical slabs, 256 energies
+    in [-3, 3] eV, one kz, lead eta 1e-5; each step is the whole job (all 256
+    points in one device batch), G^R/G^</G^> diagonals + contact currents."""
+    nb, nx, nen = 64, 32, 256
+    _, dev = hm.synthetic_device(nb, nx, seed=args.seed)
+    Eall = hm.energy_grid(-3.0, 3.0, nen)
+    Kall = np.zeros(nen, np.int32)
+    rgf.init(0)
+    H = rgf.Hamiltonian.from_device(dev)
+    # kernel breakdown (HIP events around every product and inversion), then
+    # the throughput from a run without the events
+    _, acc = time_batches(rgf, H, Eall, Kall, nen, nb * nx, args.steps, args.warmup)
+    bufs = {"tr": rgf.DeviceBuffer(8 * nen), "tre": rgf.DeviceBuffer(8 * nen)}
+    for key in ("gr_diag", "gl_diag", "gg_diag"):
+        bufs[key] = rgf.DeviceBuffer(16 * nen * nb * nx)
+    step = lambda: rgf.solve_batch_device(H, Eall, Kall, bufs, mul=0.0, mur=-0.1)
+    for _ in range(args.warmup):
+        step()
+    rgf.lib().qt_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    rgf.lib().qt_synchronize()
+    elapsed = time.perf_counter() - t0
+    for b in bufs.values():
+        b.free()
+    H.close()
+    points = nen * args.steps
+    gemm_tf = acc["gemm_flops"] / (acc["gemm_ms"] * 1e-3) / 1e12 if acc["gemm_ms"] > 0 else 0.0
+    inv_tf = acc["inv_flops"] / (acc["inv_ms"] * 1e-3) / 1e12 if acc["inv_ms"] > 0 else 0.0
+    K = acc["lead_iterations"] / max(1, 2 * points)
+    n3 = float(nb) ** 3
+    path_flops = nx * 120 * n3 + 2 * (K * 56 + 24) * n3
+    return {
+        "metric": "RGF energy-points/s, nb=64 x 32 slabs, 256 E, FP64",
+        "value": round(points / elapsed, 2), "unit": "energy points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 generator)",
+        "config": {"workload": "ballistic RGF, nb=64 x 32 slabs, 256 E in [-3,3] eV, one kz, "
+                               "G^R/G^</G^> diagonals + contact currents, plain FP64",
+                   "nb": nb, "slabs": nx, "energies": nen, "points_per_step": nen, "precision": "fp64"},
+        "roofline": {"bound": "mfma", "achieved": round(gemm_tf, 3), "peak": PEAK_FP64,
+                     "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4), "traffic": None,
+                     "kernel": "zgemm_mfma_kernel (batched complex128 GEMM)",
+                     "launches": acc["gemm_launches"]},
+        "path": {"lead_iterations_avg": round(K, 2),
+                 "algorithmic_tflops": round(points * path_flops / elapsed / 1e12, 3),
+                 "inversion_tflops": round(inv_tf, 3),
+                 "gemm_share_of_wall": round(acc["gemm_ms"] / (1e3 * elapsed), 3),
+                 "inversion_share_of_wall": round(acc["inv_ms"] / (1e3 * elapsed), 3),
+                 "lead_ms": round(acc["lead_ms"], 1), "forward_ms": round(acc["forward_ms"], 1),
+                 "backward_ms": round(acc["backward_ms"], 1)},
+        "cpu_baseline": None,
+    }
+
+
 def run_scba(args) -> dict:
     """One GW-SCBA iteration on the device (qt_gw_scba_step: G of every energy
     by the green_RGF_RS recursion with the current Sigma, P, W, Sigma, mixing;
@@ -717,7 +776,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--workload", choices=("rgf", "uneven", "leads", "warmstart", "gw", "corrconv", "ephoton",
-                                               "vendor-zgemm", "copy", "scba"),
+                                               "vendor-zgemm", "copy", "scba", "small"),
                     default="rgf",
                     help="rgf: the headline RGF sweep; uneven: 80 slabs of 128..768 orbitals; "
                          "leads: Sancho-Rubio surface Green's functions at nb=256, eta=1e-6; "
@@ -725,7 +784,8 @@ def main() -> int:
                          "gw: the GW energy-axis stage; corrconv: batched corr1d2/conv1d2; "
                          "vendor-zgemm: rocBLAS batched ZGEMM at the sweep's shape, for comparison; "
                          "copy: 256 MiB device-to-device copies, the HBM counter calibration; "
-                         "scba: one GW-SCBA iteration on the device at nm=256, 2048 E")
+                         "scba: one GW-SCBA iteration on the device at nm=256, 2048 E; "
+                         "small: the reference's CPU-case shape, nb=64 x 32 slabs x 256 E")
     ap.add_argument("--gw-slabs", type=int, default=2)
     ap.add_argument("--scba-slabs", type=int, default=6, help="scba: slabs of the device")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
@@ -770,6 +830,9 @@ def main() -> int:
         return 0
     if args.workload == "warmstart":
         print(json.dumps(run_warmstart(args, hm, rgf)), flush=True)
+        return 0
+    if args.workload == "small":
+        print(json.dumps(run_small(args, hm, rgf)), flush=True)
         return 0
 
     nb, nx, B = args.nb, args.slabs, args.batch

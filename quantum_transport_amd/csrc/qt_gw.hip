@@ -353,27 +353,33 @@ __global__ __launch_bounds__(NT) void gw_pol_pair_kernel(GwDims d, const int* __
 }
 
 // ---------------------------------------------------------------- Sigma
-// Sigma of element e at slab ix and point k, the q-sums in two passes:
+// Sigma of the element pair (ij, ji), row <= col, at slab ix and point k.
+// The transpose fold (S + S^T)/2 (gw_mod.f90:732-743) is linear and so is
+// the Sigma^R rule before it (:694-698), so the spectra of both elements are
+// summed and each output takes one inverse transform, written to ij and ji.
+// The q-sums of each element in two passes:
 //   1. i Im G^< and i Im G^R at k' share one packed transform; with the
 //      transforms of W^< and W^R they give Sigma^< and the three terms of
 //      Sigma^R (W^< + W^R enters by linearity);
 //   2. i Im G^> at k' and W^> give Sigma^>.
-// The reference takes eight forward transforms per q, these take five.
-// Every thread owns the same samples i = tid + j NT of every output, so the
-// (S + S^T)/2 fold reads back what the same thread wrote for the transposed
-// element eo (second == true) and writes the mean to both.
+// The reference takes eight forward transforms per q and element and three
+// inverses per element; these take four and a half per q and element (the
+// i Im G^> of both elements share one) and three inverses per pair.
 template <int L, int NT>
-__device__ __forceinline__ void sigma_elem(double2* s, const double2* tw, const GwDims& d, int e, int eo,
+__device__ __forceinline__ void sigma_pair(double2* s, const double2* tw, const GwDims& d, int ij, int ji,
                                            int ix, int ik, const double2* Gr, const double2* Gl,
                                            const double2* Gg, const double2* Wr, const double2* Wl,
                                            const double2* Wg, double2 dE, double2* Sr, double2* Sl,
-                                           double2* Sg, bool second) {
+                                           double2* Sg) {
     using Sp = Spec<L, NT>;
     constexpr int PER = Sp::PER;
     constexpr int EXT = (L / 2 + NT - 1) / NT;
     const int n = d.nen;
-    const double inv = 1.0 / (double)L;
-    // dE/L * F^-1[acc] at the samples this thread owns
+    const bool diag = ij == ji;
+    const int nel = diag ? 1 : 2;
+    // dE/L (one element) or dE/(2L) (the mean of two)
+    const double inv = (diag ? 1.0 : 0.5) / (double)L;
+    // the scaled inverse of acc at the samples this thread owns
     // (the inverse takes acc from registers and hands back the thread's own
     // samples, where the transform allows it)
     auto extract = [&](const Sp& acc, double2* t) {
@@ -386,20 +392,14 @@ __device__ __forceinline__ void sigma_elem(double2* s, const double2* tw, const 
             });
     };
     auto store = [&](double2* S, const double2* t) {
-        double2* own = series(S, d, e, ix, ik);
-        double2* other = series(S, d, eo, ix, ik);
+        double2* a = series(S, d, ij, ix, ik);
+        double2* b = series(S, d, ji, ix, ik);
 #pragma unroll
         for (int q = 0; q < EXT; ++q) {
             const int i = threadIdx.x + q * NT;
             if (i >= n) continue;
-            if (second) {
-                const double2 o = other[i];
-                const double2 m = make_double2(0.5 * (o.x + t[q].x), 0.5 * (o.y + t[q].y));
-                own[i] = m;
-                other[i] = m;
-            } else {
-                own[i] = t[q];
-            }
+            a[i] = t[q];
+            if (!diag) b[i] = t[q];
         }
     };
     // F(Yr), Yr[l mod L] = Y[l + n/2]  (conv)
@@ -412,43 +412,44 @@ __device__ __forceinline__ void sigma_elem(double2* s, const double2* tw, const 
             return src >= 0 ? w[src] : make_double2(0.0, 0.0);
         });
     };
-    // the first element's outputs are read back below by the same threads
-    if (second) __threadfence_block();
     double2 t[EXT], rre[EXT];
     double dim[EXT];
     {
         Sp accl, accr;
         accl.zero();
         accr.zero();
-        for (int iq = 0; iq < d.nk; ++iq) {
-            const int kd = kd_index(ik, iq, d.nky, d.nkz);
-            fwd_pack<L, NT>(s, tw, series(Gl, d, e, ix, kd), series(Gr, d, e, ix, kd), n);
-            double2 gl[PER], gr[PER];
+        for (int m = 0; m < nel; ++m) {
+            const int e = m == 0 ? ij : ji;
+            for (int iq = 0; iq < d.nk; ++iq) {
+                const int kd = kd_index(ik, iq, d.nky, d.nkz);
+                fwd_pack<L, NT>(s, tw, series(Gl, d, e, ix, kd), series(Gr, d, e, ix, kd), n);
+                double2 gl[PER], gr[PER];
 #pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                if (!Sp::ok(threadIdx.x + q * NT)) continue;
-                double2 zk, zm;
-                zpair<L, NT>(s, q, zk, zm);
-                gl[q] = spec1(zk, zm);
-                gr[q] = spec2(zk, zm);
-            }
-            // Sigma^< += F(G^<) F(W^<r), Sigma^R += F(G^R) F(W^<r)
-            fwd_w(series(Wl, d, e, ix, iq));
+                for (int q = 0; q < PER; ++q) {
+                    if (!Sp::ok(threadIdx.x + q * NT)) continue;
+                    double2 zk, zm;
+                    zpair<L, NT>(s, q, zk, zm);
+                    gl[q] = spec1(zk, zm);
+                    gr[q] = spec2(zk, zm);
+                }
+                // Sigma^< += F(G^<) F(W^<r), Sigma^R += F(G^R) F(W^<r)
+                fwd_w(series(Wl, d, e, ix, iq));
 #pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                const int i = threadIdx.x + q * NT;
-                if (!Sp::ok(i)) continue;
-                const double2 wv = s[pad(i)];
-                accl.v[q] = cadd(accl.v[q], cmul(gl[q], wv));
-                accr.v[q] = cadd(accr.v[q], cmul(gr[q], wv));
-            }
-            // Sigma^R += (F(G^<) + F(G^R)) F(W^Rr)
-            fwd_w(series(Wr, d, e, ix, iq));
+                for (int q = 0; q < PER; ++q) {
+                    const int i = threadIdx.x + q * NT;
+                    if (!Sp::ok(i)) continue;
+                    const double2 wv = s[pad(i)];
+                    accl.v[q] = cadd(accl.v[q], cmul(gl[q], wv));
+                    accr.v[q] = cadd(accr.v[q], cmul(gr[q], wv));
+                }
+                // Sigma^R += (F(G^<) + F(G^R)) F(W^Rr)
+                fwd_w(series(Wr, d, e, ix, iq));
 #pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                const int i = threadIdx.x + q * NT;
-                if (!Sp::ok(i)) continue;
-                accr.v[q] = cadd(accr.v[q], cmul(cadd(gl[q], gr[q]), s[pad(i)]));
+                for (int q = 0; q < PER; ++q) {
+                    const int i = threadIdx.x + q * NT;
+                    if (!Sp::ok(i)) continue;
+                    accr.v[q] = cadd(accr.v[q], cmul(cadd(gl[q], gr[q]), s[pad(i)]));
+                }
             }
         }
         extract(accl, t);
@@ -460,25 +461,35 @@ __device__ __forceinline__ void sigma_elem(double2* s, const double2* tw, const 
         for (int q = 0; q < EXT; ++q) rre[q] = t[q];
     }
     {
+        // i Im G^> of both elements in one packed transform
         Sp accg;
         accg.zero();
         for (int iq = 0; iq < d.nk; ++iq) {
             const int kd = kd_index(ik, iq, d.nky, d.nkz);
-            fwd_pack<L, NT>(s, tw, series(Gg, d, e, ix, kd), nullptr, n);
-            double2 gg[PER];
+            fwd_pack<L, NT>(s, tw, series(Gg, d, ij, ix, kd), diag ? nullptr : series(Gg, d, ji, ix, kd), n);
+            double2 g1[PER], g2[PER];
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 if (!Sp::ok(threadIdx.x + q * NT)) continue;
                 double2 zk, zm;
                 zpair<L, NT>(s, q, zk, zm);
-                gg[q] = spec1(zk, zm);
+                g1[q] = spec1(zk, zm);
+                g2[q] = spec2(zk, zm);
             }
-            fwd_w(series(Wg, d, e, ix, iq));
+            fwd_w(series(Wg, d, ij, ix, iq));
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 const int i = threadIdx.x + q * NT;
                 if (!Sp::ok(i)) continue;
-                accg.v[q] = cadd(accg.v[q], cmul(gg[q], s[pad(i)]));
+                accg.v[q] = cadd(accg.v[q], cmul(g1[q], s[pad(i)]));
+            }
+            if (diag) continue;
+            fwd_w(series(Wg, d, ji, ix, iq));
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int i = threadIdx.x + q * NT;
+                if (!Sp::ok(i)) continue;
+                accg.v[q] = cadd(accg.v[q], cmul(g2[q], s[pad(i)]));
             }
         }
         extract(accg, t);
@@ -509,11 +520,8 @@ __global__ __launch_bounds__(NT) void gw_sigma_kernel(GwDims d, const int* __res
     const int ij = pairs[p];
     const int row = ij % d.nm, col = ij / d.nm;
     const int ji = col + row * d.nm;
-    for (int ik = 0; ik < d.nk; ++ik) {
-        sigma_elem<L, NT>(s, tw, d, ij, ij, ix, ik, Gr, Gl, Gg, Wr, Wl, Wg, dE, Sr, Sl, Sg, false);
-        if (ji != ij)
-            sigma_elem<L, NT>(s, tw, d, ji, ij, ix, ik, Gr, Gl, Gg, Wr, Wl, Wg, dE, Sr, Sl, Sg, true);
-    }
+    for (int ik = 0; ik < d.nk; ++ik)
+        sigma_pair<L, NT>(s, tw, d, ij, ji, ix, ik, Gr, Gl, Gg, Wr, Wl, Wg, dE, Sr, Sl, Sg);
 }
 
 // e-photon self-energy, monochromatic field: the two energy-shifted blocks

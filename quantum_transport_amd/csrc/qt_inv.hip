@@ -28,6 +28,9 @@
 //      other rows were eliminated with, which is what gives Gauss-Jordan its
 //      small residual |A X - I|; multiplying by the panel's explicit
 //      inv(A_KK) instead loses up to cond(A_KK) of it.
+// Blocks of n <= 64 are inverted by one kernel holding the whole matrix in
+// registers (gj_small_kernel: the same pivot rule and row operations, no
+// panel records).
 // A zero pivot sets info (first column, 1-based) and the output block is
 // zeroed (linalg.f90:50-58).
 #include <algorithm>
@@ -382,6 +385,117 @@ __global__ __launch_bounds__(256) void gj_finish_kernel(const double2* W, int64_
     }
 }
 
+// ------------------------------------------------------------ blocks of n <= 64
+// The whole inversion in one launch: one 256-thread workgroup per member,
+// thread t holding row t / 4, columns 16 (t % 4) .. 16 (t % 4) + 15 in
+// registers.  Unblocked Gauss-Jordan with the pivot rule and the row
+// operations of the panel kernel (first-indexed maximal |re| + |im| among the
+// rows not yet pivoted, the pivot row kept unscaled with its factor 1/pivot
+// applied at the write-back), two barriers per column; a row's multiplier
+// comes from the thread of its quad holding the pivot column.  The output
+// gather X[pinv[R]][pivp[C]] = W[R][C] (W: the working array, row R scaled
+// by its factor) and the zero block of a failed factorisation are written by
+// the same workgroup.  A and X may be the same array: every element of A is
+// read before the first barrier and X is written after the last one.
+__global__ __launch_bounds__(256) void gj_small_kernel(const double2* A, int64_t sa, double2* X, int64_t sx,
+                                                       int n, int* info, const int* active) {
+    const int b = blockIdx.x;
+    if (active && !active[b]) {
+        if (threadIdx.x == 0) info[b] = 0;
+        return;
+    }
+    __shared__ double red_v[2][4];
+    __shared__ int red_i[2][4];
+    __shared__ double2 prow[2][64];
+    __shared__ int pvs[64];   // physical row pivoted at step j
+    __shared__ int pinv[64];  // step at which physical row R was pivoted
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int r = t >> 2, q = t & 3;
+    const double2* Ab = A + b * sa;
+    double2 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int c = 16 * q + k;
+        v[k] = (r < n && c < n) ? Ab[r + c * n] : make_double2(0.0, 0.0);
+    }
+    bool mine = r < n;  // row not yet pivoted
+    double2 srow = make_double2(1.0, 0.0);
+    int zero_col = 0;   // first zero pivot (1-based), thread 0
+    const int quad = lane & ~3;
+#pragma unroll 1
+    for (int jq = 0; jq < 4; ++jq) {
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const int j = 16 * jq + jj;
+            if (j >= n) continue;  // uniform over the workgroup
+            const int buf = j & 1;
+            const double cand = (mine && q == jq) ? cabs1(v[jj]) : -1.0;
+            const double wmax = wave_max(cand);
+            const unsigned long long hit = __ballot(cand == wmax);
+            if (lane == 0) {
+                red_v[buf][wave] = wmax;
+                red_i[buf][wave] = (wave * 64 + (__ffsll((long long)hit) - 1)) >> 2;
+            }
+            __syncthreads();
+            // lane w < 4 takes wave w's candidate; waves hold ascending rows
+            const double rvv = lane < 4 ? red_v[buf][lane] : -2.0;
+            const int riv = lane < 4 ? red_i[buf][lane] : 0;
+            const double bv = wave_max(rvv);
+            const unsigned long long hitw = __ballot(rvv == bv);
+            int p = __builtin_amdgcn_readlane(riv, __ffsll((long long)hitw) - 1);
+            if (p < 0 || p >= n) p = 0;  // unreachable: n - j rows are always unpivoted
+            if (r == p) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) prow[buf][16 * q + k] = v[k];
+                mine = false;
+            }
+            if (t == 0) {
+                pvs[j] = p;
+                pinv[p] = j;
+                if (!(bv > 0.0) && zero_col == 0) zero_col = j + 1;
+            }
+            __syncthreads();
+            const double2 pivv = prow[buf][j];
+            const double2 d = (pivv.x == 0.0 && pivv.y == 0.0) ? make_double2(0.0, 0.0) : crecip(pivv);
+            // this row's entry in column j, from the quad's thread holding it
+            double2 f;
+            f.x = __shfl(v[jj].x, quad | jq, 64);
+            f.y = __shfl(v[jj].y, quad | jq, 64);
+            const double2 gf = cmul(f, d);
+            if (r == p) {
+                srow = d;  // true row = d * v, true v[j] = d
+                if (q == jq) v[jj] = make_double2(1.0, 0.0);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const double2 u = prow[buf][16 * q + k];
+                    double2 w = v[k];
+                    w.x = fma(-gf.x, u.x, w.x);
+                    w.x = fma(gf.y, u.y, w.x);
+                    w.y = fma(-gf.x, u.y, w.y);
+                    w.y = fma(-gf.y, u.x, w.y);
+                    v[k] = w;
+                }
+                if (q == jq) v[jj] = make_double2(-gf.x, -gf.y);
+            }
+        }
+    }
+    if (t == 0) red_i[0][0] = zero_col;
+    __syncthreads();
+    const bool bad = red_i[0][0] != 0;
+    if (t == 0) info[b] = zero_col;
+    if (r >= n) return;
+    double2* Xb = X + b * sx;
+    const int orow = pinv[r];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int c = 16 * q + k;
+        if (c >= n) continue;
+        const double2 o = bad ? make_double2(0.0, 0.0) : cmul(v[k], srow);
+        Xb[orow + (int64_t)pvs[c] * n] = o;
+    }
+}
+
 __global__ void zero_int_kernel(int* p, int n) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -411,6 +525,12 @@ namespace {
 
 void zinv_part(hipStream_t s, const InvArgs& a) {
     const int n = a.n;
+    if (n <= 64) {
+        hipLaunchKernelGGL(gj_small_kernel, dim3(a.batch), dim3(256), 0, s, a.A, a.sa, a.X, a.sx, n, a.info,
+                           a.active);
+        QT_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(zero_int_kernel, dim3((a.batch + 255) / 256), dim3(256), 0, s, a.info, a.batch);
     if (n <= 256)
         run_panels<32, 256>(s, a);
@@ -471,7 +591,8 @@ int inv_parts() {
 
 }  // namespace
 
-// The panel kernel keeps one workgroup per member busy for a whole latency-
+// Blocks of n <= 64 take the one-launch kernel above on stream s.  For larger
+// ones, the panel kernel keeps one workgroup per member busy for a whole latency-
 // bound column sweep, leaving most CUs idle; the batch is therefore split in
 // parts on as many streams so that the panels of one part run while the
 // MFMA updates of the others fill the remaining CUs.
@@ -488,7 +609,7 @@ void zinv(hipStream_t s, const InvArgs& a, Timer* t) {
     if (n > 1024) throw hip_error("zinv: block size above 1024", 6);
     if (!a.G || a.sg < (int64_t)n * kInvPanelMax) throw hip_error("zinv: multiplier workspace missing", 1);
     if (t && t->enabled) t->begin(s, 1);
-    if (a.batch >= 8 && n >= 64) {
+    if (a.batch >= 8 && n > 64) {
         Side& sd = side();
         const int parts = std::min(inv_parts(), a.batch / 4);
         QT_HIP(hipEventRecord(sd.fork, s));

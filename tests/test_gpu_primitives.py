@@ -41,7 +41,7 @@ def test_zgemm_vs_numpy(qt, opa, opb, mnk):
     assert err < 1e-13, err
 
 
-@pytest.mark.parametrize("n", [1, 3, 12, 16, 17, 40, 96, 200, 513, 700, 1000])
+@pytest.mark.parametrize("n", [1, 3, 12, 16, 17, 40, 63, 64, 65, 96, 200, 513, 700, 1000])
 def test_invert_vs_numpy(qt, n):
     rng = np.random.default_rng(n)
     A = _rand(rng, 3, n, n) + 2.0 * np.eye(n)
