@@ -387,16 +387,25 @@ __global__ __launch_bounds__(256) void gj_finish_kernel(const double2* W, int64_
 
 // ------------------------------------------------------------ blocks of n <= 64
 // The whole inversion in one launch: one 256-thread workgroup per member,
-// thread t holding row t / 4, columns 16 (t % 4) .. 16 (t % 4) + 15 in
+// wave w holding columns 16 w .. 16 w + 15, lane r holding row r, in
 // registers.  Unblocked Gauss-Jordan with the pivot rule and the row
 // operations of the panel kernel (first-indexed maximal |re| + |im| among the
 // rows not yet pivoted, the pivot row kept unscaled with its factor 1/pivot
-// applied at the write-back), two barriers per column; a row's multiplier
-// comes from the thread of its quad holding the pivot column.  The output
+// applied at the write-back).  Column j lives in one wave, which finds the
+// pivot with a wave maximum and publishes the column (every row's
+// multiplier) through LDS: one barrier per column (double-buffered, so the
+// next owner never overwrites a buffer still being read).  Each wave takes
+// the pivot row's values in its columns from lane p by readlane.  The output
 // gather X[pinv[R]][pivp[C]] = W[R][C] (W: the working array, row R scaled
 // by its factor) and the zero block of a failed factorisation are written by
 // the same workgroup.  A and X may be the same array: every element of A is
 // read before the first barrier and X is written after the last one.
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+    return __hiloint2double(hi, lo);
+}
+
 __global__ __launch_bounds__(256) void gj_small_kernel(const double2* A, int64_t sa, double2* X, int64_t sx,
                                                        int n, int* info, const int* active) {
     const int b = blockIdx.x;
@@ -404,92 +413,88 @@ __global__ __launch_bounds__(256) void gj_small_kernel(const double2* A, int64_t
         if (threadIdx.x == 0) info[b] = 0;
         return;
     }
-    __shared__ double red_v[2][4];
-    __shared__ int red_i[2][4];
-    __shared__ double2 prow[2][64];
-    __shared__ int pvs[64];   // physical row pivoted at step j
-    __shared__ int pinv[64];  // step at which physical row R was pivoted
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int r = t >> 2, q = t & 3;
+    __shared__ double2 fcol[2][64];  // column j of the working array
+    __shared__ int piv_s[2];         // pivot row of step j
+    __shared__ double pmax_s[2];     // its |re| + |im|
+    __shared__ int pvs[64];          // physical row pivoted at step j
+    __shared__ int pinv[64];         // step at which physical row R was pivoted
+    __shared__ int zero_s;
+    const int t = threadIdx.x, r = t & 63, w = t >> 6;
     const double2* Ab = A + b * sa;
     double2 v[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        const int c = 16 * q + k;
+        const int c = 16 * w + k;
         v[k] = (r < n && c < n) ? Ab[r + c * n] : make_double2(0.0, 0.0);
     }
     bool mine = r < n;  // row not yet pivoted
     double2 srow = make_double2(1.0, 0.0);
     int zero_col = 0;   // first zero pivot (1-based), thread 0
-    const int quad = lane & ~3;
 #pragma unroll 1
-    for (int jq = 0; jq < 4; ++jq) {
+    for (int jw = 0; jw < 4; ++jw) {
 #pragma unroll
         for (int jj = 0; jj < 16; ++jj) {
-            const int j = 16 * jq + jj;
+            const int j = 16 * jw + jj;
             if (j >= n) continue;  // uniform over the workgroup
             const int buf = j & 1;
-            const double cand = (mine && q == jq) ? cabs1(v[jj]) : -1.0;
-            const double wmax = wave_max(cand);
-            const unsigned long long hit = __ballot(cand == wmax);
-            if (lane == 0) {
-                red_v[buf][wave] = wmax;
-                red_i[buf][wave] = (wave * 64 + (__ffsll((long long)hit) - 1)) >> 2;
+            if (w == jw) {  // the wave holding column j
+                const double cand = mine ? cabs1(v[jj]) : -1.0;
+                const double wmax = wave_max(cand);
+                const unsigned long long hit = __ballot(cand == wmax);
+                fcol[buf][r] = v[jj];
+                if (r == 0) {
+                    piv_s[buf] = __ffsll((long long)hit) - 1;
+                    pmax_s[buf] = wmax;
+                }
             }
             __syncthreads();
-            // lane w < 4 takes wave w's candidate; waves hold ascending rows
-            const double rvv = lane < 4 ? red_v[buf][lane] : -2.0;
-            const int riv = lane < 4 ? red_i[buf][lane] : 0;
-            const double bv = wave_max(rvv);
-            const unsigned long long hitw = __ballot(rvv == bv);
-            int p = __builtin_amdgcn_readlane(riv, __ffsll((long long)hitw) - 1);
+            int p = __builtin_amdgcn_readfirstlane(piv_s[buf]);
             if (p < 0 || p >= n) p = 0;  // unreachable: n - j rows are always unpivoted
-            if (r == p) {
-#pragma unroll
-                for (int k = 0; k < 16; ++k) prow[buf][16 * q + k] = v[k];
-                mine = false;
-            }
+            const double2 f = fcol[buf][r];
+            const double2 pivv = fcol[buf][p];
+            const double2 d = (pivv.x == 0.0 && pivv.y == 0.0) ? make_double2(0.0, 0.0) : crecip(pivv);
             if (t == 0) {
                 pvs[j] = p;
                 pinv[p] = j;
-                if (!(bv > 0.0) && zero_col == 0) zero_col = j + 1;
+                if (!(pmax_s[buf] > 0.0) && zero_col == 0) zero_col = j + 1;
             }
-            __syncthreads();
-            const double2 pivv = prow[buf][j];
-            const double2 d = (pivv.x == 0.0 && pivv.y == 0.0) ? make_double2(0.0, 0.0) : crecip(pivv);
-            // this row's entry in column j, from the quad's thread holding it
-            double2 f;
-            f.x = __shfl(v[jj].x, quad | jq, 64);
-            f.y = __shfl(v[jj].y, quad | jq, 64);
-            const double2 gf = cmul(f, d);
-            if (r == p) {
-                srow = d;  // true row = d * v, true v[j] = d
-                if (q == jq) v[jj] = make_double2(1.0, 0.0);
-            } else {
+            const double2 gf = cmul(f, d);  // f / pivot
+            const bool piv = r == p;
+            // every row but the pivot row: v -= gf * (pivot row); the pivot
+            // row's value of column k is read from lane p before it changes
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const double2 u = prow[buf][16 * q + k];
-                    double2 w = v[k];
-                    w.x = fma(-gf.x, u.x, w.x);
-                    w.x = fma(gf.y, u.y, w.x);
-                    w.y = fma(-gf.x, u.y, w.y);
-                    w.y = fma(-gf.y, u.x, w.y);
-                    v[k] = w;
-                }
-                if (q == jq) v[jj] = make_double2(-gf.x, -gf.y);
+            for (int k = 0; k < 16; ++k) {
+                const double ux = readlane_f64(v[k].x, p), uy = readlane_f64(v[k].y, p);
+                double2 x = v[k];
+                x.x = fma(-gf.x, ux, x.x);
+                x.x = fma(gf.y, uy, x.x);
+                x.y = fma(-gf.x, uy, x.y);
+                x.y = fma(-gf.y, ux, x.y);
+                v[k].x = piv ? v[k].x : x.x;
+                v[k].y = piv ? v[k].y : x.y;
+            }
+            if (w == jw) {
+                v[jj].x = piv ? 1.0 : -gf.x;
+                v[jj].y = piv ? 0.0 : -gf.y;
+            }
+            if (piv) {
+                srow = d;  // the row stays unscaled; true row = d * v, true v[j] = d
+                mine = false;
             }
         }
     }
-    if (t == 0) red_i[0][0] = zero_col;
+    if (t == 0) {
+        zero_s = zero_col;
+        info[b] = zero_col;
+    }
     __syncthreads();
-    const bool bad = red_i[0][0] != 0;
-    if (t == 0) info[b] = zero_col;
+    const bool bad = zero_s != 0;
     if (r >= n) return;
     double2* Xb = X + b * sx;
     const int orow = pinv[r];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        const int c = 16 * q + k;
+        const int c = 16 * w + k;
         if (c >= n) continue;
         const double2 o = bad ? make_double2(0.0, 0.0) : cmul(v[k], srow);
         Xb[orow + (int64_t)pvs[c] * n] = o;
