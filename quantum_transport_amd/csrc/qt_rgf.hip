@@ -1623,7 +1623,8 @@ int qt_gw_polarization(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_
             a.Pl = (double2*)Pl;
             a.Pg = (double2*)Pg;
             const size_t bytes = sizeof(double2) * (size_t)(nij * nx * per_slab);
-            for (double2* p : {a.Pr, a.Pl, a.Pg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
+            if (ndiag < nm - 1)  // out-of-band series stay zero; in-band ones are written whole
+                for (double2* p : {a.Pr, a.Pl, a.Pg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
             gw_polarization(c.stream, a, del, (int)el.size(), dpr, (int)pr.size());
             QT_HIP(hipStreamSynchronize(c.stream));
             return QT_OK;
@@ -1639,7 +1640,7 @@ int qt_gw_polarization(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_
             const size_t bytes = sizeof(double2) * (size_t)(nij * cnt * per_slab);
             for (int i = 0; i < 3; ++i) {
                 slabs_h2d(c, g[i], src[i], nij, nx, x0, cnt, per_slab);
-                QT_HIP(hipMemsetAsync(pc[i], 0, bytes, c.stream));
+                if (ndiag < nm - 1) QT_HIP(hipMemsetAsync(pc[i], 0, bytes, c.stream));
             }
             a.nx = cnt;
             a.Gr = g[0];
@@ -1700,7 +1701,8 @@ int qt_gw_selfenergy(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t 
             a.Sl = (double2*)Sl;
             a.Sg = (double2*)Sg;
             const size_t bytes = sizeof(double2) * (size_t)(nij * nx * per_slab);
-            for (double2* p : {a.Sr, a.Sl, a.Sg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
+            if (ndiag < nm - 1)
+                for (double2* p : {a.Sr, a.Sl, a.Sg}) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
             gw_selfenergy(c.stream, a, dpr, (int)pr.size());
             QT_HIP(hipStreamSynchronize(c.stream));
             return QT_OK;
@@ -1715,7 +1717,8 @@ int qt_gw_selfenergy(int32_t nen, int32_t nky, int32_t nkz, int32_t nx, int32_t 
             const int cnt = std::min(cx, nx - x0);
             const size_t bytes = sizeof(double2) * (size_t)(nij * cnt * per_slab);
             for (int i = 0; i < 6; ++i) slabs_h2d(c, in[i], src[i], nij, nx, x0, cnt, per_slab);
-            for (double2* p : sc) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
+            if (ndiag < nm - 1)
+                for (double2* p : sc) QT_HIP(hipMemsetAsync(p, 0, bytes, c.stream));
             a.nx = cnt;
             a.Gr = in[0];
             a.Gl = in[1];
@@ -2311,7 +2314,8 @@ int qt_gw_scba_step(qt_ham* h, const qt_rgf_params* p, int32_t nen, const double
         a.Pr = Pd[0];
         a.Pl = Pd[1];
         a.Pg = Pd[2];
-        for (int i = 0; i < 3; ++i) QT_HIP(hipMemsetAsync(Pd[i], 0, abytes, c.stream));
+        if (ndiag < nm - 1)
+            for (int i = 0; i < 3; ++i) QT_HIP(hipMemsetAsync(Pd[i], 0, abytes, c.stream));
         gw_polarization(c.stream, a, del, (int)el.size(), dpr, (int)pr.size());
         // P element-major (ij, x, e) -> blocks (e, x, ij), the W solve's member-major layout
         for (int i = 0; i < 3; ++i)
@@ -2350,7 +2354,8 @@ int qt_gw_scba_step(qt_ham* h, const qt_rgf_params* p, int32_t nen, const double
         a.Sr = Bm[0];
         a.Sl = Bm[1];
         a.Sg = Bm[2];
-        for (int i = 0; i < 3; ++i) QT_HIP(hipMemsetAsync(Bm[i], 0, abytes, c.stream));
+        if (ndiag < nm - 1)
+            for (int i = 0; i < 3; ++i) QT_HIP(hipMemsetAsync(Bm[i], 0, abytes, c.stream));
         gw_selfenergy(c.stream, a, dpr, (int)pr.size());
         for (int i = 0; i < 3; ++i) gw_mix(c.stream, total, alpha, Sd[i], Bm[i]);
         QT_HIP(hipStreamSynchronize(c.stream));
