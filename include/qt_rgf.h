@@ -398,6 +398,42 @@ int qt_gw_sigma_ephoton(int32_t nm, int32_t length, int32_t nen, int32_t nop, do
                         double intensity, double hw, const void* Mii, const void* Gl, const void* Gg,
                         void* Sr, void* Sl, void* Sg, int32_t flags);
 
+/* Cell-bond current spectra and their energy integrals, the role of
+ * calc_block_current (called at gw_mod.f90:384 and :1001 with that argument
+ * list; the routine itself is not in the reference's sources, so this is
+ * the definition: the NEGF bond current between neighbouring cells of nb
+ * orbitals, ns cells per slab, nm = nb ns).  Host arrays, Fortran layouts:
+ *   Hii (nm, nm, nx) complex: the slab blocks of H;
+ *   GL (nm, nm, nx, nen) complex: the G^< diagonal blocks of every energy;
+ *   cur (nm, nm, nx, nen) real: green_RGF_RS's slab-bond matrices
+ *        Re H_{l,l+1} G^<_{l+1,l} - Re H_{l+1,l} G^<_{l,l+1} (gw_mod.f90:1250-1273);
+ *   jdens (nb, nb, nx ns, nen) real out: for bond c = (l, j) (cell j of slab l
+ *        to the next cell), j < ns: Re H_{j,j+1} G^<_{j+1,j} - Re H_{j+1,j} G^<_{j,j+1}
+ *        (nb x nb products of the cell blocks of Hii(l) and GL(l)); j = ns: the
+ *        sum of the ns diagonal nb x nb blocks of cur(l) (zero for l = nx).
+ *        The trace of a bond's matrix is the particle current through it;
+ *   tot_cur, tot_ecur (nb, nb, nx ns) real out: sum_e jdens(e) w and
+ *        sum_e En[e] jdens(e) w with w = dE e0 / (2 pi hbar) e0 spindeg,
+ *        dE = En[1] - En[0] (the reference's terminal-current weight,
+ *        gw_mod.f90:424-428; 0 for one energy).
+ * Energies are processed in chunks sized like qt_gw_sigma_ephoton's. */
+int qt_gw_block_current(int32_t nen, int32_t nx, int32_t nb, int32_t ns, const double* En, double spindeg,
+                        const void* Hii, const void* GL, const double* cur, double* jdens, double* tot_cur,
+                        double* tot_ecur);
+
+/* Collision integrals, the role of calc_block_collision (called at
+ * gw_mod.f90:795 and :926 with that argument list; not in the reference's
+ * sources, so this is the definition): per slab block and energy
+ *   Ispec(E) = Sigma^<(E) G^>(E) - Sigma^>(E) G^<(E)   (nm x nm products),
+ * the scattering-in minus scattering-out term of the kinetic equation for
+ * G^<, which vanishes at equilibrium (Sigma and G with one Fermi function);
+ *   Itot = sum_e Ispec(e) dE spindeg / (2 pi),  dE = En[1] - En[0].
+ * Host arrays (nm, nm, nx, nen) complex in, Ispec the same, Itot (nm, nm, nx).
+ * Energies are processed in chunks sized like qt_gw_sigma_ephoton's. */
+int qt_gw_block_collision(int32_t nen, int32_t nx, int32_t nm, const double* En, double spindeg,
+                          const void* Sl, const void* Sg, const void* Gl, const void* Gg, void* Itot,
+                          void* Ispec);
+
 /* S += alpha (Snew - S) over n complex128 samples (gw_mod.f90:747-749). */
 int qt_gw_mix(int64_t n, double alpha, void* S, const void* Snew, int32_t flags);
 

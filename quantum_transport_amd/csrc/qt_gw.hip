@@ -540,6 +540,62 @@ __global__ void eph_retarded_kernel(int64_t n, const double2* __restrict__ Sl, c
         Sr[t] = make_double2(0.0, 0.5 * (Sg[t].y - Sl[t].y));
 }
 
+// ---------------------------------------------------------------- currents, collision integrals
+// U, L: the cell-superdiagonal blocks (j, j+1) and the cell-subdiagonal
+// blocks (j+1, j) of each slab's diagonal block H (nm x nm, nm = nb ns),
+// every other entry zero
+__global__ void cell_bands_kernel(int nm, int nb, int nx, const double2* __restrict__ H, double2* U,
+                                  double2* L) {
+    const int64_t nn = (int64_t)nm * nm, total = nn * nx;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = t % nn;
+        const int r = (int)(e % nm), c = (int)(e / nm);
+        const int cr = r / nb, cc = c / nb;
+        const double2 h = H[t];
+        U[t] = cc == cr + 1 ? h : make_double2(0.0, 0.0);
+        L[t] = cr == cc + 1 ? h : make_double2(0.0, 0.0);
+    }
+}
+
+// Orbital-resolved current of every cell bond c = ix ns + j (c -> c + 1) of
+// member m = ix + nx e, nb x nb, into jd (nb, nb, nx ns, cnt):
+//   j < ns - 1:   Re (U G^<)_{jj} - Re (L G^<)_{j+1,j+1}
+//                 (= Re H_{j,j+1} G^<_{j+1,j} - Re H_{j+1,j} G^<_{j,j+1});
+//   j = ns - 1:   sum over the cells k of the diagonal blocks (k, k) of the
+//                 slab bond's current matrix cur (nm, nm, nx, cnt), zero in
+//                 the last slab.
+__global__ void cell_current_kernel(int nb, int ns, int nx, int cnt, const double2* __restrict__ C1,
+                                    const double2* __restrict__ C2, const double* __restrict__ cur, double* jd) {
+    const int nm = nb * ns;
+    const int64_t nn = (int64_t)nm * nm, bb = (int64_t)nb * nb;
+    const int64_t total = bb * ns * nx * (int64_t)cnt;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(t % nb), b = (int)((t / nb) % nb);
+        const int64_t cm = t / bb;  // cell bond of member: c + nx ns e
+        const int j = (int)(cm % ns);
+        const int64_t m = cm / ns;  // member ix + nx e
+        const int ix = (int)(m % nx);
+        double v = 0.0;
+        if (j < ns - 1) {
+            v = C1[m * nn + (j * nb + a) + (int64_t)(j * nb + b) * nm].x -
+                C2[m * nn + ((j + 1) * nb + a) + (int64_t)((j + 1) * nb + b) * nm].x;
+        } else if (ix < nx - 1) {
+            for (int k = 0; k < ns; ++k) v += cur[m * nn + (k * nb + a) + (int64_t)(k * nb + b) * nm];
+        }
+        jd[t] = v;
+    }
+}
+
+// acc[i] += sum_e X[i + n e] w[e], e = 0 .. cnt-1 in order
+__global__ void energy_sum_kernel(int64_t n, int cnt, const double* __restrict__ X, const double* __restrict__ w,
+                                  double* acc) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = acc[i];
+        for (int e = 0; e < cnt; ++e) s += X[i + n * e] * w[e];
+        acc[i] = s;
+    }
+}
+
 __global__ void gw_mix_kernel(int64_t n, double alpha, double2* S, const double2* __restrict__ Snew) {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         const double2 a = S[t], b = Snew[t];
@@ -676,6 +732,30 @@ void eph_sum(hipStream_t st, int64_t n, int64_t shift, const double2* G, double2
     if (n <= 0) return;
     const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(eph_sum_kernel, dim3((unsigned)blocks), dim3(256), 0, st, n, shift, G, S);
+    QT_HIP(hipGetLastError());
+}
+
+void cell_bands(hipStream_t st, int nm, int nb, int nx, const double2* H, double2* U, double2* L) {
+    const int64_t n = (int64_t)nm * nm * nx;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(cell_bands_kernel, dim3((unsigned)blocks), dim3(256), 0, st, nm, nb, nx, H, U, L);
+    QT_HIP(hipGetLastError());
+}
+
+void cell_current(hipStream_t st, int nb, int ns, int nx, int cnt, const double2* C1, const double2* C2,
+                  const double* cur, double* jd) {
+    const int64_t n = (int64_t)nb * nb * ns * nx * cnt;
+    if (n <= 0) return;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(cell_current_kernel, dim3((unsigned)blocks), dim3(256), 0, st, nb, ns, nx, cnt, C1, C2,
+                       cur, jd);
+    QT_HIP(hipGetLastError());
+}
+
+void energy_sum(hipStream_t st, int64_t n, int cnt, const double* X, const double* w, double* acc) {
+    if (n <= 0 || cnt <= 0) return;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(energy_sum_kernel, dim3((unsigned)blocks), dim3(256), 0, st, n, cnt, X, w, acc);
     QT_HIP(hipGetLastError());
 }
 

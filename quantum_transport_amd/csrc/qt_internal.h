@@ -199,6 +199,15 @@ void gw_mix(hipStream_t s, int64_t n, double alpha, double2* S, const double2* S
 void eph_sum(hipStream_t s, int64_t n, int64_t shift, const double2* G, double2* S);
 // Sr[t] = i Im(Sg[t] - Sl[t]) / 2
 void eph_retarded(hipStream_t s, int64_t n, const double2* Sl, const double2* Sg, double2* Sr);
+// U, L = the cell blocks (j, j+1) and (j+1, j) of each of the nx slab blocks H
+// (nm x nm, cells of nb orbitals), the rest zero
+void cell_bands(hipStream_t s, int nm, int nb, int nx, const double2* H, double2* U, double2* L);
+// orbital-resolved cell-bond currents of cnt energies (qt_gw_block_current)
+// from C1 = U G^<, C2 = L G^< and the slab-bond matrices cur
+void cell_current(hipStream_t s, int nb, int ns, int nx, int cnt, const double2* C1, const double2* C2,
+                  const double* cur, double* jd);
+// acc[i] += sum_e X[i + n e] w[e] (energies in order: deterministic)
+void energy_sum(hipStream_t s, int64_t n, int cnt, const double* X, const double* w, double* acc);
 
 // out[b] (real n x n) = Re(X1[b]) - Re(X2[b])
 void real_diff(hipStream_t s, int n, int batch, double* out, int64_t sout, ZOp X1, ZOp X2);

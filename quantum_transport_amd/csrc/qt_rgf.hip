@@ -1881,6 +1881,176 @@ int qt_gw_sigma_ephoton(int32_t nm, int32_t length, int32_t nen, int32_t nop, do
     }
 }
 
+// ---------------------------------------------------------------- block currents, collision integrals
+// Weight of energy point e in an energy integral of the reference's
+// terminal-current form (gw_mod.f90:424-428): dE e0 / (2 pi hbar) e0
+// spindeg (A per unit of the integrand), dE = En[1] - En[0] (0 for one
+// energy); times En[e] for the energy current.
+static void current_weights(int32_t nen, const double* En, double spindeg, std::vector<double>& w,
+                            std::vector<double>& we) {
+    const double e0 = 1.6022e-19, hbar = 1.0546e-34, tpi = 3.14159265359 * 2.0;
+    const double dE = nen > 1 ? En[1] - En[0] : 0.0;
+    const double f = dE * e0 / tpi / hbar * e0 * spindeg;
+    w.assign((size_t)nen, f);
+    we.resize((size_t)nen);
+    for (int32_t e = 0; e < nen; ++e) we[(size_t)e] = f * En[e];
+}
+
+int qt_gw_block_current(int32_t nen, int32_t nx, int32_t nb, int32_t ns, const double* En, double spindeg,
+                        const void* Hii, const void* GL, const double* cur, double* jdens, double* tot_cur,
+                        double* tot_ecur) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (nen <= 0 || nx <= 0 || nb <= 0 || ns <= 0 || !En || !Hii || !GL || !cur || !jdens || !tot_cur ||
+        !tot_ecur)
+        return QT_ERR_INVALID;
+    const int64_t nm = (int64_t)nb * ns;
+    if (nm > 4096 || nm * nm * nx > ((int64_t)1 << 40)) return QT_ERR_UNSUPPORTED;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        const int64_t nn = nm * nm, per_e = nn * nx;  // samples of a (nm, nm, nx) array per energy
+        const int64_t jper = (int64_t)nb * nb * ns * nx;  // jdens values per energy
+        const size_t z = sizeof(double2);
+        // energies in chunks: G^<, C1, C2 (complex), cur, jdens, weights per energy;
+        // fixed: H, U, L, kidx and the two totals
+        const size_t per = (3 * z + sizeof(double)) * (size_t)per_e + sizeof(double) * (size_t)jper +
+                           sizeof(int) * (size_t)nx + 2 * sizeof(double) + 2048;
+        const size_t fixed = 3 * z * (size_t)per_e + 2 * sizeof(double) * (size_t)jper + 16384;
+        const size_t budget = workset_budget(c);
+        const int ce = (int)std::min<int64_t>(
+            nen, std::max<int64_t>(1, budget > fixed + per ? (int64_t)((budget - fixed) / per) : 1));
+        c.arena.reserve(fixed + per * (size_t)ce);
+        c.arena.reset();
+        double2* H = c.arena.take<double2>(per_e);
+        double2* U = c.arena.take<double2>(per_e);
+        double2* L = c.arena.take<double2>(per_e);
+        double* acc = c.arena.take<double>(2 * jper);  // tot_cur, tot_ecur
+        double2* g = c.arena.take<double2>(per_e * ce);
+        double2* C1 = c.arena.take<double2>(per_e * ce);
+        double2* C2 = c.arena.take<double2>(per_e * ce);
+        double* cu = c.arena.take<double>(per_e * ce);
+        double* jd = c.arena.take<double>(jper * ce);
+        double* dw = c.arena.take<double>(2 * (int64_t)ce);
+        int* kidx = c.arena.take<int>((int64_t)nx * ce);
+        std::vector<double> w, we;
+        current_weights(nen, En, spindeg, w, we);
+        QT_HIP(hipMemcpyAsync(H, Hii, z * (size_t)per_e, hipMemcpyHostToDevice, c.stream));
+        cell_bands(c.stream, (int)nm, nb, nx, H, U, L);
+        QT_HIP(hipMemsetAsync(acc, 0, sizeof(double) * (size_t)(2 * jper), c.stream));
+        {
+            std::vector<int> k((size_t)nx * ce);
+            for (size_t b = 0; b < k.size(); ++b) k[b] = (int)(b % nx);
+            QT_HIP(hipMemcpyAsync(kidx, k.data(), sizeof(int) * k.size(), hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipStreamSynchronize(c.stream));
+        }
+        for (int e0 = 0; e0 < nen; e0 += ce) {
+            const int cnt = std::min(ce, nen - e0);
+            QT_HIP(hipMemcpyAsync(g, (const double2*)GL + per_e * e0, z * (size_t)(per_e * cnt),
+                                  hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(cu, cur + per_e * e0, sizeof(double) * (size_t)(per_e * cnt),
+                                  hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(dw, w.data() + e0, sizeof(double) * cnt, hipMemcpyHostToDevice, c.stream));
+            QT_HIP(hipMemcpyAsync(dw + ce, we.data() + e0, sizeof(double) * cnt, hipMemcpyHostToDevice, c.stream));
+            if (ns > 1) {  // C1 = U_ix G^<, C2 = L_ix G^< for every member (ix, e)
+                ZGemmArgs a;
+                a.M = a.N = a.K = (int)nm;
+                a.batch = nx * cnt;
+                a.B = ZOp{g, nm, nn, 0};
+                a.ldc = nm;
+                a.sbc = nn;
+                a.kidx = kidx;
+                a.A = ZOp{U, nm, 0, nn};
+                a.C = C1;
+                zgemm(c.stream, 0, 0, a, &c.timer);
+                a.A = ZOp{L, nm, 0, nn};
+                a.C = C2;
+                zgemm(c.stream, 0, 0, a, &c.timer);
+            }
+            cell_current(c.stream, nb, ns, nx, cnt, C1, C2, cu, jd);
+            energy_sum(c.stream, jper, cnt, jd, dw, acc);
+            energy_sum(c.stream, jper, cnt, jd, dw + ce, acc + jper);
+            QT_HIP(hipMemcpyAsync(jdens + jper * e0, jd, sizeof(double) * (size_t)(jper * cnt),
+                                  hipMemcpyDeviceToHost, c.stream));
+            QT_HIP(hipStreamSynchronize(c.stream));
+        }
+        QT_HIP(hipMemcpyAsync(tot_cur, acc, sizeof(double) * (size_t)jper, hipMemcpyDeviceToHost, c.stream));
+        QT_HIP(hipMemcpyAsync(tot_ecur, acc + jper, sizeof(double) * (size_t)jper, hipMemcpyDeviceToHost,
+                              c.stream));
+        QT_HIP(hipStreamSynchronize(c.stream));
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
+int qt_gw_block_collision(int32_t nen, int32_t nx, int32_t nm, const double* En, double spindeg,
+                          const void* Sl, const void* Sg, const void* Gl, const void* Gg, void* Itot,
+                          void* Ispec) {
+    if (!g_ctx) return QT_ERR_NOT_INIT;
+    if (nen <= 0 || nx <= 0 || nm <= 0 || !En || !Sl || !Sg || !Gl || !Gg || !Itot || !Ispec)
+        return QT_ERR_INVALID;
+    if (nm > 4096 || (int64_t)nm * nm * nx > ((int64_t)1 << 40)) return QT_ERR_UNSUPPORTED;
+    Ctx& c = *g_ctx;
+    std::lock_guard<std::mutex> lk(c.mu);
+    try {
+        QT_HIP(hipSetDevice(c.device));
+        const int64_t nn = (int64_t)nm * nm, per_e = nn * nx;
+        const size_t z = sizeof(double2);
+        // energies in chunks: Sigma^<, Sigma^>, G^<, G^>, I (complex) and a weight per energy
+        const size_t per = 5 * z * (size_t)per_e + sizeof(double) + 1024;
+        const size_t fixed = z * (size_t)per_e + 16384;
+        const size_t budget = workset_budget(c);
+        const int ce = (int)std::min<int64_t>(
+            nen, std::max<int64_t>(1, budget > fixed + per ? (int64_t)((budget - fixed) / per) : 1));
+        c.arena.reserve(fixed + per * (size_t)ce);
+        c.arena.reset();
+        double2* acc = c.arena.take<double2>(per_e);
+        double2* a4[4];
+        for (auto& p : a4) p = c.arena.take<double2>(per_e * ce);
+        double2* I = c.arena.take<double2>(per_e * ce);
+        double* dw = c.arena.take<double>(ce);
+        // Itot = sum_e Ispec(e) dE spindeg / (2 pi)
+        const double tpi = 3.14159265359 * 2.0;
+        const double f = (nen > 1 ? En[1] - En[0] : 0.0) * spindeg / tpi;
+        const std::vector<double> w((size_t)ce, f);
+        QT_HIP(hipMemcpyAsync(dw, w.data(), sizeof(double) * ce, hipMemcpyHostToDevice, c.stream));
+        QT_HIP(hipMemsetAsync(acc, 0, z * (size_t)per_e, c.stream));
+        const void* src[4] = {Sl, Sg, Gl, Gg};
+        for (int e0 = 0; e0 < nen; e0 += ce) {
+            const int cnt = std::min(ce, nen - e0);
+            for (int i = 0; i < 4; ++i)
+                QT_HIP(hipMemcpyAsync(a4[i], (const double2*)src[i] + per_e * e0, z * (size_t)(per_e * cnt),
+                                      hipMemcpyHostToDevice, c.stream));
+            // I = Sigma^< G^> - Sigma^> G^< per (slab, energy) block
+            ZGemmArgs a;
+            a.M = a.N = a.K = nm;
+            a.batch = nx * cnt;
+            a.ldc = nm;
+            a.sbc = nn;
+            a.C = I;
+            a.A = ZOp{a4[0], nm, nn, 0};
+            a.B = ZOp{a4[3], nm, nn, 0};
+            zgemm(c.stream, 0, 0, a, &c.timer);
+            a.A = ZOp{a4[1], nm, nn, 0};
+            a.B = ZOp{a4[2], nm, nn, 0};
+            a.alpha = -1.0;
+            a.D = ZOp{I, nm, nn, 0};
+            a.beta = 1.0;
+            zgemm(c.stream, 0, 0, a, &c.timer);
+            energy_sum(c.stream, 2 * per_e, cnt, (const double*)I, dw, (double*)acc);
+            QT_HIP(hipMemcpyAsync((double2*)Ispec + per_e * e0, I, z * (size_t)(per_e * cnt), hipMemcpyDeviceToHost,
+                                  c.stream));
+            QT_HIP(hipStreamSynchronize(c.stream));
+        }
+        QT_HIP(hipMemcpyAsync(Itot, acc, z * (size_t)per_e, hipMemcpyDeviceToHost, c.stream));
+        QT_HIP(hipStreamSynchronize(c.stream));
+        return QT_OK;
+    } catch (const hip_error& e) {
+        return status_of(e);
+    }
+}
+
 // ---------------------------------------------------------------- W solve
 // W^R = V (I - P^R V)^-1 and W^<,> = W^R P^<,> W^R^H (diagonal blocks) for B
 // frequencies at once.  V (Vii, V1i: nm x nm x nx, V1i[i] = V_{i,i-1}) is

@@ -12,13 +12,16 @@
 ! element-major by_blocks arrays (nen, nk, nx, nm*nm), and green_rgf_calc_w
 ! (the W solve called at :597) on the energy-major (nm, nm, nx, nw) blocks,
 ! and calc_sigma_ephoton_monochromatic_ext (:1075-1122) with its argument list.
+! calc_block_current and calc_block_collision keep the argument lists of
+! their call sites (:384, :795); the reference does not define them, the
+! definitions are the library's (qt_gw_block_current, qt_gw_block_collision).
 module gw_mod
     use iso_c_binding
     use qt_capi
     implicit none
     private
     public :: green_RGF_RS, green_rgf_calc_w, gw_polarization, gw_selfenergy, gw_mix, &
-              calc_sigma_ephoton_monochromatic_ext
+              calc_sigma_ephoton_monochromatic_ext, calc_block_current, calc_block_collision
 
     interface
         integer(c_int) function qt_gw_polarization(nen, nky, nkz, nx, nm, ndiag, dEn, spindeg, Gr, Gl, Gg, &
@@ -59,6 +62,24 @@ module gw_mod
             real(c_double), value :: pre_fact, intensity, hw
             type(c_ptr), value :: Mii, Gl, Gg, Sr, Sl, Sg
         end function qt_gw_sigma_ephoton
+
+        integer(c_int) function qt_gw_block_current(nen, nx, nb, ns, En, spindeg, Hii, GL, cur, jdens, &
+                                                    tot_cur, tot_ecur) bind(C, name="qt_gw_block_current")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            integer(c_int32_t), value :: nen, nx, nb, ns
+            type(c_ptr), value :: En
+            real(c_double), value :: spindeg
+            type(c_ptr), value :: Hii, GL, cur, jdens, tot_cur, tot_ecur
+        end function qt_gw_block_current
+
+        integer(c_int) function qt_gw_block_collision(nen, nx, nm, En, spindeg, Sl, Sg, Gl, Gg, Itot, Ispec) &
+            bind(C, name="qt_gw_block_collision")
+            import :: c_int, c_int32_t, c_double, c_ptr
+            integer(c_int32_t), value :: nen, nx, nm
+            type(c_ptr), value :: En
+            real(c_double), value :: spindeg
+            type(c_ptr), value :: Sl, Sg, Gl, Gg, Itot, Ispec
+        end function qt_gw_block_collision
     end interface
 
 contains
@@ -204,6 +225,49 @@ contains
                                           c_loc(Sig_lesser), c_loc(Sig_greater), 0_c_int32_t), &
                       'calc_sigma_ephoton_monochromatic_ext')
     end subroutine calc_sigma_ephoton_monochromatic_ext
+
+    ! Current matrix of every cell bond (cells of nb orbitals, ns per slab) and
+    ! energy, and its energy integrals tot_cur, tot_ecur (A; the energy
+    ! current weighted by E)
+    subroutine calc_block_current(Hii, G_lesser, cur, nen, en, spindeg, nb, ns, nm, nx, tot_cur, tot_ecur, jdens)
+        integer, intent(in) :: nen, nb, ns, nm, nx
+        real(8), intent(in) :: en(nen), spindeg
+        complex(8), intent(in), dimension(nm, nm, nx), target :: Hii
+        complex(8), intent(in), dimension(nm, nm, nx, nen), target :: G_lesser
+        real(8), intent(in), dimension(nm, nm, nx, nen), target :: cur
+        real(8), intent(inout), dimension(nb, nb, nx * ns), target :: tot_cur, tot_ecur
+        real(8), intent(inout), dimension(nb, nb, nx * ns, nen), target :: jdens
+        real(8), target :: e(nen)
+        if (nm /= nb * ns) then
+            write (*, *) 'calc_block_current: nm must be nb*ns'
+            error stop 1
+        end if
+        e = en
+        call qt_ensure_init()
+        call qt_check(qt_gw_block_current(int(nen, c_int32_t), int(nx, c_int32_t), int(nb, c_int32_t), &
+                                          int(ns, c_int32_t), c_loc(e), real(spindeg, c_double), c_loc(Hii), &
+                                          c_loc(G_lesser), c_loc(cur), c_loc(jdens), c_loc(tot_cur), &
+                                          c_loc(tot_ecur)), 'calc_block_current')
+    end subroutine calc_block_current
+
+    ! Collision integrals Ispec = Sig^< G^> - Sig^> G^< per slab block and
+    ! energy, Itot their energy integral (dE spindeg / (2 pi))
+    subroutine calc_block_collision(Sig_lesser, Sig_greater, G_lesser, G_greater, nen, en, spindeg, nm, nx, &
+                                    Itot, Ispec)
+        integer, intent(in) :: nen, nm, nx
+        real(8), intent(in) :: en(nen), spindeg
+        complex(8), intent(in), dimension(nm, nm, nx, nen), target :: Sig_lesser, Sig_greater, G_lesser, &
+                                                                    G_greater
+        complex(8), intent(inout), dimension(nm, nm, nx), target :: Itot
+        complex(8), intent(inout), dimension(nm, nm, nx, nen), target :: Ispec
+        real(8), target :: e(nen)
+        e = en
+        call qt_ensure_init()
+        call qt_check(qt_gw_block_collision(int(nen, c_int32_t), int(nx, c_int32_t), int(nm, c_int32_t), &
+                                            c_loc(e), real(spindeg, c_double), c_loc(Sig_lesser), &
+                                            c_loc(Sig_greater), c_loc(G_lesser), c_loc(G_greater), c_loc(Itot), &
+                                            c_loc(Ispec)), 'calc_block_collision')
+    end subroutine calc_block_collision
 
     ! W^R = V (I - P^R V)^-1, W^<,> = W^R P^<,> W^R^H for nw frequencies in one
     ! call (the reference calls it per energy, gw_mod.f90:592-599); nbc = 0

@@ -10,7 +10,8 @@ module Output
     private
     integer, parameter :: dp = 8
     public :: write_spectrum_summed_over_k, write_spectrum_from_diagonals, write_transmission_spectrum_k, &
-              write_total_current
+              write_total_current, write_spectrum, write_current_spectrum_summed_over_kz, &
+              write_current_summed_over_k
     ! constants of the reference's current formula (static.f90:10,14, fft_mod.f90:19 there)
     real(dp), parameter :: hbar = 1.0546d-34, e0 = 1.6022d-19, tpi = 3.14159265359d0*2.0d0
 
@@ -122,5 +123,92 @@ contains
             sum_tre*dE*e0/tpi/hbar*e0*spindeg/dble(nk)
         close (101)
     end subroutine write_total_current
+
+    ! The spectrum of plain (nb ns, nb ns, length, nen) blocks, the writer the
+    ! GW driver calls for its collision integrals ('gw_Scat', 'eph_Scat',
+    ! gw_mod.f90:803, :934 there) and never defines: the records of
+    ! write_spectrum_summed_over_k for one k point.
+    subroutine write_spectrum(dataset, i, G, nen, en, length, NB, NS, Lx, coeff, append)
+        character(len=*), intent(in) :: dataset
+        integer, intent(in) :: i, nen, length, NB, NS
+        complex(dp), intent(in) :: G(NB*NS, NB*NS, length, nen)
+        real(dp), intent(in) :: Lx, en(nen), coeff(2)
+        logical, intent(in), optional :: append
+        logical :: app
+        integer :: ie, j, c, b, cell
+        complex(dp) :: t
+        app = .false.
+        if (present(append)) app = append
+        call open_spectrum(dataset, i, app)
+        do ie = 1, nen
+            cell = 0
+            do j = 1, length
+                do c = 1, NS
+                    cell = cell + 1
+                    t = (0d0, 0d0)
+                    do b = 1, NB
+                        t = t + G((c - 1)*NB + b, (c - 1)*NB + b, j, ie)
+                    end do
+                    write (11, '(4E18.4)') dble(cell)*Lx, en(ie), dble(t)*coeff(1), aimag(t)*coeff(2)
+                end do
+            end do
+            write (11, '(A)') ''
+        end do
+        close (11)
+    end subroutine write_spectrum
+
+    ! Bond-current spectrum, the file the GW driver names 'gw_Jdens'
+    ! (gw_mod.f90:405-406 there) and never defines: one record '(3E18.4)'
+    ! per cell bond and energy, x of the bond's cell, E and the k-average of
+    ! the trace of the bond's current matrix J(:, :, cell, ie, ik); an empty
+    ! record after each energy.
+    subroutine write_current_spectrum_summed_over_kz(dataset, i, J, nen, en, length, NB, Lx, nk, append)
+        character(len=*), intent(in) :: dataset
+        integer, intent(in) :: i, nen, length, NB, nk
+        real(dp), intent(in) :: J(NB, NB, length, nen, nk), en(nen), Lx
+        logical, intent(in), optional :: append
+        logical :: app
+        integer :: ie, cell, ik, b
+        real(dp) :: t
+        app = .false.
+        if (present(append)) app = append
+        call open_spectrum(dataset, i, app)
+        do ie = 1, nen
+            do cell = 1, length
+                t = 0d0
+                do ik = 1, nk
+                    do b = 1, NB
+                        t = t + J(b, b, cell, ie, ik)
+                    end do
+                end do
+                write (11, '(3E18.4)') dble(cell)*Lx, en(ie), t/dble(nk)
+            end do
+            write (11, '(A)') ''
+        end do
+        close (11)
+    end subroutine write_current_spectrum_summed_over_kz
+
+    ! Energy-integrated bond currents, the files the GW driver names 'gw_I' and
+    ! 'gw_EI' (gw_mod.f90:429-430 there) and never defines: one record
+    ! '(2E18.4)' per cell bond, x of the bond's cell and the k-average of the
+    ! trace of Cur(:, :, cell, ik).
+    subroutine write_current_summed_over_k(dataset, i, Cur, length, NB, Lx, nk)
+        character(len=*), intent(in) :: dataset
+        integer, intent(in) :: i, length, NB, nk
+        real(dp), intent(in) :: Cur(NB, NB, length, nk), Lx
+        integer :: cell, ik, b
+        real(dp) :: t
+        call open_spectrum(dataset, i, .false.)
+        do cell = 1, length
+            t = 0d0
+            do ik = 1, nk
+                do b = 1, NB
+                    t = t + Cur(b, b, cell, ik)
+                end do
+            end do
+            write (11, '(2E18.4)') dble(cell)*Lx, t/dble(nk)
+        end do
+        close (11)
+    end subroutine write_current_summed_over_k
 
 end module Output

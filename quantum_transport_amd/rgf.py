@@ -43,7 +43,8 @@ EXPORTED = ["qt_init", "qt_finish", "qt_status_string", "qt_ham_create", "qt_ham
             "qt_conv1d2_batched", "qt_gw_polarization", "qt_gw_selfenergy", "qt_gw_mix",
             "qt_gw_wsolve_batched", "qt_gw_sigma_ephoton", "qt_invert_batched_noalloc", "qt_lead_create",
             "qt_lead_destroy", "qt_lead_train", "qt_lead_surface", "qt_lead_param_count", "qt_lead_get_params",
-            "qt_lead_set_params", "qt_lead_eval", "qt_graph_slice", "qt_gw_scba_step"]
+            "qt_lead_set_params", "qt_lead_eval", "qt_graph_slice", "qt_gw_scba_step", "qt_gw_block_current",
+            "qt_gw_block_collision"]
 
 
 class QTError(RuntimeError):
@@ -152,6 +153,8 @@ def lib() -> C.CDLL:
         L.qt_gw_wsolve_batched.argtypes = [C.c_int32] * 3 + [C.c_void_p] * 8 + [C.c_int32]
         L.qt_gw_sigma_ephoton.argtypes = [C.c_int32] * 4 + [C.c_double] * 3 + [C.c_void_p] * 6 \
             + [C.c_int32]
+        L.qt_gw_block_current.argtypes = [C.c_int32] * 4 + [C.c_void_p, C.c_double] + [C.c_void_p] * 6
+        L.qt_gw_block_collision.argtypes = [C.c_int32] * 3 + [C.c_void_p, C.c_double] + [C.c_void_p] * 6
         L.qt_graph_slice.argtypes = [C.c_int32, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                      C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]
         L.qt_gw_scba_step.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_void_p, C.c_int32,
@@ -556,6 +559,51 @@ def gw_sigma_ephoton(Mii, G_lesser, G_greater, nop: int, pre_fact: float, intens
     check(lib().qt_gw_sigma_ephoton(nm, length, nen, nop, pre_fact, intensity, hw, M.ctypes.data,
                                     Gl.ctypes.data, Gg.ctypes.data, *(x.ctypes.data for x in S), 0))
     return tuple(S)
+
+
+def gw_block_current(Hii, G_lesser, cur, En, spindeg: float, nb: int, ns: int):
+    """Cell-bond current spectra and their energy integrals (the role of
+    calc_block_current, gw_mod.f90:384; definition in qt_gw_block_current)
+    on the device.  Hii: (nm, nm, nx); G_lesser: (nm, nm, nx, nen) diagonal
+    blocks; cur: (nm, nm, nx, nen) slab-bond matrices of green_RGF_RS;
+    nm = nb ns.  Returns jdens (nb, nb, nx ns, nen), tot_cur and tot_ecur
+    (nb, nb, nx ns), Fortran-ordered."""
+    init()
+    H = np.asfortranarray(Hii, dtype=np.complex128)
+    GL = np.asfortranarray(G_lesser, dtype=np.complex128)
+    cu = np.asfortranarray(cur, dtype=np.float64)
+    E = np.ascontiguousarray(En, dtype=np.float64)
+    if H.ndim != 3 or GL.ndim != 4 or cu.shape != GL.shape or GL.shape[:3] != H.shape:
+        raise ValueError("Hii (nm, nm, nx), G_lesser and cur (nm, nm, nx, nen)")
+    nm, _, nx, nen = GL.shape
+    if nm != nb * ns or len(E) != nen:
+        raise ValueError("nm must be nb * ns and En hold nen energies")
+    jd = np.zeros((nb, nb, nx * ns, nen), order="F")
+    tc = np.zeros((nb, nb, nx * ns), order="F")
+    te = np.zeros((nb, nb, nx * ns), order="F")
+    check(lib().qt_gw_block_current(nen, nx, nb, ns, E.ctypes.data, spindeg, H.ctypes.data, GL.ctypes.data,
+                                    cu.ctypes.data, jd.ctypes.data, tc.ctypes.data, te.ctypes.data))
+    return jd, tc, te
+
+
+def gw_block_collision(Sigma_lesser, Sigma_greater, G_lesser, G_greater, En, spindeg: float):
+    """Collision integrals (the role of calc_block_collision, gw_mod.f90:795;
+    definition in qt_gw_block_collision) on the device.  All four inputs
+    (nm, nm, nx, nen) diagonal blocks.  Returns Itot (nm, nm, nx) and Ispec
+    (nm, nm, nx, nen), Fortran-ordered."""
+    init()
+    A = [np.asfortranarray(x, dtype=np.complex128) for x in (Sigma_lesser, Sigma_greater, G_lesser, G_greater)]
+    E = np.ascontiguousarray(En, dtype=np.float64)
+    if A[0].ndim != 4 or any(x.shape != A[0].shape for x in A) or A[0].shape[0] != A[0].shape[1]:
+        raise ValueError("inputs must be (nm, nm, nx, nen)")
+    nm, _, nx, nen = A[0].shape
+    if len(E) != nen:
+        raise ValueError("En must hold nen energies")
+    Ispec = np.zeros((nm, nm, nx, nen), dtype=np.complex128, order="F")
+    Itot = np.zeros((nm, nm, nx), dtype=np.complex128, order="F")
+    check(lib().qt_gw_block_collision(nen, nx, nm, E.ctypes.data, spindeg, *(x.ctypes.data for x in A),
+                                      Itot.ctypes.data, Ispec.ctypes.data))
+    return Itot, Ispec
 
 
 def gw_mix(S, Snew, alpha: float) -> np.ndarray:

@@ -308,3 +308,77 @@ def test_fortran_per_point_refuses_nonuniform_potential(tmp_path):
     r, got = _point_run(tmp_path, [6] * 3, dev.Hii[0], dev.H1i[0], 0.3, 0.0, -0.1, 300.0, 300.0, mode=1)
     assert r.returncode != 0 and got is None
     assert "mul must be a uniform matrix" in r.stdout + r.stderr
+
+
+CURRENT = os.path.join(ROOT, "tests", "fortran", "_build", "current_check")
+
+
+def _text_rows(path, ncol):
+    rows = [ln.split() for ln in open(path) if ln.strip()]
+    assert all(len(r) == ncol for r in rows)
+    return np.array([[float(v) for v in r] for r in rows])
+
+
+@pytest.mark.gpu
+def test_fortran_block_current_and_collision(tmp_path):
+    """gw_mod::calc_block_current (per k point, as gw_mod.f90:384 calls it)
+    and calc_block_collision (:795) from a Fortran driver, against the
+    oracle; the gw_Jdens, gw_I, gw_EI and gw_Scat files the driver writes
+    (:405-406, :429-430, :803) against the same values at their 4 digits."""
+    from oracle import current as cur_oracle
+
+    rng = np.random.default_rng(21)
+    nb, ns, nx, nen, nk = 2, 2, 3, 5, 2
+    nm = nb * ns
+    spindeg, Lx = 2.0, 0.5
+    en = np.linspace(-0.4, 0.6, nen)
+    cr = lambda *sh: rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+    Hii, GL, cu = cr(nm, nm, nx, nk), cr(nm, nm, nx, nen, nk), rng.standard_normal((nm, nm, nx, nen, nk))
+    S4 = [cr(nm, nm, nx, nen) for _ in range(4)]
+    with open(tmp_path / "current_in.bin", "wb") as fh:
+        fh.write(np.array([nb, ns, nx, nen, nk], np.int32).tobytes())
+        fh.write(np.concatenate([[spindeg, Lx], en]).astype(np.float64).tobytes())
+        for a in [Hii, GL, cu] + S4:
+            fh.write(np.asfortranarray(a).tobytes(order="F"))
+    print("fortran block current / collision driver", flush=True)
+    r = subprocess.run([CURRENT], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(tmp_path / "current_out.bin", dtype=np.uint8)
+    pos = 0
+
+    def take(shape, dtype):
+        nonlocal pos
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        a = raw[pos:pos + n].view(dtype).reshape(shape, order="F")
+        pos += n
+        return a
+
+    nc = nx * ns
+    jd = take((nb, nb, nc, nen, nk), np.float64)
+    tc = take((nb, nb, nc, nk), np.float64)
+    te = take((nb, nb, nc, nk), np.float64)
+    It = take((nm, nm, nx), np.complex128)
+    Isp = take((nm, nm, nx, nen), np.complex128)
+    assert pos == raw.size
+    for ik in range(nk):
+        ref = cur_oracle.block_current(Hii[..., ik], GL[..., ik], cu[..., ik], en, spindeg, nb, ns)
+        for g, rf in zip((jd[..., ik], tc[..., ik], te[..., ik]), ref):
+            assert np.abs(g - rf).max() <= 1e-12 * np.abs(rf).max()
+    rI, rS = cur_oracle.block_collision(*S4, en, spindeg)
+    assert np.abs(It - rI).max() <= 1e-12 * np.abs(rI).max()
+    assert np.abs(Isp - rS).max() <= 1e-12 * np.abs(rS).max()
+    # the written records
+    x = (np.arange(nc) + 1) * Lx
+    J = _text_rows(tmp_path / "gw_Jdens0001.dat", 3)
+    flux = np.einsum("bbcek->ce", jd) / nk
+    want = np.stack([np.tile(x, nen), np.repeat(en, nc), flux.T.ravel()], 1)
+    np.testing.assert_allclose(J, want, rtol=1e-3, atol=1e-4 * np.abs(flux).max())
+    for name, arr in (("gw_I", tc), ("gw_EI", te)):
+        rec = _text_rows(tmp_path / f"{name}0001.dat", 2)
+        tot = np.einsum("bbck->c", arr) / nk
+        np.testing.assert_allclose(rec, np.stack([x, tot], 1), rtol=1e-3, atol=1e-4 * np.abs(tot).max())
+    S = _text_rows(tmp_path / "gw_Scat0001.dat", 4)
+    cells = np.array([[np.trace(Isp[c * nb:(c + 1) * nb, c * nb:(c + 1) * nb, s, e]) for s in range(nx)
+                       for c in range(ns)] for e in range(nen)]) / nk
+    want = np.stack([np.tile(x, nen), np.repeat(en, nc), cells.real.ravel(), cells.imag.ravel()], 1)
+    np.testing.assert_allclose(S, want, rtol=1e-3, atol=1e-4 * np.abs(cells).max())
