@@ -65,7 +65,8 @@ enum {
                               maximum (6 base-128 digits); inversions, leads and
                               elementwise work in complex128 (no reference
                               counterpart).  Measured: spectra within ~1e-7 and
-                              contact currents within ~1e-4 of QT_PREC_FP64. */
+                              contact currents within ~1e-6 of QT_PREC_FP64
+                              (relative to each quantity's largest value). */
 
 /* option flags (qt_rgf_params.flags) */
 #define QT_F_COUPLING_FIX 0x1   /* forward variant: the second pass couples slab
@@ -375,12 +376,14 @@ int qt_gw_wsolve_batched(int32_t nm, int32_t nx, int32_t nw, const void* Vii, co
  *     S <- S + alpha (Sigma_new - S) (qt_gw_mix).
  * S[3], G[3], P[3]: element-major (nm*nm, nx, 1, nen) complex128 device
  * arrays (R, <, >): S read and updated, G and P written.  tr, tre (host, nen
- * each, may be NULL): the contact currents of the G pass.  Needs device
+ * each, may be NULL): the contact currents of the G pass; cur (host, may be
+ * NULL): green_RGF_RS's slab-bond current matrices of the G pass, real
+ * (nm, nm, nx, nen) (the cur blocks of qt_rgf_outputs).  Needs device
  * scratch of six such arrays.  QT_ERR_SINGULAR when a block inversion met a
  * zero pivot (the iteration still completes). */
 int qt_gw_scba_step(qt_ham* h, const qt_rgf_params* p, int32_t nen, const double* En, int32_t ndiag,
                     double spindeg, double alpha, const void* Vii, const void* V1i, void* const* S,
-                    void* const* G, void* const* P, double* tr, double* tre);
+                    void* const* G, void* const* P, double* tr, double* tre, double* cur);
 
 /* e-photon self-energies in the monochromatic assumption, replaces
  * calc_sigma_ephoton_monochromatic_ext (gw_mod.f90:1075-1122):

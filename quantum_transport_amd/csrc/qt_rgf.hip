@@ -2337,7 +2337,7 @@ int qt_gw_wsolve_batched(int32_t nm, int32_t nx, int32_t nw, const void* Vii, co
 // stage the kernels of its own entry point.
 int qt_gw_scba_step(qt_ham* h, const qt_rgf_params* p, int32_t nen, const double* En, int32_t ndiag,
                     double spindeg, double alpha, const void* Vii, const void* V1i, void* const* S,
-                    void* const* G, void* const* P, double* tr, double* tre) {
+                    void* const* G, void* const* P, double* tr, double* tre, double* cur) {
     if (!g_ctx) return QT_ERR_NOT_INIT;
     if (!h || !p || !En || !Vii || !S || !G || !P || nen < 2 || ndiag < 0) return QT_ERR_INVALID;
     if (p->variant != QT_RGF_GW || p->precision != QT_PREC_FP64) return QT_ERR_UNSUPPORTED;
@@ -2372,6 +2372,9 @@ int qt_gw_scba_step(qt_ham* h, const qt_rgf_params* p, int32_t nen, const double
             std::vector<int> hk(2 * B, 0), hinfo(B);
             std::vector<int64_t> doff(nx);
             for (int s = 0; s < nx; ++s) doff[s] = (int64_t)s * nm;
+            // host blocks of the slab-bond current matrices, (nm, nm, nx, nen)
+            std::vector<void*> curp(cur ? (size_t)nen * nx : 0);
+            for (size_t i = 0; i < curp.size(); ++i) curp[i] = cur + (int64_t)i * nn;
             for (int p0 = 0; p0 < nen; p0 += B) {
                 const int nb = std::min(B, nen - p0);
                 for (int b = 0; b < B; ++b) {
@@ -2432,6 +2435,7 @@ int qt_gw_scba_step(qt_ham* h, const qt_rgf_params* p, int32_t nen, const double
                 sv.full.nb = nb;
                 sv.full.nen = nen;
                 for (int i = 0; i < 3; ++i) sv.full.em[i] = Gd[i];
+                if (cur) sv.full.cur = curp.data();
                 sv.run(tm);
                 QT_HIP(hipMemcpyAsync(htr.data(), w.tr, sizeof(double) * B, hipMemcpyDeviceToHost, c.stream));
                 QT_HIP(hipMemcpyAsync(htre.data(), w.tre, sizeof(double) * B, hipMemcpyDeviceToHost, c.stream));

@@ -15,13 +15,16 @@
 ! calc_block_current and calc_block_collision keep the argument lists of
 ! their call sites (:384, :795); the reference does not define them, the
 ! definitions are the library's (qt_gw_block_current, qt_gw_block_collision).
+! gw_ephoton_3d_ijs (:78-1060) keeps its argument list and runs the SCBA
+! loop with the self-energies, G and P on the device (qt_gw_scba_step) for
+! one process, one k point and no e-photon coupling.
 module gw_mod
     use iso_c_binding
     use qt_capi
     implicit none
     private
     public :: green_RGF_RS, green_rgf_calc_w, gw_polarization, gw_selfenergy, gw_mix, &
-              calc_sigma_ephoton_monochromatic_ext, calc_block_current, calc_block_collision
+              calc_sigma_ephoton_monochromatic_ext, calc_block_current, calc_block_collision, gw_ephoton_3d_ijs
 
     interface
         integer(c_int) function qt_gw_polarization(nen, nky, nkz, nx, nm, ndiag, dEn, spindeg, Gr, Gl, Gg, &
@@ -71,6 +74,40 @@ module gw_mod
             real(c_double), value :: spindeg
             type(c_ptr), value :: Hii, GL, cur, jdens, tot_cur, tot_ecur
         end function qt_gw_block_current
+
+        integer(c_int) function qt_gw_scba_step(h, prm, nen, En, ndiag, spindeg, alpha, Vii, V1i, S, G, Pol, &
+                                                tr, tre, cur) bind(C, name="qt_gw_scba_step")
+            import :: c_int, c_int32_t, c_double, c_ptr, qt_rgf_params
+            type(c_ptr), value :: h
+            type(qt_rgf_params), intent(in) :: prm
+            integer(c_int32_t), value :: nen, ndiag
+            type(c_ptr), value :: En
+            real(c_double), value :: spindeg, alpha
+            type(c_ptr), value :: Vii, V1i, S, G, Pol, tr, tre, cur
+        end function qt_gw_scba_step
+
+        integer(c_int) function qt_device_alloc(bytes, out) bind(C, name="qt_device_alloc")
+            import :: c_int, c_size_t, c_ptr
+            integer(c_size_t), value :: bytes
+            type(c_ptr), intent(out) :: out
+        end function qt_device_alloc
+
+        integer(c_int) function qt_device_free(ptr) bind(C, name="qt_device_free")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ptr
+        end function qt_device_free
+
+        integer(c_int) function qt_memcpy_h2d(dst, src, bytes) bind(C, name="qt_memcpy_h2d")
+            import :: c_int, c_size_t, c_ptr
+            type(c_ptr), value :: dst, src
+            integer(c_size_t), value :: bytes
+        end function qt_memcpy_h2d
+
+        integer(c_int) function qt_memcpy_d2h(dst, src, bytes) bind(C, name="qt_memcpy_d2h")
+            import :: c_int, c_size_t, c_ptr
+            type(c_ptr), value :: dst, src
+            integer(c_size_t), value :: bytes
+        end function qt_memcpy_d2h
 
         integer(c_int) function qt_gw_block_collision(nen, nx, nm, En, spindeg, Sl, Sg, Gl, Gg, Itot, Ispec) &
             bind(C, name="qt_gw_block_collision")
@@ -268,6 +305,141 @@ contains
                                             c_loc(Sig_greater), c_loc(G_lesser), c_loc(G_greater), c_loc(Itot), &
                                             c_loc(Ispec)), 'calc_block_collision')
     end subroutine calc_block_collision
+
+    ! The SCBA loop of gw_ephoton_3d_ijs (gw_mod.f90:78-1060 there) with its
+    ! argument list, iterations iter = 0 .. niter: G by green_RGF_RS with the
+    ! current self-energies, P, W, Sigma and mixing on the device
+    ! (qt_gw_scba_step; Sigma, G and P stay there between iterations), then the
+    ! reference's outputs of the iteration: gw_ldos, gw_Jdens, gw_trL, gw_trR,
+    ! Id_iteration.dat, gw_I, gw_EI and gw_Scat (the collision integrals of the
+    ! mixed self-energy).  Provided for one process (comm_size = 1,
+    ! local_NE = nen), one k point (nky = nkz = 1), slabs of NB*NS = nm
+    ! orbitals, and without the e-photon coupling (intensity = 0,
+    ! labs = .false.); Pii, P1i, polarization and hw are then not read.
+    subroutine gw_ephoton_3d_ijs(alpha_mix, niter, NB, NS, nm, nx, nky, nkz, ndiag, Lx, nen, en, temp, mu, &
+                                 Hii, H1i, Vii, V1i, spindeg, Pii, P1i, polarization, intensity, hw, labs, &
+                                 comm_size, comm_rank, local_NE, first_local_energy)
+        use Output, only: write_spectrum_from_diagonals, write_transmission_spectrum_k, write_total_current, &
+                          write_current_spectrum_summed_over_kz, write_current_summed_over_k, write_spectrum
+        integer(kind=4), intent(in) :: comm_size, comm_rank, local_NE, first_local_energy
+        integer, intent(in) :: nm, nx, nen, niter, NB, NS, ndiag, nky, nkz
+        real(8), intent(in) :: en(nen), temp(2), mu(2), Lx, alpha_mix, spindeg
+        complex(8), intent(in), dimension(nm, nm, nx, nky*nkz), target :: Hii, H1i, Vii, V1i
+        real(8), intent(in) :: polarization(3), intensity, hw
+        logical, intent(in) :: labs
+        complex(8), intent(in) :: Pii(nm, nm, 3, nx, nky*nkz), P1i(nm, nm, 3, nx, nky*nkz)
+
+        integer(c_int32_t), target :: nmv(nx)
+        type(c_ptr), target :: hp(nx), cp(nx + 1), Sd(3), Gd(3), Pd(3)
+        type(c_ptr) :: ham, dV(2)
+        type(qt_rgf_params) :: p
+        real(8), target :: e(nen), tr(1, nen), tre(1, nen)
+        real(8), allocatable, target :: cur(:, :, :, :), jdens(:, :, :, :), tot_cur(:, :, :), tot_ecur(:, :, :)
+        complex(8), allocatable, target :: stage(:, :, :), blk(:, :, :, :, :), D(:, :, :), Itot(:, :, :), &
+                                           Ispec(:, :, :, :)
+        integer(c_size_t) :: arr, vbytes
+        integer(c_int) :: rc
+        integer :: s, i, iter, ie, q
+
+        if (comm_size /= 1 .or. local_NE /= nen .or. nky*nkz /= 1 .or. labs .or. intensity /= 0d0 .or. &
+            nm /= NB*NS) then
+            write (*, *) 'gw_ephoton_3d_ijs: provided for comm_size = 1, local_NE = nen, nky = nkz = 1, ', &
+                'nm = NB*NS, intensity = 0 and labs = .false.'
+            error stop 1
+        end if
+        call qt_ensure_init()
+        e = en
+        do s = 1, nx
+            nmv(s) = nm
+            hp(s) = c_loc(Hii(1, 1, s, 1))
+            cp(s) = c_loc(H1i(1, 1, s, 1))
+        end do
+        cp(nx + 1) = c_loc(H1i(1, 1, nx, 1))  ! the right lead couples through H1i(nx)
+        call qt_check(qt_ham_create(int(nx, c_int32_t), nmv, 1_c_int32_t, c_loc(hp), c_loc(cp), c_null_ptr, ham), &
+                      'qt_ham_create')
+        p%variant = QT_RGF_GW
+        p%flags = QT_F_LEAD_RESET
+        p%lead_nmax = 100
+        p%lead_eta = 1d-3
+        p%mul = mu(1)
+        p%mur = mu(2)
+        p%templ = temp(1)
+        p%tempr = temp(2)
+        ! device arrays: Sigma (zero at the start, gw_mod.f90:330-332 there), G, P,
+        ! element-major: series (ij, ix) of nen energies, ij = row + col*nm, i.e.
+        ! stage(ie, ix, ij) here; blk holds blocks by energy, (nm, nm, nx, nen)
+        arr = int(16, c_size_t)*int(nm, c_size_t)*int(nm, c_size_t)*int(nx, c_size_t)*int(nen, c_size_t)
+        vbytes = int(16, c_size_t)*int(nm, c_size_t)*int(nm, c_size_t)*int(nx, c_size_t)
+        allocate (stage(nen, nx, nm*nm), blk(nm, nm, nx, nen, 4), D(nx*nm, 1, nen), cur(nm, nm, nx, nen))
+        allocate (jdens(NB, NB, nx*NS, nen), tot_cur(NB, NB, nx*NS), tot_ecur(NB, NB, nx*NS))
+        allocate (Itot(nm, nm, nx), Ispec(nm, nm, nx, nen))
+        stage = (0d0, 0d0)
+        do i = 1, 3
+            call qt_check(qt_device_alloc(arr, Sd(i)), 'qt_device_alloc')
+            call qt_check(qt_device_alloc(arr, Gd(i)), 'qt_device_alloc')
+            call qt_check(qt_device_alloc(arr, Pd(i)), 'qt_device_alloc')
+            call qt_check(qt_memcpy_h2d(Sd(i), c_loc(stage), arr), 'qt_memcpy_h2d')
+        end do
+        call qt_check(qt_device_alloc(vbytes, dV(1)), 'qt_device_alloc')
+        call qt_check(qt_device_alloc(vbytes, dV(2)), 'qt_device_alloc')
+        call qt_check(qt_memcpy_h2d(dV(1), c_loc(Vii), vbytes), 'qt_memcpy_h2d')
+        call qt_check(qt_memcpy_h2d(dV(2), c_loc(V1i), vbytes), 'qt_memcpy_h2d')
+
+        do iter = 0, niter
+            rc = qt_gw_scba_step(ham, p, int(nen, c_int32_t), c_loc(e), int(ndiag, c_int32_t), &
+                                 real(spindeg, c_double), real(alpha_mix, c_double), dV(1), dV(2), c_loc(Sd), &
+                                 c_loc(Gd), c_loc(Pd), c_loc(tr), c_loc(tre), c_loc(cur))
+            if (rc == QT_ERR_SINGULAR) then
+                write (*, *) 'gw_ephoton_3d_ijs: singular block inversion (block zeroed), iter =', iter
+            else
+                call qt_check(rc, 'qt_gw_scba_step')
+            end if
+            ! G^R of this iteration: the diagonals of gw_ldos
+            call qt_check(qt_memcpy_d2h(c_loc(stage), Gd(1), arr), 'qt_memcpy_d2h')
+            do ie = 1, nen
+                do s = 1, nx
+                    do i = 1, nm
+                        D((s - 1)*nm + i, 1, ie) = stage(ie, s, i + (i - 1)*nm)
+                    end do
+                end do
+            end do
+            call write_spectrum_from_diagonals('gw_ldos', iter, D, nen, en, 1, nx, NB, NS, Lx, (/1.0d0, -2.0d0/), &
+                                               .false.)
+            ! blocks by energy of G^<, G^> and of the mixed Sigma^<, Sigma^>
+            do q = 1, 4
+                if (q <= 2) then
+                    call qt_check(qt_memcpy_d2h(c_loc(stage), Gd(q + 1), arr), 'qt_memcpy_d2h')
+                else
+                    call qt_check(qt_memcpy_d2h(c_loc(stage), Sd(q - 1), arr), 'qt_memcpy_d2h')
+                end if
+                do ie = 1, nen
+                    do s = 1, nx
+                        blk(:, :, s, ie, q) = reshape(stage(ie, s, :), (/nm, nm/))
+                    end do
+                end do
+            end do
+            call calc_block_current(Hii(:, :, :, 1), blk(:, :, :, :, 1), cur, nen, en, spindeg, NB, NS, nm, nx, &
+                                    tot_cur, tot_ecur, jdens)
+            call write_current_spectrum_summed_over_kz('gw_Jdens', iter, jdens, nen, en, nx*NS, NB, Lx, 1, .false.)
+            call write_transmission_spectrum_k('gw_trL', iter, tr, nen, en, 1, .false.)
+            call write_transmission_spectrum_k('gw_trR', iter, tre, nen, en, 1, .false.)
+            call write_total_current('Id_iteration.dat', iter, sum(tr), sum(tre), en(2) - en(1), spindeg, 1)
+            call write_current_summed_over_k('gw_I', iter, tot_cur, nx*NS, NB, Lx, 1)
+            call write_current_summed_over_k('gw_EI', iter, tot_ecur, nx*NS, NB, Lx, 1)
+            call calc_block_collision(blk(:, :, :, :, 3), blk(:, :, :, :, 4), blk(:, :, :, :, 1), &
+                                      blk(:, :, :, :, 2), nen, en, spindeg, nm, nx, Itot, Ispec)
+            call write_spectrum('gw_Scat', iter, Ispec, nen, en, nx, NB, NS, Lx, (/1.0d0, 1.0d0/), .false.)
+        end do
+
+        do i = 1, 3
+            call qt_check(qt_device_free(Sd(i)), 'qt_device_free')
+            call qt_check(qt_device_free(Gd(i)), 'qt_device_free')
+            call qt_check(qt_device_free(Pd(i)), 'qt_device_free')
+        end do
+        call qt_check(qt_device_free(dV(1)), 'qt_device_free')
+        call qt_check(qt_device_free(dV(2)), 'qt_device_free')
+        call qt_check(qt_ham_destroy(ham), 'qt_ham_destroy')
+    end subroutine gw_ephoton_3d_ijs
 
     ! W^R = V (I - P^R V)^-1, W^<,> = W^R P^<,> W^R^H for nw frequencies in one
     ! call (the reference calls it per energy, gw_mod.f90:592-599); nbc = 0

@@ -159,7 +159,7 @@ def lib() -> C.CDLL:
                                      C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]
         L.qt_gw_scba_step.argtypes = [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_void_p, C.c_int32,
                                       C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                      C.c_void_p, C.c_void_p, C.c_void_p]
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -758,10 +758,12 @@ class GWState:
 
 
 def gw_scba_step(H: "Hamiltonian", state: GWState, En, mu, TEMP, ndiag: int, spindeg: float = 2.0,
-                 alpha: float = 0.5, lead_eta: float = 1e-3, lead_nmax: int = 100):
+                 alpha: float = 0.5, lead_eta: float = 1e-3, lead_nmax: int = 100, cur=None):
     """One GW-SCBA iteration on the device (qt_gw_scba_step): G by the
     green_RGF_RS recursion with the state's Sigma, P, W, Sigma_new, mixing.
-    Returns (tr, tre, status)."""
+    cur: None or a Fortran-ordered float64 (nm, nm, nx, nen) array that
+    receives the slab-bond current matrices of the G pass.  Returns (tr, tre,
+    status)."""
     En = np.ascontiguousarray(En, dtype=np.float64)
     p = make_params("gw", mu[0], mu[1], TEMP[0], TEMP[1], lead_eta=lead_eta, lead_nmax=lead_nmax,
                     lead_reset=True)
@@ -769,9 +771,12 @@ def gw_scba_step(H: "Hamiltonian", state: GWState, En, mu, TEMP, ndiag: int, spi
     tre = np.zeros(len(En))
     ptrs = lambda bufs: (C.c_void_p * 3)(*(b.ptr.value for b in bufs))
     S, G, P = ptrs(state.S), ptrs(state.G), ptrs(state.P)
+    if cur is not None and (cur.dtype != np.float64 or not cur.flags.f_contiguous
+                            or cur.shape != (state.nm, state.nm, state.nx, len(En))):
+        raise ValueError("cur must be a Fortran-ordered float64 (nm, nm, nx, nen) array")
     st = lib().qt_gw_scba_step(H.handle, C.byref(p), len(En), En.ctypes.data_as(C.c_void_p), ndiag, spindeg,
                                alpha, state.V[0].ptr, state.V[1].ptr, S, G, P, tr.ctypes.data_as(C.c_void_p),
-                               tre.ctypes.data_as(C.c_void_p))
+                               tre.ctypes.data_as(C.c_void_p), None if cur is None else cur.ctypes.data)
     if st != QT_ERR_SINGULAR:
         check(st)
     return tr, tre, st

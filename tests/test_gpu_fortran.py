@@ -382,3 +382,82 @@ def test_fortran_block_current_and_collision(tmp_path):
                        for c in range(ns)] for e in range(nen)]) / nk
     want = np.stack([np.tile(x, nen), np.repeat(en, nc), cells.real.ravel(), cells.imag.ravel()], 1)
     np.testing.assert_allclose(S, want, rtol=1e-3, atol=1e-4 * np.abs(cells).max())
+
+
+SCBA = os.path.join(ROOT, "tests", "fortran", "_build", "scba_check")
+
+
+@pytest.mark.gpu
+def test_fortran_gw_ephoton_3d_ijs_loop(tmp_path):
+    """gw_mod::gw_ephoton_3d_ijs with its argument list (gw_mod.f90:78-89),
+    iterations 0..niter on the device, against the same iterations through
+    rgf.gw_scba_step and the oracle's current and collision definitions:
+    every record of gw_ldos, gw_Jdens, gw_trL, gw_trR, gw_I, gw_EI, gw_Scat
+    and Id_iteration.dat at the files' printed digits."""
+    from oracle import current as cur_oracle
+    from quantum_transport_amd import hamiltonian as hm
+    from quantum_transport_amd import rgf
+
+    NB, NS, nx, nen, niter, ndiag = 2, 2, 3, 32, 1, 2
+    nm = NB * NS
+    alpha, spindeg, Lx = 0.5, 2.0, 0.3
+    temp, mu = [300.0, 300.0], [0.2, -0.2]
+    rng = np.random.default_rng(31)
+    _, dev = hm.synthetic_device(nm, nx, seed=31, coupling_density=0.4)
+    Hii = np.asfortranarray(np.stack(dev.Hii[0], axis=2))
+    H1i = np.asfortranarray(np.stack(dev.H1i[0][:nx], axis=2))
+    cr = lambda *sh, s=1.0: s * (rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+    Vii = cr(nm, nm, nx, s=0.05)
+    Vii = np.asfortranarray((Vii + Vii.conj().transpose(1, 0, 2)) / 2)
+    V1i = np.asfortranarray(cr(nm, nm, nx, s=0.02))
+    E = hm.energy_grid(-2.0, 2.0, nen)
+    with open(tmp_path / "scba_in.bin", "wb") as fh:
+        fh.write(np.array([NB, NS, nx, nen, niter, ndiag], np.int32).tobytes())
+        fh.write(np.concatenate([[alpha, spindeg, Lx], temp, mu, E]).astype(np.float64).tobytes())
+        for a in (Hii, H1i, Vii, V1i):
+            fh.write(np.asfortranarray(a).tobytes(order="F"))
+    print("fortran gw_ephoton_3d_ijs driver", flush=True)
+    r = subprocess.run([SCBA], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    rgf.init(0)
+    H = rgf.Hamiltonian([[Hii[:, :, s] for s in range(nx)]],
+                        [[H1i[:, :, s] for s in range(nx)] + [H1i[:, :, nx - 1]]])
+    st = rgf.GWState(nm, nx, nen, Vii, V1i)
+    nc = nx * NS
+    x = (np.arange(nc) + 1) * Lx
+    close = lambda got, want: np.testing.assert_allclose(got, want, rtol=2e-3, atol=2e-4 * np.abs(want).max())
+    ids = []
+    try:
+        for it in range(niter + 1):
+            cur = np.zeros((nm, nm, nx, nen), order="F")
+            tr, tre, status = rgf.gw_scba_step(H, st, E, mu, temp, ndiag, spindeg, alpha, cur=cur)
+            assert status == 0
+            G = [g[:, :, 0, :].reshape(nm, nm, nx, nen, order="F") for g in st.host("G")]
+            S = [g[:, :, 0, :].reshape(nm, nm, nx, nen, order="F") for g in st.host("S")]
+            tag = f"{it:04d}"
+            print("iteration", tag, flush=True)
+            close(_text_rows(tmp_path / f"gw_trL{tag}.dat", 2), np.stack([E, tr], 1))
+            close(_text_rows(tmp_path / f"gw_trR{tag}.dat", 2), np.stack([E, tre], 1))
+            # gw_ldos: per energy and cell, -2 Im of the cell's trace of G^R
+            cells = np.array([[np.trace(G[0][c * NB:(c + 1) * NB, c * NB:(c + 1) * NB, s, e]) for s in range(nx)
+                               for c in range(NS)] for e in range(nen)])
+            want = np.stack([np.tile(x, nen), np.repeat(E, nc), cells.real.ravel(), -2 * cells.imag.ravel()], 1)
+            close(_text_rows(tmp_path / f"gw_ldos{tag}.dat", 4), want)
+            jd, tc, te = cur_oracle.block_current(Hii, G[1], cur, E, spindeg, NB, NS)
+            flux = np.einsum("bbce->ce", jd)
+            close(_text_rows(tmp_path / f"gw_Jdens{tag}.dat", 3),
+                  np.stack([np.tile(x, nen), np.repeat(E, nc), flux.T.ravel()], 1))
+            for name, arr in (("gw_I", tc), ("gw_EI", te)):
+                close(_text_rows(tmp_path / f"{name}{tag}.dat", 2), np.stack([x, np.einsum("bbc->c", arr)], 1))
+            _, Isp = cur_oracle.block_collision(S[1], S[2], G[1], G[2], E, spindeg)
+            cells = np.array([[np.trace(Isp[c * NB:(c + 1) * NB, c * NB:(c + 1) * NB, s, e]) for s in range(nx)
+                               for c in range(NS)] for e in range(nen)])
+            close(_text_rows(tmp_path / f"gw_Scat{tag}.dat", 4),
+                  np.stack([np.tile(x, nen), np.repeat(E, nc), cells.real.ravel(), cells.imag.ravel()], 1))
+            w = (E[1] - E[0]) * 1.6022e-19 / (2 * 3.14159265359) / 1.0546e-34 * 1.6022e-19 * spindeg
+            ids.append([it, tr.sum() * w, tre.sum() * w])
+    finally:
+        st.free()
+        H.close()
+    close(_text_rows(tmp_path / "Id_iteration.dat", 3), np.array(ids))

@@ -54,6 +54,7 @@ def _scba(backend, kind, Hii, H1i, Vii, V1i, E, iters, ndiag, alpha):
     for _ in range(iters):
         SR, SL = _blocks(S[0], nm), _blocks(S[1], nm)
         G = [np.zeros((nm, nm, nx, nen), complex) for _ in range(3)]
+        Cur = np.zeros((nm, nm, nx, nen))
         for e in range(nen):
             sr = np.asfortranarray(SR[..., e])
             sl = np.asfortranarray(SL[..., e])
@@ -64,6 +65,7 @@ def _scba(backend, kind, Hii, H1i, Vii, V1i, E, iters, ndiag, alpha):
                 out = backend.green_rgf_rs(Hii, H1i, E[e], mu, T, sigma_lesser_ph=sl, sigma_r_ph=sr)
             for i, key in enumerate(("ldos", "ndens", "pdens")):
                 G[i][..., e] = out[key]
+            Cur[..., e] = out["cur"]
         Gr, Gl, Gg = (_em(g) for g in G)
         P = backend.gw_polarization(Gr, Gl, Gg, 1, 1, ndiag, dEn, 2.0)
         PR, PL, PG = (_blocks(p, nm) for p in P)
@@ -78,7 +80,7 @@ def _scba(backend, kind, Hii, H1i, Vii, V1i, E, iters, ndiag, alpha):
         Snew = backend.gw_selfenergy(Gr, Gl, Gg, Wr, Wl, Wg, 1, 1, ndiag, dEn)
         S = [np.asarray(backend.gw_mix(s.ravel(), n.ravel(), alpha)).reshape(s.shape)
              for s, n in zip(S, Snew)]
-    return Gr, Gl, Gg, S
+    return Gr, Gl, Gg, S, Cur
 
 
 def test_two_scba_iterations_match_oracle(qt, oracle_mod):
@@ -123,15 +125,17 @@ def test_device_scba_step_equals_host_chain(qt, nm, nx, nen, ndiag):
     result)."""
     Hii, H1i, Vii, V1i = _system(nm, nx, seed=82 + nm)
     E = hm.energy_grid(-2.0, 2.0, nen)
-    Gr, Gl, Gg, S = _scba(qt, "device", Hii, H1i, Vii, V1i, E, 2, ndiag, 0.5)
+    Gr, Gl, Gg, S, Cur = _scba(qt, "device", Hii, H1i, Vii, V1i, E, 2, ndiag, 0.5)
     hii = [Hii[:, :, s] for s in range(nx)]
     h1i = [H1i[:, :, s] for s in range(nx)] + [H1i[:, :, nx - 1]]  # right lead: H1i(nx)
     H = qt.Hamiltonian([hii], [h1i])
     st = qt.GWState(nm, nx, nen, Vii, V1i)
+    cur = np.zeros((nm, nm, nx, nen), order="F")
     try:
         for it in range(2):
             print("device scba step", it, flush=True)
-            tr, tre, status = qt.gw_scba_step(H, st, E, [0.05, -0.05], [300.0, 300.0], ndiag, 2.0, 0.5)
+            tr, tre, status = qt.gw_scba_step(H, st, E, [0.05, -0.05], [300.0, 300.0], ndiag, 2.0, 0.5,
+                                              cur=cur if it == 1 else None)
             assert status == 0
         G = st.host("G")
         Sd = st.host("S")
@@ -143,3 +147,7 @@ def test_device_scba_step_equals_host_chain(qt, nm, nx, nen, ndiag):
         err = np.abs(a - b).max() / np.abs(b).max()
         print(f"scba step {name}: max |device loop - host chain| / max = {err:.1e}")
         assert err <= 1e-12, (name, err)
+    # the slab-bond current matrices of the second iteration's G pass
+    err = np.abs(cur - Cur).max() / np.abs(Cur).max()
+    print(f"scba step cur: max |device loop - host chain| / max = {err:.1e}")
+    assert np.abs(Cur).max() > 0 and err <= 1e-12
