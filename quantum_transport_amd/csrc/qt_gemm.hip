@@ -16,8 +16,12 @@
 // Work mapping: the 1-D grid is ordered so that all tiles of one batch
 // member are dispatched to the same XCD (workgroups are dealt round-robin
 // over the 8 XCDs), keeping that member's operands in one L2.
+#include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 
 #include "qt_internal.h"
 
@@ -385,14 +389,30 @@ void launch_wide(hipStream_t s, const ZGemmArgs& a) {
     QT_HIP(hipGetLastError());
 }
 
-// tile variant: 1 = 128x64 where use_wide() says so (measured 81% of the
-// FP64 peak at 512^3 against 75.5% for 64x64), 0 = always 64x64;
-// QT_GEMM_TILE=64x64 forces the narrow kernel
-int tile_variant() {
+// Tile choice.  QT_GEMM_TILE=64x64 or =128x64 fixes the kernel; otherwise
+// the autotuner below picks per shape (QT_GEMM_TUNE=0: the row-waste rule of
+// use_wide alone).  Both kernels accumulate every output element in the same
+// order (k ascending in groups of 4, the same two MFMAs per group and
+// component, the same epilogue), so the choice never changes a result.
+enum { TILE_AUTO = 0, TILE_NARROW = 1, TILE_WIDE = 2 };
+int tile_forced() {
     static const int v = [] {
         const char* e = getenv("QT_GEMM_TILE");
-        return (e && std::string(e) == "64x64") ? 0 : 1;
+        if (e && std::string(e) == "64x64") return (int)TILE_NARROW;
+        if (e && std::string(e) == "128x64") return (int)TILE_WIDE;
+        return (int)TILE_AUTO;
     }();
+    return v;
+}
+bool tune_enabled() {
+    static const bool v = [] {
+        const char* e = getenv("QT_GEMM_TUNE");
+        return !(e && std::string(e) == "0");
+    }();
+    return v;
+}
+bool tune_log() {
+    static const bool v = getenv("QT_GEMM_TUNE_LOG") != nullptr;
     return v;
 }
 
@@ -406,11 +426,7 @@ bool use_wide(int M) {
 }
 
 template <int OPA, int OPB>
-void launch(hipStream_t s, const ZGemmArgs& a) {
-    if (tile_variant() == 1 && use_wide(a.M)) {
-        launch_wide<OPA, OPB>(s, a);
-        return;
-    }
+void launch_narrow(hipStream_t s, const ZGemmArgs& a) {
     const int tiles_m = (a.M + BM - 1) / BM;
     const int tiles_n = (a.N + BN - 1) / BN;
     const int tiles = tiles_m * tiles_n;
@@ -420,6 +436,88 @@ void launch(hipStream_t s, const ZGemmArgs& a) {
     hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB>), dim3((unsigned)blocks), dim3(256), 0, s, a,
                        tiles_m, tiles, batch_pad);
     QT_HIP(hipGetLastError());
+}
+
+// Autotuner: the first product of each shape (op pair, M, N, K, batch) runs
+// both kernels twice into a scratch output (the operands and the addend are
+// only read; at most 512 MiB of scratch, fewer members where the full batch
+// needs more) after the work already queued on the stream, and the faster
+// kernel is kept for the shape for the life of the process.  Where the
+// scratch cannot be allocated the row-waste rule decides.
+struct TuneState {
+    std::mutex mu;
+    std::map<std::tuple<int, int, int, int, int>, bool> wide;
+};
+TuneState& tune_state() {
+    static TuneState t;
+    return t;
+}
+
+template <int OPA, int OPB>
+bool tuned_wide(hipStream_t s, const ZGemmArgs& a) {
+    const auto key = std::make_tuple(OPA * 3 + OPB, a.M, a.N, a.K, a.batch);
+    TuneState& ts = tune_state();
+    {
+        std::lock_guard<std::mutex> lk(ts.mu);
+        auto it = ts.wide.find(key);
+        if (it != ts.wide.end()) return it->second;
+    }
+    const size_t per = sizeof(double2) * (size_t)a.M * (size_t)a.N;
+    const size_t cap = (size_t)512 << 20;
+    const int members = (int)std::max<size_t>(1, std::min<size_t>((size_t)a.batch, cap / std::max<size_t>(per, 1)));
+    ZGemmArgs t = a;
+    t.batch = members;
+    t.ldc = a.M;
+    t.sbc = (int64_t)a.M * a.N;
+    t.skc = 0;
+    t.skip_lo = t.skip_hi = 0;
+    void* scratch = nullptr;
+    hipEvent_t ev[3];
+    float ms[2] = {1e30f, 1e30f};
+    if (hipMalloc(&scratch, per * (size_t)members) != hipSuccess) {
+        (void)hipGetLastError();  // no room for the scratch output: the row-waste rule, kept
+        const bool wide = use_wide(a.M);
+        std::lock_guard<std::mutex> lk(ts.mu);
+        ts.wide[key] = wide;
+        return wide;
+    }
+    for (auto& e : ev) QT_HIP(hipEventCreate(&e));
+    t.C = (double2*)scratch;
+    for (int rep = 0; rep < 2; ++rep)
+        for (int v = 0; v < 2; ++v) {
+            QT_HIP(hipEventRecord(ev[0], s));
+            if (v) launch_wide<OPA, OPB>(s, t); else launch_narrow<OPA, OPB>(s, t);
+            QT_HIP(hipEventRecord(ev[1], s));
+            QT_HIP(hipEventSynchronize(ev[1]));
+            float m = 0.0f;
+            QT_HIP(hipEventElapsedTime(&m, ev[0], ev[1]));
+            ms[v] = std::min(ms[v], m);
+        }
+    for (auto& e : ev) QT_HIP(hipEventDestroy(e));
+    QT_HIP(hipFree(scratch));
+    const bool wide = ms[1] < ms[0];
+    if (tune_log())
+        fprintf(stderr, "qt gemm tune: ops %d M %d N %d K %d batch %d (%d timed): 64x64 %.4f ms, 128x64 %.4f ms -> %s\n",
+                OPA * 3 + OPB, a.M, a.N, a.K, a.batch, members, ms[0], ms[1], wide ? "128x64" : "64x64");
+    std::lock_guard<std::mutex> lk(ts.mu);
+    ts.wide[key] = wide;
+    return wide;
+}
+
+template <int OPA, int OPB>
+void launch(hipStream_t s, const ZGemmArgs& a) {
+    const int forced = tile_forced();
+    bool wide;
+    if (forced != TILE_AUTO)
+        wide = forced == TILE_WIDE;
+    else if (tune_enabled())
+        wide = tuned_wide<OPA, OPB>(s, a);
+    else
+        wide = use_wide(a.M);
+    if (wide)
+        launch_wide<OPA, OPB>(s, a);
+    else
+        launch_narrow<OPA, OPB>(s, a);
 }
 
 }  // namespace
