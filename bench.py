@@ -413,17 +413,18 @@ def spectra_errors(mix: dict, ref: dict, nm) -> dict:
     return out
 
 
-def run_mixed_leg(args, rgf, H, Eall, Kall, nb: int, nx: int, fp64_value: float) -> dict:
-    """The same workload with QT_PREC_MIXED (sweep products on the int8 matrix
-    cores, operands rounded to 40 bits; leads and inversions FP64): timed like
-    the FP64 leg, and its error measured on 32 points spread over the grid."""
+def run_mixed_leg(args, rgf, H, Eall, Kall, nb: int, nx: int, fp64_value: float, digits: int = 6) -> dict:
+    """The same workload with QT_PREC_MIXED (digits = 6: sweep products on the
+    int8 matrix cores, operands rounded to 40 bits) or QT_PREC_MIXED47
+    (digits = 7, 47 bits); leads and inversions FP64: timed like the FP64
+    leg, and its error measured on 32 points spread over the grid."""
     B = args.batch
-    elapsed, acc = time_batches(rgf, H, Eall, Kall, B, nb * nx, args.steps, args.warmup,
-                                precision=rgf.QT_PREC_MIXED)
+    prec = rgf.QT_PREC_MIXED if digits == 6 else rgf.QT_PREC_MIXED47
+    elapsed, acc = time_batches(rgf, H, Eall, Kall, B, nb * nx, args.steps, args.warmup, precision=prec)
     value = B * args.steps / elapsed
     idx = np.linspace(0, len(Eall) - 1, 32).astype(int)
     ref = rgf.solve_batch(H, Eall[idx], Kall[idx], mul=0.0, mur=-0.1)
-    mix = rgf.solve_batch(H, Eall[idx], Kall[idx], mul=0.0, mur=-0.1, precision=rgf.QT_PREC_MIXED)
+    mix = rgf.solve_batch(H, Eall[idx], Kall[idx], mul=0.0, mur=-0.1, precision=prec)
     errs = spectra_errors(mix, ref, [nb] * nx)
     return {"value": round(value, 4), "unit": "energy points/s",
             "speedup_vs_fp64": round(value / fp64_value, 3),
@@ -432,7 +433,8 @@ def run_mixed_leg(args, rgf, H, Eall, Kall, nb: int, nx: int, fp64_value: float)
             "max_rel_err": max(errs.values()), "errors": errs, "bar": 1e-8,
             "meets_bar": bool(max(errs.values()) <= 1e-8),
             "error_sample": "32 points evenly spread over the 4096-point (E, kz) list, against the FP64 run",
-            "products": "int8 MFMA, 6 base-128 digits per value (40-bit operands), Karatsuba complex product"}
+            "products": f"int8 MFMA, {digits} base-128 digits per value ({7 * digits - 2}-bit operands), "
+                        "Karatsuba complex product"}
 
 
 def run_leads(args, hm, rgf) -> dict:
@@ -890,6 +892,7 @@ def main() -> int:
     }
     if not args.no_mixed:
         res["mixed"] = run_mixed_leg(args, rgf, H, Eall, Kall, nb, nx, value)
+        res["mixed47"] = run_mixed_leg(args, rgf, H, Eall, Kall, nb, nx, value, digits=7)
     H.close()
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -67,6 +67,12 @@ enum {
                               counterpart).  Measured: spectra within ~1e-7 and
                               contact currents within ~1e-6 of QT_PREC_FP64
                               (relative to each quantity's largest value). */
+#define QT_PREC_MIXED47 2  /* as QT_PREC_MIXED with 7 digits (47 bits) and 28
+                              digit-pair products instead of 21 (no reference
+                              counterpart).  Measured at nb = 512 x 100 slabs:
+                              spectra within ~1e-9 and contact currents within
+                              1e-8 of QT_PREC_FP64; on a 24-slab device with kz
+                              the contact currents within ~1e-6. */
 
 /* option flags (qt_rgf_params.flags) */
 #define QT_F_COUPLING_FIX 0x1   /* forward variant: the second pass couples slab
@@ -262,7 +268,7 @@ int qt_zgemm_batched(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_t k, 
                      int64_t ldb, int64_t strideB, double beta, void* R, int64_t ldr,
                      int64_t strideR);
 
-/* qt_zgemm_batched with a precision mode (QT_PREC_FP64 or QT_PREC_MIXED).  No
+/* qt_zgemm_batched with a precision mode (QT_PREC_FP64, _MIXED or _MIXED47).  No
  * reference counterpart: the mixed mode is new.  Shapes with M, N or K below 64
  * or K above 1024 run in FP64 whatever the mode. */
 int qt_zgemm_batched_prec(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_t k, int32_t batch,

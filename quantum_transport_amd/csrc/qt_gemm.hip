@@ -526,8 +526,8 @@ void zgemm(hipStream_t s, int opa, int opb, const ZGemmArgs& a, Timer* t) {
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return;
     if (t && t->enabled) t->begin(s, 0);
     const int key = opa * 3 + opb;
-    if (a.prec == 1 && zgemm_i8_fits(a)) {
-        zgemm_i8(s, opa, opb, a);
+    if (a.prec != 0 && zgemm_i8_fits(a)) {
+        zgemm_i8(s, opa, opb, a, a.prec);
         if (t && t->enabled)
             t->end(s, 0, 8.0 * a.M * a.N * (double)a.K * (a.flop_batch >= 0 ? a.flop_batch : a.batch));
         return;

@@ -5,9 +5,9 @@
 // brings its largest |re| or |im| into [2^(Q-1), 2^Q); the scaled real and
 // imaginary parts are rounded to integers and written as ND balanced
 // base-128 digits in [-64, 63] (most significant first), so a value is kept
-// to Q = 7 ND - 2 bits relative to its row maximum (ND = QT_I8_DIGITS: 6 by
-// default, Q = 40; a build with 7 digits keeps 47 bits at 28/21 of the MFMA
-// work and runs the 64x64 kernel only).  A third digit set holds re + im digit
+// to Q = 7 ND - 2 bits relative to its row maximum (ND = 6 for QT_PREC_MIXED,
+// Q = 40; ND = 7 for QT_PREC_MIXED47, Q = 47 at 28/21 of the MFMA work, on
+// the 64x64 kernel only).  A third digit set holds re + im digit
 // by digit (range [-128, 126]) for the Karatsuba product
 //     Re C = P1 - P2,  Im C = P3 - P1 - P2,
 //     P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi).
@@ -35,13 +35,15 @@ namespace qt {
 
 namespace {
 
-#ifndef QT_I8_DIGITS
-#define QT_I8_DIGITS 6
-#endif
-constexpr int ND = QT_I8_DIGITS;  // digits per value
-constexpr int NPL = 3 * ND;    // planes per operand
-constexpr int QB = 7 * ND - 2;  // row maximum scaled into [2^(QB-1), 2^QB): top digit <= 32
-constexpr int EW = 7 * (ND - 1);  // weight of the leading digit pair, 2^EW
+// ND digits per value: 3 ND planes per operand; a row maximum is scaled
+// into [2^(QB-1), 2^QB), QB = 7 ND - 2 (top digit <= 32); the leading digit
+// pair weighs 2^EW, EW = 7 (ND - 1).
+template <int ND>
+constexpr int npl() { return 3 * ND; }
+template <int ND>
+constexpr int qbits() { return 7 * ND - 2; }
+template <int ND>
+constexpr int ew() { return 7 * (ND - 1); }
 constexpr int TM = 64, TN = 64, TK = 64;
 constexpr int LROW = 80;       // LDS row stride in bytes (64 + 16: conflict-free 16-byte reads)
 constexpr int NXCD = 8;
@@ -51,8 +53,10 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 // Writes the 3 ND digit planes of four consecutive values of one row (k0 a
 // multiple of 4): one 32-bit store per plane.
+template <int ND>
 __device__ __forceinline__ void write_digits(int8_t* __restrict__ base, int64_t pstride, int k0,
                                              const double2 (&v)[4], int s) {
+    constexpr int NPL = npl<ND>();
     unsigned w[NPL];
 #pragma unroll
     for (int p = 0; p < NPL; ++p) w[p] = 0u;
@@ -75,10 +79,12 @@ __device__ __forceinline__ void write_digits(int8_t* __restrict__ base, int64_t 
     for (int p = 0; p < NPL; ++p) *reinterpret_cast<unsigned*>(base + p * pstride + k0) = w[p];
 }
 
-__device__ __forceinline__ int row_scale(double mx) { return mx > 0.0 ? (QB - 1) - ilogb(mx) : 0; }
+template <int ND>
+__device__ __forceinline__ int row_scale(double mx) { return mx > 0.0 ? (qbits<ND>() - 1) - ilogb(mx) : 0; }
 
 // Rows stored contiguously, Y[r][k] = X[k + r ld] (conj: imaginary part
 // negated): one wave per row, each lane four consecutive k per 256.
+template <int ND>
 __global__ __launch_bounds__(256) void split_rows_kernel(ZOp X, int conj, int R, int K, int Rp, int Kp,
                                                          const int* kidx, int kshift, const int* active,
                                                          int8_t* __restrict__ out, int* __restrict__ sc) {
@@ -106,22 +112,23 @@ __global__ __launch_bounds__(256) void split_rows_kernel(ZOp X, int conj, int R,
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-    const int s = row_scale(mx);
+    const int s = row_scale<ND>(mx);
     if (lane == 0) sc[(int64_t)m * Rp + r] = s;
 
-    int8_t* __restrict__ base = out + (int64_t)m * NPL * Rp * Kp + (int64_t)r * Kp;
+    int8_t* __restrict__ base = out + (int64_t)m * npl<ND>() * Rp * Kp + (int64_t)r * Kp;
     const int64_t pstride = (int64_t)Rp * Kp;
 #pragma unroll
     for (int j = 0; j < KMAX / 256; ++j) {
         const int k0 = lane * 4 + 256 * j;
         if (k0 >= Kp) break;
-        write_digits(base, pstride, k0, v[j], s);
+        write_digits<ND>(base, pstride, k0, v[j], s);
     }
 }
 
 // Rows strided, Y[r][k] = X[r + k ld]: eight rows per workgroup, thread t
 // takes row t % 8 and four consecutive k per 128 (k = 4 (t / 8) + 128 j), so
 // the eight lanes of a row group read 128 contiguous bytes.
+template <int ND>
 __global__ __launch_bounds__(256) void split_cols_kernel(ZOp X, int conj, int R, int K, int Rp, int Kp,
                                                          const int* kidx, int kshift, const int* active,
                                                          int8_t* __restrict__ out, int* __restrict__ sc) {
@@ -154,17 +161,17 @@ __global__ __launch_bounds__(256) void split_cols_kernel(ZOp X, int conj, int R,
     if (lane < 8) red[wave][lane] = mx;
     __syncthreads();
     mx = fmax(fmax(red[0][rr], red[1][rr]), fmax(red[2][rr], red[3][rr]));
-    const int s = row_scale(mx);
+    const int s = row_scale<ND>(mx);
     if (r >= Rp) return;
     if (kq == 0) sc[(int64_t)m * Rp + r] = s;
 
-    int8_t* __restrict__ base = out + (int64_t)m * NPL * Rp * Kp + (int64_t)r * Kp;
+    int8_t* __restrict__ base = out + (int64_t)m * npl<ND>() * Rp * Kp + (int64_t)r * Kp;
     const int64_t pstride = (int64_t)Rp * Kp;
 #pragma unroll
     for (int j = 0; j < KMAX / 128; ++j) {
         const int k0 = kq * 4 + 128 * j;
         if (k0 >= Kp) break;
-        write_digits(base, pstride, k0, v[j], s);
+        write_digits<ND>(base, pstride, k0, v[j], s);
     }
 }
 
@@ -175,6 +182,7 @@ __global__ __launch_bounds__(256) void split_cols_kernel(ZOp X, int conj, int R,
 // (one per digit weight); the product is then folded into FP64 Re/Im sums.
 // v_mfma_i32_16x16x64_i8 is fed (B fragment, A fragment), so lane l,
 // register q holds C[row = l % 16][col = 4 * (l / 16) + q] of a block.
+template <int ND>
 __global__ __launch_bounds__(256, 2) void zgemm_i8_kernel(ZGemmArgs a, const int8_t* __restrict__ PA,
                                                            const int* __restrict__ SA, int64_t pa_stride,
                                                            int64_t sa_stride, const int8_t* __restrict__ PB,
@@ -290,7 +298,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_i8_kernel(ZGemmArgs a, const int
     for (int i = 0; i < 2; ++i) {
         const int gi = m0 + wm * 32 + i * 16 + li;
         if (gi >= a.M) continue;
-        const int es = EW - sa[gi];
+        const int es = ew<ND>() - sa[gi];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
 #pragma unroll
@@ -310,7 +318,6 @@ __global__ __launch_bounds__(256, 2) void zgemm_i8_kernel(ZGemmArgs a, const int
     }
 }
 
-#if QT_I8_DIGITS == 6  // two buffers of 192 rows x 64 k x ND digits fit the 160 KB LDS for ND <= 6
 // 128x64 tiles, 8 waves (4 x 2 sub-tiles of 32x32), digit planes copied from
 // global memory straight into LDS (global_load_lds_dwordx4: lane L of a wave
 // writes 16 bytes at base + 16 L), double buffered, one barrier per chunk.
@@ -318,18 +325,22 @@ __global__ __launch_bounds__(256, 2) void zgemm_i8_kernel(ZGemmArgs a, const int
 // part ^ ((r >> 1) & 3), so the 16-byte fragment reads of eight consecutive
 // rows fall in distinct banks.
 constexpr int VM = 128, VN = 64;
-constexpr int VA = VM * ND * 64, VB = VN * ND * 64;  // bytes of the A and B tiles of one chunk
 typedef __attribute__((address_space(3))) void lds_void;
 typedef __attribute__((address_space(1))) void gbl_void;
 
 __device__ __forceinline__ int swz(int row) { return (row >> 1) & 3; }
 
+template <int ND>
 __global__ __launch_bounds__(512, 1) void zgemm_i8_lds_kernel(ZGemmArgs a, const int8_t* __restrict__ PA,
                                                                const int* __restrict__ SA, int64_t pa_stride,
                                                                int64_t sa_stride, const int8_t* __restrict__ PB,
                                                                const int* __restrict__ SB, int64_t pb_stride,
                                                                int64_t sb_stride, int Mp, int Np, int Kp,
                                                                int tiles_m, int tiles) {
+    constexpr int VA = VM * ND * 64, VB = VN * ND * 64;  // bytes of the A and B tiles of one chunk
+    // two buffers of 192 rows x 64 k x ND digits must fit the 160 KB LDS
+    // (ND <= 6), and the 4 ND row groups of B spread evenly over 8 waves
+    static_assert(2 * (VA + VB) <= 160 * 1024 && (4 * ND) % 8 == 0, "digit count outside the LDS kernel");
     const int L = blockIdx.x;
     const int rest = L / NXCD;
     const int tile = rest % tiles;
@@ -362,7 +373,7 @@ __global__ __launch_bounds__(512, 1) void zgemm_i8_lds_kernel(ZGemmArgs a, const
             __builtin_amdgcn_global_load_lds((gbl_void*)(srcA + d * pla), (lds_void*)&lds[buf][d * VM * 64 + wave * 1024],
                                              16, 0, 0);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < ND / 2; ++i) {
             const int J = wave + 8 * i, d = J >> 2, grp = J & 3;
             const int rb = 16 * grp + (lane >> 2);
             const int8_t* srcB = Bb + (int64_t)(prod * ND + d) * plb + (int64_t)rb * Kp + k0 + 16 * (pp ^ swz(rb));
@@ -449,7 +460,7 @@ __global__ __launch_bounds__(512, 1) void zgemm_i8_lds_kernel(ZGemmArgs a, const
     for (int i = 0; i < 2; ++i) {
         const int gi = m0 + wm * 32 + i * 16 + li;
         if (gi >= a.M) continue;
-        const int es = EW - sa[gi];
+        const int es = ew<ND>() - sa[gi];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
 #pragma unroll
@@ -469,18 +480,14 @@ __global__ __launch_bounds__(512, 1) void zgemm_i8_lds_kernel(ZGemmArgs a, const
     }
 }
 
-#endif
-
-int kernel_variant() {
-#if QT_I8_DIGITS == 6
+// 2: the 128x64 LDS kernel (6 digits), 1: the 64x64 kernel (QT_I8_KERNEL=1
+// forces it for 6 digits; 7 digits always run it)
+int kernel_variant(int nd) {
     static const int v = [] {
         const char* e = std::getenv("QT_I8_KERNEL");
         return (e && e[0] == '1') ? 1 : 2;
     }();
-    return v;
-#else
-    return 1;
-#endif
+    return nd == 6 ? v : 1;
 }
 
 // per-stream workspace for the digit planes, grown on demand: growth waits
@@ -521,29 +528,23 @@ Ws& workspace(hipStream_t s, size_t pbytes, size_t sints) {
 
 int64_t up64(int64_t x) { return (x + 63) / 64 * 64; }
 
+template <int ND>
 void split(hipStream_t s, const ZOp& X, bool strided, bool conj, int R, int K, int Rp, int Kp, int members,
            const int* kidx, int kshift, const int* active, int8_t* out, int* sc) {
     if (strided)
-        hipLaunchKernelGGL(split_cols_kernel, dim3((unsigned)(Rp / 8), members), dim3(256), 0, s, X, conj ? 1 : 0,
-                           R, K, Rp, Kp, kidx, kshift, active, out, sc);
+        hipLaunchKernelGGL(split_cols_kernel<ND>, dim3((unsigned)(Rp / 8), members), dim3(256), 0, s, X,
+                           conj ? 1 : 0, R, K, Rp, Kp, kidx, kshift, active, out, sc);
     else
-        hipLaunchKernelGGL(split_rows_kernel, dim3((unsigned)(Rp / 4), members), dim3(256), 0, s, X, conj ? 1 : 0,
-                           R, K, Rp, Kp, kidx, kshift, active, out, sc);
+        hipLaunchKernelGGL(split_rows_kernel<ND>, dim3((unsigned)(Rp / 4), members), dim3(256), 0, s, X,
+                           conj ? 1 : 0, R, K, Rp, Kp, kidx, kshift, active, out, sc);
     QT_HIP(hipGetLastError());
 }
 
-}  // namespace
-
-bool zgemm_i8_fits(const ZGemmArgs& a) { return a.M >= 64 && a.N >= 64 && a.K >= 64 && a.K <= KMAX; }
-
-void zgemm_i8(hipStream_t s, int opa, int opb, const ZGemmArgs& a) {
-    if (!zgemm_i8_fits(a)) throw hip_error("zgemm_i8: shape outside 64 <= M, N, K and K <= 1024", 6);
-    const int var = kernel_variant();
-#if QT_I8_DIGITS == 6
+template <int ND>
+void zgemm_i8_nd(hipStream_t s, int opa, int opb, const ZGemmArgs& a) {
+    constexpr int NPL = npl<ND>();
+    const int var = kernel_variant(ND);
     const int tm = var == 2 ? VM : TM, tn = var == 2 ? VN : TN;
-#else
-    const int tm = TM, tn = TN;
-#endif
     const int64_t Mp = (a.M + tm - 1) / tm * tm, Np = (a.N + tn - 1) / tn * tn, Kp = up64(a.K);
     const bool sharedA = a.A.sb == 0 && (a.A.sk == 0 || !a.kidx);
     const bool sharedB = a.B.sb == 0 && (a.B.sk == 0 || !a.kidx);
@@ -556,11 +557,11 @@ void zgemm_i8(hipStream_t s, int opa, int opb, const ZGemmArgs& a) {
     int* SA = w.scales;
     int* SB = w.scales + sa;
     // op(A) rows: N -> X[r + k ld] (strided); T, C -> X[k + r ld]
-    split(s, a.A, opa == OP_N, opa == OP_C, a.M, a.K, (int)Mp, (int)Kp, sharedA ? 1 : a.batch,
-          sharedA ? nullptr : a.kidx, a.kshift, sharedA ? nullptr : a.active, PA, SA);
+    split<ND>(s, a.A, opa == OP_N, opa == OP_C, a.M, a.K, (int)Mp, (int)Kp, sharedA ? 1 : a.batch,
+              sharedA ? nullptr : a.kidx, a.kshift, sharedA ? nullptr : a.active, PA, SA);
     // op(B) columns: N -> X[k + c ld]; T, C -> X[c + k ld] (strided)
-    split(s, a.B, opb != OP_N, opb == OP_C, a.N, a.K, (int)Np, (int)Kp, sharedB ? 1 : a.batch,
-          sharedB ? nullptr : a.kidx, a.kshift, sharedB ? nullptr : a.active, PB, SB);
+    split<ND>(s, a.B, opb != OP_N, opb == OP_C, a.N, a.K, (int)Np, (int)Kp, sharedB ? 1 : a.batch,
+              sharedB ? nullptr : a.kidx, a.kshift, sharedB ? nullptr : a.active, PB, SB);
     const int tiles_m = (int)(Mp / tm), tiles_n = (int)(Np / tn);
     const int tiles = tiles_m * tiles_n;
     const int batch_pad = (a.batch + NXCD - 1) / NXCD * NXCD;
@@ -568,15 +569,31 @@ void zgemm_i8(hipStream_t s, int opa, int opb, const ZGemmArgs& a) {
     if (blocks > 0x7fffffff) throw hip_error("zgemm_i8: grid too large", 1);
     const int64_t psa = sharedA ? 0 : (int64_t)NPL * Mp * Kp, ssa = sharedA ? 0 : Mp;
     const int64_t psb = sharedB ? 0 : (int64_t)NPL * Np * Kp, ssb = sharedB ? 0 : Np;
-#if QT_I8_DIGITS == 6
-    if (var == 2)
-        hipLaunchKernelGGL(zgemm_i8_lds_kernel, dim3((unsigned)blocks), dim3(512), 0, s, a, PA, SA, psa, ssa, PB, SB,
-                           psb, ssb, (int)Mp, (int)Np, (int)Kp, tiles_m, tiles);
-    else
-#endif
-        hipLaunchKernelGGL(zgemm_i8_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, PA, SA, psa, ssa, PB, SB, psb,
-                           ssb, (int)Mp, (int)Np, (int)Kp, tiles_m, tiles);
+    if constexpr (ND == 6) {
+        if (var == 2) {
+            hipLaunchKernelGGL(zgemm_i8_lds_kernel<ND>, dim3((unsigned)blocks), dim3(512), 0, s, a, PA, SA, psa, ssa,
+                               PB, SB, psb, ssb, (int)Mp, (int)Np, (int)Kp, tiles_m, tiles);
+            QT_HIP(hipGetLastError());
+            return;
+        }
+    }
+    hipLaunchKernelGGL(zgemm_i8_kernel<ND>, dim3((unsigned)blocks), dim3(256), 0, s, a, PA, SA, psa, ssa, PB, SB,
+                       psb, ssb, (int)Mp, (int)Np, (int)Kp, tiles_m, tiles);
     QT_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+bool zgemm_i8_fits(const ZGemmArgs& a) { return a.M >= 64 && a.N >= 64 && a.K >= 64 && a.K <= KMAX; }
+
+void zgemm_i8(hipStream_t s, int opa, int opb, const ZGemmArgs& a, int digits) {
+    if (!zgemm_i8_fits(a)) throw hip_error("zgemm_i8: shape outside 64 <= M, N, K and K <= 1024", 6);
+    if (digits == 6)
+        zgemm_i8_nd<6>(s, opa, opb, a);
+    else if (digits == 7)
+        zgemm_i8_nd<7>(s, opa, opb, a);
+    else
+        throw hip_error("zgemm_i8: digit count must be 6 or 7", 6);
 }
 
 }  // namespace qt

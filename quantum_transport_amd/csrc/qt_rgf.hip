@@ -38,6 +38,17 @@ Ctx* g_ctx = nullptr;
 std::mutex g_init_mu;
 static thread_local std::string g_last_error;
 
+// int8 digits per value of a precision mode's products: 0 for QT_PREC_FP64,
+// -1 for a mode that does not exist
+static int prec_digits(int precision) {
+    switch (precision) {
+        case QT_PREC_FP64: return 0;
+        case QT_PREC_MIXED: return 6;
+        case QT_PREC_MIXED47: return 7;
+        default: return -1;
+    }
+}
+
 size_t work_bytes(const qt_ham& h, int B, bool sigma) {
     const int64_t nn = (int64_t)h.nmax * h.nmax;
     const int64_t tiles = (h.nmax + 31) / 32;
@@ -557,7 +568,7 @@ struct Solve {
         QT_HIP(hipEventRecord(c.ev[1], s));
         // mixed mode: the sweep's block products on the int8 matrix cores (the
         // decimation, its self-energies and every inversion stay in FP64)
-        g.prec = p.precision == QT_PREC_MIXED ? 1 : 0;
+        g.prec = prec_digits(p.precision);
 
         // ---------------- pass 1, first slab (rgf_mod.f90:91-104 / 312-325)
         {
@@ -987,7 +998,7 @@ int qt_rgf_solve(qt_ham* h, const qt_rgf_params* p, int32_t npts, const double* 
     if (!g_ctx) return QT_ERR_NOT_INIT;
     if (!h || !p || npts < 0 || (npts > 0 && !En) || !out) return QT_ERR_INVALID;
     if (npts == 0) return QT_OK;
-    if (p->precision != QT_PREC_FP64 && p->precision != QT_PREC_MIXED) return QT_ERR_UNSUPPORTED;
+    if (prec_digits(p->precision) < 0) return QT_ERR_UNSUPPORTED;
     if (p->variant != QT_RGF_FORWARD && p->variant != QT_RGF_BACKWARD && p->variant != QT_RGF_GW)
         return QT_ERR_INVALID;
     const bool gw = p->variant == QT_RGF_GW;
@@ -1347,7 +1358,7 @@ int qt_zgemm_batched_prec(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_
                           int64_t ldb, int64_t strideB, double beta, void* R, int64_t ldr,
                           int64_t strideR, int32_t precision) {
     if (!g_ctx) return QT_ERR_NOT_INIT;
-    if (precision != QT_PREC_FP64 && precision != QT_PREC_MIXED) return QT_ERR_INVALID;
+    if (prec_digits(precision) < 0) return QT_ERR_INVALID;
     if (m <= 0 || n <= 0 || k < 0 || batch <= 0 || !A || !B || !R) return QT_ERR_INVALID;
     if (opa < 0 || opa > 2 || opb < 0 || opb > 2) return QT_ERR_INVALID;
     const int64_t ar = opa == OP_N ? m : k, ac = opa == OP_N ? k : m;
@@ -1382,7 +1393,7 @@ int qt_zgemm_batched_prec(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_
         if (beta != 0.0) g.D = op_member(dR, ldr, strideR);
         g.alpha = alpha;
         g.beta = beta;
-        g.prec = precision == QT_PREC_MIXED ? 1 : 0;
+        g.prec = prec_digits(precision);
         zgemm(c.stream, opa, opb, g, &c.timer);
         QT_HIP(hipStreamSynchronize(c.stream));
         QT_HIP(hipMemcpy(R, dR, sizeof(double2) * er, hipMemcpyDeviceToHost));

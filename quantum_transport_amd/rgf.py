@@ -27,6 +27,7 @@ QT_RGF_BACKWARD = 1
 QT_RGF_GW = 2
 QT_PREC_FP64 = 0
 QT_PREC_MIXED = 1
+QT_PREC_MIXED47 = 2
 QT_F_COUPLING_FIX = 0x1
 QT_F_DEVICE_OUTPUTS = 0x2
 QT_F_LEAD_RESET = 0x4
@@ -411,7 +412,8 @@ def zgemm_batched(A: np.ndarray, B: np.ndarray, opa: int = 0, opb: int = 0, alph
     """R[b] = alpha op(A[b]) op(B[b]) + beta R[b] on the device (MUL_C,
     matrix_c.f90:615-660).  A, B: (batch, rows, cols) stacks of column-major
     blocks (each block stored Fortran-ordered).  precision=QT_PREC_MIXED runs
-    the product on the int8 matrix cores (operands rounded to 40 bits)."""
+    the product on the int8 matrix cores (operands rounded to 40 bits),
+    QT_PREC_MIXED47 likewise with 47 bits."""
     init()
     A = np.asarray(A, dtype=np.complex128)
     B = np.asarray(B, dtype=np.complex128)

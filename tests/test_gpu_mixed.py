@@ -1,11 +1,12 @@
-"""The int8 digit-slice complex GEMM (QT_PREC_MIXED) through the C ABI.
+"""The int8 digit-slice complex GEMM (QT_PREC_MIXED: 6 digits, and
+QT_PREC_MIXED47: 7 digits) through the C ABI.
 
-Operands are rounded to 40 bits relative to their row (op(A)) or column
-(op(B)) maximum, so for every element
-    |C - C_exact| <= 16 K 2^-40 max_k|op(A)[r, k]| max_k|op(B)[k, c]|
+Operands are rounded to Q = 40 (47) bits relative to their row (op(A)) or
+column (op(B)) maximum, so for every element
+    |C - C_exact| <= 16 K 2^-Q max_k|op(A)[r, k]| max_k|op(B)[k, c]|
 (8 real rounding terms per complex product term, dropped digit pairs below
-2^-42); integer operands of up to 20 bits are represented exactly, and then
-the product is exact."""
+2^-(Q+2)); integer operands of up to 20 (22) bits are represented exactly,
+and then the product is exact."""
 import numpy as np
 import pytest
 
@@ -28,15 +29,16 @@ def _rand(rng, *shape):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
-def _bound(opA, opB, k):
+def _bound(opA, opB, k, bits=40):
     ra = np.maximum(np.abs(opA.real), np.abs(opA.imag)).max(axis=1)
     cb = np.maximum(np.abs(opB.real), np.abs(opB.imag)).max(axis=0)
-    return 16.0 * k * 2.0 ** -40 * np.outer(ra, cb)
+    return 16.0 * k * 2.0 ** -bits * np.outer(ra, cb)
 
 
+@pytest.mark.parametrize("bits", [40, 47])
 @pytest.mark.parametrize("opa,opb", [(0, 0), (0, 2), (2, 0), (2, 2), (0, 1), (1, 0)])
 @pytest.mark.parametrize("mnk", [(64, 64, 64), (70, 100, 130), (128, 96, 200), (200, 64, 513)])
-def test_mixed_zgemm_bound(qt, opa, opb, mnk):
+def test_mixed_zgemm_bound(qt, opa, opb, mnk, bits):
     m, n, k = mnk
     rng = np.random.default_rng(m + 5 * n + 11 * k + opa * 3 + opb)
     batch = 3
@@ -46,31 +48,41 @@ def test_mixed_zgemm_bound(qt, opa, opb, mnk):
     A[1] *= np.logspace(-7, 3, A.shape[1])[:, None]
     B[2] *= np.logspace(5, -4, B.shape[2])[None, :]
     R0 = _rand(rng, batch, m, n)
-    got = qt.zgemm_batched(A, B, opa, opb, alpha=-1.0, beta=1.0, R=R0, precision=qt.QT_PREC_MIXED)
+    prec = qt.QT_PREC_MIXED if bits == 40 else qt.QT_PREC_MIXED47
+    got = qt.zgemm_batched(A, B, opa, opb, alpha=-1.0, beta=1.0, R=R0, precision=prec)
     for b in range(batch):
         oa, ob = OPS[opa](A[b]), OPS[opb](B[b])
         ref = R0[b] - oa @ ob
         err = np.abs(got[b] - ref)
-        bound = _bound(oa, ob, k)
+        # plus the FP64 rounding of the beta R additions here and on the device
+        # (below the operand term at 40 bits, not at 47 on the rows scaled
+        # down to 1e-7)
+        bound = _bound(oa, ob, k, bits) + 2.0 ** -51 * (np.abs(R0[b]) + np.abs(ref))
         assert (err <= bound).all(), (b, float((err / bound).max()))
         # and the error is not far below the bound either: the operands are rounded
         assert np.abs(got[b] - ref).max() > 0.0
 
 
+@pytest.mark.parametrize("digits", [6, 7])
 @pytest.mark.parametrize("mnk", [(64, 64, 64), (96, 160, 300), (512, 512, 512)])
-def test_mixed_zgemm_exact_on_integers(qt, mnk):
-    """20-bit integer operands are exact in 40 bits, so every digit product is
-    exact and the result equals the integer product bit for bit"""
+def test_mixed_zgemm_exact_on_integers(qt, mnk, digits):
+    """20-bit (7 digits: 22-bit) integer operands are exact in 40 (47) bits,
+    so every digit product is exact and the result equals the integer product
+    bit for bit"""
     m, n, k = mnk
     rng = np.random.default_rng(k)
     batch = 2
-    # |x| < 2^19 keeps every row maximum below 2^19, so the scaled values are
-    # multiples of 2^21 and only the top three digits are nonzero (x = -2^19
-    # would need a fourth, whose pair with itself is one of the dropped ones)
-    lo, hi = -(2 ** 19) + 1, 2 ** 19
+    # 6 digits: |x| < 2^19 keeps every row maximum below 2^19, so the scaled
+    # values are multiples of 2^21 and only the top three digits are nonzero
+    # (x = -2^19 would need a fourth, whose pair with itself is one of the
+    # dropped ones).  7 digits: |x| < 2^21, scaled values multiples of 2^26,
+    # the top four digits nonzero, and every pair of them is kept.  Sums stay
+    # below 2^53 at K <= 512, so the FP64 fold is exact too.
+    lim = 2 ** 19 if digits == 6 else 2 ** 21
+    lo, hi = -lim + 1, lim
     A = rng.integers(lo, hi, (batch, m, k)) + 1j * rng.integers(lo, hi, (batch, m, k))
     B = rng.integers(lo, hi, (batch, k, n)) + 1j * rng.integers(lo, hi, (batch, k, n))
-    got = qt.zgemm_batched(A, B, 0, 0, precision=qt.QT_PREC_MIXED)
+    got = qt.zgemm_batched(A, B, 0, 0, precision=qt.QT_PREC_MIXED if digits == 6 else qt.QT_PREC_MIXED47)
     Ar, Ai = A.real.astype(object), A.imag.astype(object)
     for b in range(batch):
         # exact integer reference (object arithmetic), then rounded once to double
@@ -196,3 +208,29 @@ def test_mixed_small_blocks_stay_fp64(qt):
     H.close()
     for key in ("gr_diag", "gl_diag", "gg_diag", "tr", "tre"):
         assert np.array_equal(mix[key], fp[key]), key
+
+
+def test_mixed47_more_accurate(qt):
+    """QT_PREC_MIXED47 (47-bit operands) against the FP64 device path on the
+    nb = 256 x 24 slab, 2 kz case above and on the uneven device: every
+    observable at least 16x closer to FP64 than the 40-bit mode (7 more bits
+    of every operand: 128x in the error bound), and the spectra within 1e-8.
+    Measured on MI355X, 40 / 47 bits: LDOS 1.5e-8 / 1.4e-10, ndos 9.1e-8 /
+    5.5e-10, pdos 4.3e-8 / 2.5e-10, tr 2.4e-6 / 4.2e-8, tre 6.6e-5 / 7.5e-7."""
+    kz = [0.0, np.pi / 2]
+    _, dev = hm.synthetic_device(256, 24, seed=1002, kz=kz, tz_density=0.02)
+    E = np.repeat(hm.energy_grid(-2.0, 2.0, 8), 2)
+    k = np.tile(np.arange(2), 8)
+    H = qt.Hamiltonian.from_device(dev)
+    fp = qt.solve_batch(H, E, k, mul=0.0, mur=-0.1)
+    m6 = qt.solve_batch(H, E, k, mul=0.0, mur=-0.1, precision=qt.QT_PREC_MIXED)
+    m7 = qt.solve_batch(H, E, k, mul=0.0, mur=-0.1, precision=qt.QT_PREC_MIXED47)
+    H.close()
+    assert (m7["info"] == 0).all()
+    e6, e7 = _mixed_errors(m6, fp, dev.nm), _mixed_errors(m7, fp, dev.nm)
+    print("40-bit", e6, "\n47-bit", e7)
+    assert max(e7.values()) > 0.0  # the int8 products ran
+    for key in e6:
+        assert e7[key] <= e6[key] / 16, (key, e6[key], e7[key])
+        if key not in ("tr", "tre"):
+            assert e7[key] <= 1e-8, (key, e7[key])
