@@ -35,6 +35,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "RGF energy-points/s @nb=512×100 slabs: FP64 vs mixed+warm-start @≤1e-8 err"
+# complex products and inversions run per slab of a ballistic sweep, 8 n^3 flops
+# each: pass 1 five products and the inversion, pass 2 seven products between
+# slabs of equal size (eight otherwise)
+SLAB_FLOPS_EQUAL = 13 * 8
+SLAB_FLOPS_UNEVEN = 14 * 8
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 # HBM bytes moved by one 512^3 complex product without an addend (A and B read
 # once, C written once), measured with rocprofv3 FETCH_SIZE / WRITE_SIZE and a
@@ -612,8 +617,9 @@ def run_uneven(args, hm, rgf) -> dict:
     gemm_tf = acc["gemm_flops"] / (acc["gemm_ms"] * 1e-3) / 1e12 if acc["gemm_ms"] > 0 else 0.0
     inv_tf = acc["inv_flops"] / (acc["inv_ms"] * 1e-3) / 1e12 if acc["inv_ms"] > 0 else 0.0
     K = acc["lead_iterations"] / max(1, 2 * points)
-    # 120 n^3 per slab with n^3 -> (n_i n_{i-1} terms); leads at the end slab sizes
-    sweep = sum(120.0 * float(n) ** 3 for n in sizes)
+    # 112 n^3 per slab (neighbours of unequal size) with n^3 standing for the
+    # n_i n_{i-1} terms; leads at the end slab sizes
+    sweep = sum(float(SLAB_FLOPS_UNEVEN) * float(n) ** 3 for n in sizes)
     lead = (K * 56 + 24) * (float(sizes[0]) ** 3 + float(sizes[-1]) ** 3)
     return {
         "metric": "RGF energy-points/s, uneven slabs {128..768} x 80, FP64",
@@ -672,7 +678,7 @@ def run_small(args, hm, rgf) -> dict:
     inv_tf = acc["inv_flops"] / (acc["inv_ms"] * 1e-3) / 1e12 if acc["inv_ms"] > 0 else 0.0
     K = acc["lead_iterations"] / max(1, 2 * points)
     n3 = float(nb) ** 3
-    path_flops = nx * 120 * n3 + 2 * (K * 56 + 24) * n3
+    path_flops = nx * SLAB_FLOPS_EQUAL * n3 + 2 * (K * 56 + 24) * n3
     return {
         "metric": "RGF energy-points/s, nb=64 x 32 slabs, 256 E, FP64",
         "value": round(points / elapsed, 2), "unit": "energy points/s", "n_gpus": 1,
@@ -857,7 +863,7 @@ def main() -> int:
     K = acc["lead_iterations"] / max(1, 2 * points)
     n3 = float(nb) ** 3
     lead_flops = (K * 56 + 24) * n3  # per lead: K (INV + 6 GEMM) + INV + 2 GEMM
-    path_flops = nx * 120 * n3 + 2 * lead_flops
+    path_flops = nx * SLAB_FLOPS_EQUAL * n3 + 2 * lead_flops
     res = {
         "metric": METRIC,
         "value": round(value, 4),

@@ -70,8 +70,8 @@ enum {
 #define QT_PREC_MIXED47 2  /* as QT_PREC_MIXED with 7 digits (47 bits) and 28
                               digit-pair products instead of 21 (no reference
                               counterpart).  Measured at nb = 512 x 100 slabs:
-                              spectra within ~1e-9 and contact currents within
-                              1e-8 of QT_PREC_FP64; on a 24-slab device with kz
+                              spectra within ~1e-9 and contact currents at
+                              ~1e-8 of QT_PREC_FP64; on a 24-slab device with kz
                               the contact currents within ~1e-6. */
 
 /* option flags (qt_rgf_params.flags) */

@@ -741,18 +741,34 @@ struct Solve {
             // in the forward variant
             const ZOp W = opC(fwd && fix ? sc + 1 : sc);  // nq x n
             double2 *X = T[0], *GN0 = T[1], *P = T[2], *Y = T[3], *Q = T[4], *U = T[5];
+            // Z form: GN0 (W Gl) and GN0 (W Gln) as (GN0 W) Gl and (GN0 W) Gln,
+            // three products instead of four, between slabs of equal size (the
+            // RGF variants; the GW variant keeps GN0 and Q for its extra
+            // blocks).  The choice depends on the device and the variant only,
+            // so a point's results do not depend on the outputs requested.
+            const bool zform = !gw && nq == n;
+            double2* Z = P;  // n x n, ld n
             g(OP_N, OP_C, n, nq, n, opGl(sc), W, X, n, nn);                                   // X = Gl W^H
             g(OP_N, OP_N, n, nq, nq, op_member(X, n, nn), op_member(Gc, nq, nn), GN0, n, nn);  // GN0 = X G
-            g(OP_N, OP_N, nq, n, n, W, opGl(sc), P, nq, nn);                                   // P = W Gl
-            g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(P, nq, nn), Gn, n, nn, 1.0,
-              opGl(sc), 1.0);                                                                  // G = Gl + GN0 P
+            if (zform) {
+                g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), W, Z, n, nn);                  // Z = GN0 W
+                g(OP_N, OP_N, n, n, n, op_member(Z, n, nn), opGl(sc), Gn, n, nn, 1.0, opGl(sc),
+                  1.0);                                                                        // G = Gl + Z Gl
+            } else {
+                g(OP_N, OP_N, nq, n, n, W, opGl(sc), P, nq, nn);                               // P = W Gl
+                g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(P, nq, nn), Gn, n, nn, 1.0,
+                  opGl(sc), 1.0);                                                              // G = Gl + GN0 P
+            }
             g(OP_N, OP_N, n, nq, nq, op_member(X, n, nn), op_member(Lc, nq, nn), Y, n, nn);    // Y = X G<
             g(OP_N, OP_C, n, n, nq, op_member(Y, n, nn), op_member(X, n, nn), U, n, nn, 1.0,
               opGln(sc), 1.0);                                                                 // U = Gln + Y X^H
-            g(OP_N, OP_N, nq, n, n, W, opGln(sc), Q, nq, nn);                                  // Q = W Gln
+            if (!zform || want_j) g(OP_N, OP_N, nq, n, n, W, opGln(sc), Q, nq, nn);           // Q = W Gln
             if (want_gw) gw_extras(sc, n, W, X, Y, Q, GN0, Gc, Lc, g);
             if (!sigma) {
-                g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(Q, nq, nn), Y, n, nn);  // Tm = GN0 Q
+                if (zform)
+                    g(OP_N, OP_N, n, n, n, op_member(Z, n, nn), opGln(sc), Y, n, nn);         // Tm = Z Gln
+                else
+                    g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(Q, nq, nn), Y, n, nn);  // Tm = GN0 Q
                 // G< = U + Tm - Tm^H: the reference's last term Gln W^H GN0^H equals
                 // -(GN0 W Gln)^H because Gln is anti-Hermitian when every injection
                 // is a contact term -(Sigma - Sigma^H) f
@@ -769,8 +785,12 @@ struct Solve {
             } else {
                 // scattering injections (sigma_lesser_ph S, or a Sigma^< that is not
                 // anti-Hermitian) break that symmetry: the reference's four terms
-                g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(Q, nq, nn), Y, n, nn, 1.0,
-                  op_member(U, n, nn), 1.0);                                                     // U + GN0 Q
+                if (zform)
+                    g(OP_N, OP_N, n, n, n, op_member(Z, n, nn), opGln(sc), Y, n, nn, 1.0, op_member(U, n, nn),
+                      1.0);                                                                      // U + Z Gln
+                else
+                    g(OP_N, OP_N, n, n, nq, op_member(GN0, n, nn), op_member(Q, nq, nn), Y, n, nn, 1.0,
+                      op_member(U, n, nn), 1.0);                                                 // U + GN0 Q
                 g(OP_N, OP_C, n, nq, n, opGln(sc), W, P, n, nn);                                  // Gln W^H
                 g(OP_N, OP_C, n, n, nq, op_member(P, n, nn), op_member(GN0, n, nn), Ln, n, nn, 1.0,
                   op_member(Y, n, nn), 1.0);                                                     // + Gln W^H GN0^H
