@@ -420,11 +420,13 @@ def spectra_errors(mix: dict, ref: dict, nm) -> dict:
 
 def run_mixed_leg(args, rgf, H, Eall, Kall, nb: int, nx: int, fp64_value: float, digits: int = 6) -> dict:
     """The same workload with QT_PREC_MIXED (digits = 6: sweep products on the
-    int8 matrix cores, operands rounded to 40 bits) or QT_PREC_MIXED47
-    (digits = 7, 47 bits); leads and inversions FP64: timed like the FP64
-    leg, and its error measured on 32 points spread over the grid."""
+    int8 matrix cores, operands rounded to 40 bits), QT_PREC_MIXED47
+    (digits = 7, 47 bits) or QT_PREC_FP64_3M (digits = 0: FP64 products with
+    three real matrix products per complex one); leads and inversions FP64:
+    timed like the FP64 leg, and its error measured on 32 points spread over
+    the grid."""
     B = args.batch
-    prec = rgf.QT_PREC_MIXED if digits == 6 else rgf.QT_PREC_MIXED47
+    prec = {6: rgf.QT_PREC_MIXED, 7: rgf.QT_PREC_MIXED47, 0: rgf.QT_PREC_FP64_3M}[digits]
     elapsed, acc = time_batches(rgf, H, Eall, Kall, B, nb * nx, args.steps, args.warmup, precision=prec)
     value = B * args.steps / elapsed
     idx = np.linspace(0, len(Eall) - 1, 32).astype(int)
@@ -438,8 +440,9 @@ def run_mixed_leg(args, rgf, H, Eall, Kall, nb: int, nx: int, fp64_value: float,
             "max_rel_err": max(errs.values()), "errors": errs, "bar": 1e-8,
             "meets_bar": bool(max(errs.values()) <= 1e-8),
             "error_sample": "32 points evenly spread over the 4096-point (E, kz) list, against the FP64 run",
-            "products": f"int8 MFMA, {digits} base-128 digits per value ({7 * digits - 2}-bit operands), "
-                        "Karatsuba complex product"}
+            "products": (f"int8 MFMA, {digits} base-128 digits per value ({7 * digits - 2}-bit operands), "
+                         "Karatsuba complex product") if digits else
+                        "FP64 MFMA, three real products per complex one ((Ar+Ai)(Br+Bi) - ArBr - AiBi)"}
 
 
 def run_leads(args, hm, rgf) -> dict:
@@ -899,6 +902,7 @@ def main() -> int:
     if not args.no_mixed:
         res["mixed"] = run_mixed_leg(args, rgf, H, Eall, Kall, nb, nx, value)
         res["mixed47"] = run_mixed_leg(args, rgf, H, Eall, Kall, nb, nx, value, digits=7)
+        res["fp64_3m"] = run_mixed_leg(args, rgf, H, Eall, Kall, nb, nx, value, digits=0)
     H.close()
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

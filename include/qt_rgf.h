@@ -73,6 +73,11 @@ enum {
                               spectra within ~1e-9 and contact currents at
                               ~1e-8 of QT_PREC_FP64; on a 24-slab device with kz
                               the contact currents within ~1e-6. */
+#define QT_PREC_FP64_3M 3  /* complex128 everywhere, the sweep's block products
+                              with three real matrix products per complex one
+                              ((Ar + Ai)(Br + Bi) - Ar Br - Ai Bi for the
+                              imaginary part) instead of four (no reference
+                              counterpart). */
 
 /* option flags (qt_rgf_params.flags) */
 #define QT_F_COUPLING_FIX 0x1   /* forward variant: the second pass couples slab
@@ -268,7 +273,7 @@ int qt_zgemm_batched(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_t k, 
                      int64_t ldb, int64_t strideB, double beta, void* R, int64_t ldr,
                      int64_t strideR);
 
-/* qt_zgemm_batched with a precision mode (QT_PREC_FP64, _MIXED or _MIXED47).  No
+/* qt_zgemm_batched with a precision mode (QT_PREC_FP64, _MIXED, _MIXED47 or _FP64_3M).  No
  * reference counterpart: the mixed mode is new.  Shapes with M, N or K below 64
  * or K above 1024 run in FP64 whatever the mode. */
 int qt_zgemm_batched_prec(int32_t opa, int32_t opb, int32_t m, int32_t n, int32_t k, int32_t batch,

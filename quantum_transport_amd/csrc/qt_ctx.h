@@ -110,7 +110,8 @@ struct Gemm {
     const int* kidx;
     const int* active = nullptr;
     int flop_batch = -1;
-    int prec = 0;  // 6 or 7: int8 digit-slice products with that many digits (QT_PREC_MIXED, _MIXED47)
+    int prec = 0;  // 3: FP64 three-multiplication products (QT_PREC_FP64_3M); 6 or 7: int8 digit-slice
+                   // products with that many digits (QT_PREC_MIXED, _MIXED47)
     // C = alpha op(A) op(B) + beta D
     void operator()(int opa, int opb, int M, int N, int K, ZOp A, ZOp B, double2* C, int64_t ldc,
                     int64_t sbc, double alpha = 1.0, ZOp D = ZOp(), double beta = 0.0) const {

@@ -4,7 +4,14 @@
 // C = alpha*op(A)op(B) + beta*D for one batch member.  K is staged through
 // LDS in chunks of 16 complex values, double buffered (2 x 32 KB).  Each
 // wave owns a 32x32 sub-tile = 2x2 MFMA tiles of 16x16; a complex product is
-// four v_mfma_f64_16x16x4 (re*re, -im*im into Re; re*im, im*re into Im).
+// four v_mfma_f64_16x16x4 (re*re, -im*im into Re; re*im, im*re into Im), or,
+// in the three-multiplication form (G3, QT_PREC_FP64_3M), three:
+//     P1 = Ar Br,  P2 = Ai Bi,  P3 = (Ar + Ai)(Br + Bi)  (sums rounded to FP64),
+//     Re = P1 - P2,  Im = (P3 - P1) - P2  (in the epilogue),
+// 3/4 of the matrix-core work, with the imaginary part's rounding error
+// bounded by |Ar + Ai||Br + Bi| + |Ar||Br| + |Ai||Bi| instead of
+// |Ar||Bi| + |Ai||Br| (Higham, Accuracy and Stability of Numerical
+// Algorithms, 2nd ed., sec. 23.2.4).
 //
 // Operand order: the MFMA is fed (B-fragment, A-fragment), so that the
 // 16x16 result lands transposed in the lanes.  v_mfma_f64_16x16x4f64 leaves
@@ -36,7 +43,7 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double2 cconj(double2 v) { return make_double2(v.x, -v.y); }
 
-template <int OPA, int OPB>
+template <int OPA, int OPB, bool G3>
 __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int tiles_m, int tiles,
                                                             int batch_pad) {
     // XCD-aware decode: linear id L -> (member, tile), member%8 == L%8
@@ -128,13 +135,15 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int til
     const int wm = wave & 1, wn = wave >> 1;
     const int li = lane & 15, lk = lane >> 4;
 
-    double4_t accR[2][2], accI[2][2];
+    // G3: accR = P1, accI = P2, acc3 = P3
+    double4_t accR[2][2], accI[2][2], acc3[G3 ? 2 : 1][G3 ? 2 : 1];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             accR[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
             accI[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+            if constexpr (G3) acc3[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
         }
 
     const int nk = (K + BK - 1) / BK;
@@ -158,15 +167,27 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int til
             for (int i = 0; i < 2; ++i) av[i] = As[buf][k][wm * 32 + i * 16 + li];
 #pragma unroll
             for (int j = 0; j < 2; ++j) bv[j] = Bs[buf][k][wn * 32 + j * 16 + li];
+            if constexpr (G3) {
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
-                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[j].y, av[i].y, accR[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].x, accI[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].y, accI[i][j], 0, 0, 0);
-                }
+                    for (int j = 0; j < 2; ++j) {
+                        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].y, accI[i][j], 0, 0, 0);
+                        acc3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x + bv[j].y, av[i].x + av[i].y,
+                                                                         acc3[i][j], 0, 0, 0);
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
+                        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[j].y, av[i].y, accR[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].x, accI[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].y, accI[i][j], 0, 0, 0);
+                    }
+            }
         }
         if (kt + 1 < nk) {
             storeA(buf ^ 1);
@@ -189,7 +210,13 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int til
             for (int q = 0; q < 4; ++q) {
                 const int gj = n0 + wn * 32 + j * 16 + 4 * q + lk;
                 if (gj >= N || (gj >= a.skip_lo && gj < a.skip_hi)) continue;
-                double re = alpha * accR[i][j][q], im = alpha * accI[i][j][q];
+                double cr = accR[i][j][q], ci = accI[i][j][q];
+                if constexpr (G3) {
+                    const double p1 = accR[i][j][q], p2 = accI[i][j][q];
+                    cr = p1 - p2;
+                    ci = (acc3[i][j][q] - p1) - p2;
+                }
+                double re = alpha * cr, im = alpha * ci;
                 if (Dp) {
                     const double2 d = Dp[gi + gj * a.D.ld];
                     re += beta * d.x;
@@ -206,7 +233,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_kernel(ZGemmArgs a, int til
 // of the 64x64 kernel and fewer global loads per flop.
 constexpr int WBM = 128, WBN = 64, WBK = 8;
 
-template <int OPA, int OPB>
+template <int OPA, int OPB, bool G3>
 __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, int tiles_m, int tiles) {
     const int L = blockIdx.x;
     const int lane8 = L % NXCD;
@@ -299,13 +326,15 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
     const int wm = wave & 1, wn = wave >> 1;
     const int li = lane & 15, lk = lane >> 4;
 
-    double4_t accR[4][2], accI[4][2];
+    // G3: accR = P1, accI = P2, acc3 = P3
+    double4_t accR[4][2], accI[4][2], acc3[G3 ? 4 : 1][G3 ? 2 : 1];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             accR[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
             accI[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+            if constexpr (G3) acc3[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
         }
 
     const int nk = (K + WBK - 1) / WBK;
@@ -324,25 +353,48 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
 #pragma unroll
         for (int kq = 0; kq < WBK / 4; ++kq) {
             const int k = kq * 4 + lk;
-            double2 av[4], bv[2];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = As[buf][k][wm * 64 + i * 16 + li];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) bv[j] = Bs[buf][k][wn * 32 + j * 16 + li];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
+            if constexpr (G3) {
+                // one A fragment at a time keeps the registers within two
+                // workgroups per CU
+                double2 bv[2];
+                double bs[2];
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].x, accI[i][j], 0, 0, 0);
+                    bv[j] = Bs[buf][k][wn * 32 + j * 16 + li];
+                    bs[j] = bv[j].x + bv[j].y;
                 }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < 4; ++i) {
+                    const double2 av = As[buf][k][wm * 64 + i * 16 + li];
+                    const double as = av.x + av.y;
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[j].y, av[i].y, accR[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].y, accI[i][j], 0, 0, 0);
+                    for (int j = 0; j < 2; ++j) {
+                        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av.x, accR[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av.y, accI[i][j], 0, 0, 0);
+                        acc3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bs[j], as, acc3[i][j], 0, 0, 0);
+                    }
                 }
+            } else {
+                double2 av[4], bv[2];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) av[i] = As[buf][k][wm * 64 + i * 16 + li];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) bv[j] = Bs[buf][k][wn * 32 + j * 16 + li];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].x, accI[i][j], 0, 0, 0);
+                    }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[j].y, av[i].y, accR[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].y, accI[i][j], 0, 0, 0);
+                    }
+            }
         }
         if (kt + 1 < nk) {
             storeA(buf ^ 1);
@@ -364,7 +416,13 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
             for (int q = 0; q < 4; ++q) {
                 const int gj = n0 + wn * 32 + j * 16 + 4 * q + lk;
                 if (gj >= N || (gj >= a.skip_lo && gj < a.skip_hi)) continue;
-                double re = alpha * accR[i][j][q], im = alpha * accI[i][j][q];
+                double cr = accR[i][j][q], ci = accI[i][j][q];
+                if constexpr (G3) {
+                    const double p1 = accR[i][j][q], p2 = accI[i][j][q];
+                    cr = p1 - p2;
+                    ci = (acc3[i][j][q] - p1) - p2;
+                }
+                double re = alpha * cr, im = alpha * ci;
                 if (Dp) {
                     const double2 d = Dp[gi + gj * a.D.ld];
                     re += beta * d.x;
@@ -376,7 +434,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
     }
 }
 
-template <int OPA, int OPB>
+template <int OPA, int OPB, bool G3>
 void launch_wide(hipStream_t s, const ZGemmArgs& a) {
     const int tiles_m = (a.M + WBM - 1) / WBM;
     const int tiles_n = (a.N + WBN - 1) / WBN;
@@ -384,7 +442,7 @@ void launch_wide(hipStream_t s, const ZGemmArgs& a) {
     const int batch_pad = (a.batch + NXCD - 1) / NXCD * NXCD;
     const int64_t blocks = (int64_t)tiles * batch_pad;
     if (blocks > 0x7fffffff) throw hip_error("zgemm: grid too large", 1);
-    hipLaunchKernelGGL((zgemm_mfma_wide_kernel<OPA, OPB>), dim3((unsigned)blocks), dim3(256), 0, s, a,
+    hipLaunchKernelGGL((zgemm_mfma_wide_kernel<OPA, OPB, G3>), dim3((unsigned)blocks), dim3(256), 0, s, a,
                        tiles_m, tiles);
     QT_HIP(hipGetLastError());
 }
@@ -425,7 +483,7 @@ bool use_wide(int M) {
     return M >= WBM && waste_wide <= waste_narrow;
 }
 
-template <int OPA, int OPB>
+template <int OPA, int OPB, bool G3>
 void launch_narrow(hipStream_t s, const ZGemmArgs& a) {
     const int tiles_m = (a.M + BM - 1) / BM;
     const int tiles_n = (a.N + BN - 1) / BN;
@@ -433,7 +491,7 @@ void launch_narrow(hipStream_t s, const ZGemmArgs& a) {
     const int batch_pad = (a.batch + NXCD - 1) / NXCD * NXCD;
     const int64_t blocks = (int64_t)tiles * batch_pad;
     if (blocks > 0x7fffffff) throw hip_error("zgemm: grid too large", 1);
-    hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB>), dim3((unsigned)blocks), dim3(256), 0, s, a,
+    hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, G3>), dim3((unsigned)blocks), dim3(256), 0, s, a,
                        tiles_m, tiles, batch_pad);
     QT_HIP(hipGetLastError());
 }
@@ -453,9 +511,9 @@ TuneState& tune_state() {
     return t;
 }
 
-template <int OPA, int OPB>
+template <int OPA, int OPB, bool G3>
 bool tuned_wide(hipStream_t s, const ZGemmArgs& a) {
-    const auto key = std::make_tuple(OPA * 3 + OPB, a.M, a.N, a.K, a.batch);
+    const auto key = std::make_tuple((OPA * 3 + OPB) * 2 + (G3 ? 1 : 0), a.M, a.N, a.K, a.batch);
     TuneState& ts = tune_state();
     {
         std::lock_guard<std::mutex> lk(ts.mu);
@@ -486,7 +544,7 @@ bool tuned_wide(hipStream_t s, const ZGemmArgs& a) {
     for (int rep = 0; rep < 2; ++rep)
         for (int v = 0; v < 2; ++v) {
             QT_HIP(hipEventRecord(ev[0], s));
-            if (v) launch_wide<OPA, OPB>(s, t); else launch_narrow<OPA, OPB>(s, t);
+            if (v) launch_wide<OPA, OPB, G3>(s, t); else launch_narrow<OPA, OPB, G3>(s, t);
             QT_HIP(hipEventRecord(ev[1], s));
             QT_HIP(hipEventSynchronize(ev[1]));
             float m = 0.0f;
@@ -497,27 +555,37 @@ bool tuned_wide(hipStream_t s, const ZGemmArgs& a) {
     QT_HIP(hipFree(scratch));
     const bool wide = ms[1] < ms[0];
     if (tune_log())
-        fprintf(stderr, "qt gemm tune: ops %d M %d N %d K %d batch %d (%d timed): 64x64 %.4f ms, 128x64 %.4f ms -> %s\n",
-                OPA * 3 + OPB, a.M, a.N, a.K, a.batch, members, ms[0], ms[1], wide ? "128x64" : "64x64");
+        fprintf(stderr,
+                "qt gemm tune: ops %d%s M %d N %d K %d batch %d (%d timed): 64x64 %.4f ms, 128x64 %.4f ms -> %s\n",
+                OPA * 3 + OPB, G3 ? " (3m)" : "", a.M, a.N, a.K, a.batch, members, ms[0], ms[1],
+                wide ? "128x64" : "64x64");
     std::lock_guard<std::mutex> lk(ts.mu);
     ts.wide[key] = wide;
     return wide;
 }
 
-template <int OPA, int OPB>
-void launch(hipStream_t s, const ZGemmArgs& a) {
+template <int OPA, int OPB, bool G3>
+void launch_g(hipStream_t s, const ZGemmArgs& a) {
     const int forced = tile_forced();
     bool wide;
     if (forced != TILE_AUTO)
         wide = forced == TILE_WIDE;
     else if (tune_enabled())
-        wide = tuned_wide<OPA, OPB>(s, a);
+        wide = tuned_wide<OPA, OPB, G3>(s, a);
     else
         wide = use_wide(a.M);
     if (wide)
-        launch_wide<OPA, OPB>(s, a);
+        launch_wide<OPA, OPB, G3>(s, a);
     else
-        launch_narrow<OPA, OPB>(s, a);
+        launch_narrow<OPA, OPB, G3>(s, a);
+}
+
+template <int OPA, int OPB>
+void launch(hipStream_t s, const ZGemmArgs& a) {
+    if (a.prec == 3)
+        launch_g<OPA, OPB, true>(s, a);
+    else
+        launch_g<OPA, OPB, false>(s, a);
 }
 
 }  // namespace
@@ -526,7 +594,7 @@ void zgemm(hipStream_t s, int opa, int opb, const ZGemmArgs& a, Timer* t) {
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return;
     if (t && t->enabled) t->begin(s, 0);
     const int key = opa * 3 + opb;
-    if (a.prec != 0 && zgemm_i8_fits(a)) {
+    if ((a.prec == 6 || a.prec == 7) && zgemm_i8_fits(a)) {
         zgemm_i8(s, opa, opb, a, a.prec);
         if (t && t->enabled)
             t->end(s, 0, 8.0 * a.M * a.N * (double)a.K * (a.flop_batch >= 0 ? a.flop_batch : a.batch));

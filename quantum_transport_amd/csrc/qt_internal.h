@@ -50,13 +50,15 @@ struct ZGemmArgs {
     const int* active = nullptr;   // skip members with active[b] == 0
     int skip_lo = 0, skip_hi = 0;  // output columns [skip_lo, skip_hi) not written
     int flop_batch = -1;           // members counted by the timer (-1: batch)
-    int prec = 0;                  // 6 or 7: int8 digit-slice product with that many digits where the shape allows
+    int prec = 0;                  // 3: FP64 three-multiplication form; 6 or 7: int8 digit-slice product with that
+                                   // many digits where the shape allows
 };
 
 struct Timer;  // forward
 
 // C = alpha * op(A) op(B) + beta * D, complex128, MFMA f64 (or, with
-// prec = 6 or 7 and a shape zgemm_i8_fits accepts, the int8 emulation).
+// prec = 3, three real products per complex one; with prec = 6 or 7 and a
+// shape zgemm_i8_fits accepts, the int8 emulation).
 void zgemm(hipStream_t s, int opa, int opb, const ZGemmArgs& a, Timer* t);
 // Same product with operands rounded to 7 digits - 2 bits (6 digits: 40,
 // 7 digits: 47) relative to their row (op(A)) or column (op(B)) maximum, on

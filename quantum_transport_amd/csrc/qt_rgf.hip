@@ -38,13 +38,15 @@ Ctx* g_ctx = nullptr;
 std::mutex g_init_mu;
 static thread_local std::string g_last_error;
 
-// int8 digits per value of a precision mode's products: 0 for QT_PREC_FP64,
-// -1 for a mode that does not exist
+// product form of a precision mode (ZGemmArgs::prec): 0 for QT_PREC_FP64, 3
+// for the three-multiplication FP64 form, the int8 digits per value for the
+// mixed modes; -1 for a mode that does not exist
 static int prec_digits(int precision) {
     switch (precision) {
         case QT_PREC_FP64: return 0;
         case QT_PREC_MIXED: return 6;
         case QT_PREC_MIXED47: return 7;
+        case QT_PREC_FP64_3M: return 3;
         default: return -1;
     }
 }

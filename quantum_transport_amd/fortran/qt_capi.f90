@@ -8,7 +8,8 @@ module qt_capi
 
     integer(c_int), parameter :: QT_OK = 0, QT_ERR_SINGULAR = 5
     integer(c_int32_t), parameter :: QT_RGF_FORWARD = 0, QT_RGF_BACKWARD = 1, QT_RGF_GW = 2
-    integer(c_int32_t), parameter :: QT_PREC_FP64 = 0, QT_PREC_MIXED = 1, QT_PREC_MIXED47 = 2
+    integer(c_int32_t), parameter :: QT_PREC_FP64 = 0, QT_PREC_MIXED = 1, QT_PREC_MIXED47 = 2, &
+                                     QT_PREC_FP64_3M = 3
     integer(c_int32_t), parameter :: QT_F_COUPLING_FIX = 1, QT_F_DEVICE_OUTPUTS = 2, QT_F_LEAD_RESET = 4
     integer(c_int32_t), parameter :: QT_INFO_SINGULAR = 1, QT_INFO_LEAD_NMAX = 2
 

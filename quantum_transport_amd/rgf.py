@@ -28,6 +28,7 @@ QT_RGF_GW = 2
 QT_PREC_FP64 = 0
 QT_PREC_MIXED = 1
 QT_PREC_MIXED47 = 2
+QT_PREC_FP64_3M = 3
 QT_F_COUPLING_FIX = 0x1
 QT_F_DEVICE_OUTPUTS = 0x2
 QT_F_LEAD_RESET = 0x4
@@ -413,7 +414,8 @@ def zgemm_batched(A: np.ndarray, B: np.ndarray, opa: int = 0, opb: int = 0, alph
     matrix_c.f90:615-660).  A, B: (batch, rows, cols) stacks of column-major
     blocks (each block stored Fortran-ordered).  precision=QT_PREC_MIXED runs
     the product on the int8 matrix cores (operands rounded to 40 bits),
-    QT_PREC_MIXED47 likewise with 47 bits."""
+    QT_PREC_MIXED47 likewise with 47 bits; QT_PREC_FP64_3M in FP64 with three
+    real products per complex one."""
     init()
     A = np.asarray(A, dtype=np.complex128)
     B = np.asarray(B, dtype=np.complex128)

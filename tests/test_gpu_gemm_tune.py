@@ -3,7 +3,8 @@
 products are bit-identical whichever is used: the same batched products
 (NN and NC, shapes that the row-waste rule and the tuner may split either
 way) with each kernel forced (QT_GEMM_TILE, read once per process, hence one
-short child process per setting) and with the tuner."""
+short child process per setting) and with the tuner; likewise for the
+three-multiplication form (QT_PREC_FP64_3M)."""
 import os
 import subprocess
 import sys
@@ -29,6 +30,8 @@ for i, (b, m, n, k) in enumerate([(9, 192, 160, 200), (4, 512, 512, 512), (3, 13
     Bc = rng.standard_normal((b, n, k)) + 1j * rng.standard_normal((b, n, k))
     out[f"nn{i}"] = rgf.zgemm_batched(A, B, 0, 0)
     out[f"nc{i}"] = rgf.zgemm_batched(A, Bc, 0, 2)
+    out[f"nn3m{i}"] = rgf.zgemm_batched(A, B, 0, 0, precision=rgf.QT_PREC_FP64_3M)
+    out[f"nc3m{i}"] = rgf.zgemm_batched(A, Bc, 0, 2, precision=rgf.QT_PREC_FP64_3M)
 np.savez(sys.argv[1], **out)
 print("gemm child done", flush=True)
 """
