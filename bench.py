@@ -799,7 +799,8 @@ def main() -> int:
                          "small: the reference's CPU-case shape, nb=64 x 32 slabs x 256 E")
     ap.add_argument("--gw-slabs", type=int, default=2)
     ap.add_argument("--scba-slabs", type=int, default=6, help="scba: slabs of the device")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the legs of the other precision modes (mixed, mixed47, fp64_3m)")
     ap.add_argument("--warm-points", type=int, default=1024, help="warmstart: evaluated points")
     ap.add_argument("--warm-epochs", type=int, default=30, help="warmstart: training epochs")
     ap.add_argument("--cpu-validate", action="store_true",
