@@ -434,6 +434,186 @@ __global__ __launch_bounds__(256, 2) void zgemm_mfma_wide_kernel(ZGemmArgs a, in
     }
 }
 
+// 128x64 tiles with 8 waves of 32x32 (2x2 MFMA tiles each): the 128x64
+// kernel's LDS staging and global loads, the 64x64 kernel's per-wave register
+// tile, so the accumulators of the three-multiplication form fit beside
+// more waves per CU.  Every output element sees the MFMAs of the other two
+// kernels in the same order.
+template <int OPA, int OPB, bool G3>
+__global__ __launch_bounds__(512, G3 ? 1 : 2) void zgemm_mfma_w8_kernel(ZGemmArgs a, int tiles_m, int tiles) {
+    const int L = blockIdx.x;
+    const int lane8 = L % NXCD;
+    const int rest = L / NXCD;
+    const int tile = rest % tiles;
+    const int b = lane8 + NXCD * (rest / tiles);
+    if (b >= a.batch) return;
+    if (a.active && !a.active[b]) return;
+
+    const int kz = a.kidx ? a.kidx[b >> a.kshift] : 0;
+    const double2* __restrict__ Ap = a.A.p + b * a.A.sb + kz * a.A.sk;
+    const double2* __restrict__ Bp = a.B.p + b * a.B.sb + kz * a.B.sk;
+    const int64_t lda = a.A.ld, ldb = a.B.ld;
+    const int M = a.M, N = a.N, K = a.K;
+    const int m0 = (tile % tiles_m) * WBM;
+    const int n0 = (tile / tiles_m) * WBN;
+
+    __shared__ double2 As[2][WBK][WBM + 9];
+    __shared__ double2 Bs[2][WBK][WBN + 9];
+
+    const int tid = threadIdx.x;
+    double2 ra[2], rb;
+
+    auto loadA = [&](int k0) {
+        if (OPA == OP_N) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int e = tid + 512 * q, kk = e >> 7, ii = e & 127;
+                const int gi = m0 + ii, gk = k0 + kk;
+                ra[q] = (gi < M && gk < K) ? Ap[gi + gk * lda] : make_double2(0.0, 0.0);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int kk = tid & 7, ii = (tid >> 3) + 64 * q;
+                const int gi = m0 + ii, gk = k0 + kk;
+                ra[q] = (gi < M && gk < K) ? Ap[gk + gi * lda] : make_double2(0.0, 0.0);
+            }
+        }
+    };
+    auto storeA = [&](int buf) {
+        if (OPA == OP_N) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int e = tid + 512 * q;
+                As[buf][e >> 7][e & 127] = ra[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                As[buf][tid & 7][(tid >> 3) + 64 * q] = OPA == OP_C ? cconj(ra[q]) : ra[q];
+        }
+    };
+    auto loadB = [&](int k0) {
+        if (OPB == OP_N) {
+            const int kk = tid & 7, jj = tid >> 3;
+            const int gj = n0 + jj, gk = k0 + kk;
+            rb = (gj < N && gk < K) ? Bp[gk + gj * ldb] : make_double2(0.0, 0.0);
+        } else {
+            const int jj = tid & 63, kk = tid >> 6;
+            const int gj = n0 + jj, gk = k0 + kk;
+            rb = (gj < N && gk < K) ? Bp[gj + gk * ldb] : make_double2(0.0, 0.0);
+        }
+    };
+    auto storeB = [&](int buf) {
+        if (OPB == OP_N)
+            Bs[buf][tid & 7][tid >> 3] = rb;
+        else
+            Bs[buf][tid >> 6][tid & 63] = OPB == OP_C ? cconj(rb) : rb;
+    };
+
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave & 3, wn = wave >> 2;
+    const int li = lane & 15, lk = lane >> 4;
+
+    double4_t accR[2][2], accI[2][2], acc3[G3 ? 2 : 1][G3 ? 2 : 1];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            accR[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+            accI[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+            if constexpr (G3) acc3[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+        }
+
+    const int nk = (K + WBK - 1) / WBK;
+    loadA(0);
+    loadB(0);
+    storeA(0);
+    storeB(0);
+    __syncthreads();
+
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) {
+            loadA((kt + 1) * WBK);
+            loadB((kt + 1) * WBK);
+        }
+#pragma unroll
+        for (int kq = 0; kq < WBK / 4; ++kq) {
+            const int k = kq * 4 + lk;
+            double2 av[2], bv[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) av[i] = As[buf][k][wm * 32 + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bv[j] = Bs[buf][k][wn * 32 + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if constexpr (G3) {
+                        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].y, accI[i][j], 0, 0, 0);
+                        acc3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x + bv[j].y, av[i].x + av[i].y,
+                                                                         acc3[i][j], 0, 0, 0);
+                    } else {
+                        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].x, accR[i][j], 0, 0, 0);
+                        accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[j].y, av[i].y, accR[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].y, av[i].x, accI[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].x, av[i].y, accI[i][j], 0, 0, 0);
+                    }
+                }
+        }
+        if (kt + 1 < nk) {
+            storeA(buf ^ 1);
+            storeB(buf ^ 1);
+        }
+        __syncthreads();
+    }
+
+    double2* __restrict__ Cp = a.C + b * a.sbc + kz * a.skc;
+    const double2* __restrict__ Dp = a.D.p ? a.D.p + b * a.D.sb + kz * a.D.sk : nullptr;
+    const double alpha = a.alpha, beta = a.beta;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int gi = m0 + wm * 32 + i * 16 + li;
+        if (gi >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int gj = n0 + wn * 32 + j * 16 + 4 * q + lk;
+                if (gj >= N || (gj >= a.skip_lo && gj < a.skip_hi)) continue;
+                double cr = accR[i][j][q], ci = accI[i][j][q];
+                if constexpr (G3) {
+                    const double p1 = accR[i][j][q], p2 = accI[i][j][q];
+                    cr = p1 - p2;
+                    ci = (acc3[i][j][q] - p1) - p2;
+                }
+                double re = alpha * cr, im = alpha * ci;
+                if (Dp) {
+                    const double2 d = Dp[gi + gj * a.D.ld];
+                    re += beta * d.x;
+                    im += beta * d.y;
+                }
+                Cp[gi + gj * a.ldc] = make_double2(re, im);
+            }
+        }
+    }
+}
+
+template <int OPA, int OPB, bool G3>
+void launch_w8(hipStream_t s, const ZGemmArgs& a) {
+    const int tiles_m = (a.M + WBM - 1) / WBM;
+    const int tiles_n = (a.N + WBN - 1) / WBN;
+    const int tiles = tiles_m * tiles_n;
+    const int batch_pad = (a.batch + NXCD - 1) / NXCD * NXCD;
+    const int64_t blocks = (int64_t)tiles * batch_pad;
+    if (blocks > 0x7fffffff) throw hip_error("zgemm: grid too large", 1);
+    hipLaunchKernelGGL((zgemm_mfma_w8_kernel<OPA, OPB, G3>), dim3((unsigned)blocks), dim3(512), 0, s, a, tiles_m,
+                       tiles);
+    QT_HIP(hipGetLastError());
+}
+
 template <int OPA, int OPB, bool G3>
 void launch_wide(hipStream_t s, const ZGemmArgs& a) {
     const int tiles_m = (a.M + WBM - 1) / WBM;
@@ -447,17 +627,19 @@ void launch_wide(hipStream_t s, const ZGemmArgs& a) {
     QT_HIP(hipGetLastError());
 }
 
-// Tile choice.  QT_GEMM_TILE=64x64 or =128x64 fixes the kernel; otherwise
-// the autotuner below picks per shape (QT_GEMM_TUNE=0: the row-waste rule of
-// use_wide alone).  Both kernels accumulate every output element in the same
-// order (k ascending in groups of 4, the same two MFMAs per group and
-// component, the same epilogue), so the choice never changes a result.
-enum { TILE_AUTO = 0, TILE_NARROW = 1, TILE_WIDE = 2 };
+// Tile choice.  QT_GEMM_TILE=64x64, =128x64 or =128x64w8 fixes the kernel;
+// otherwise the autotuner below picks per shape (QT_GEMM_TUNE=0: the
+// row-waste rule of use_wide alone).  All three kernels accumulate every
+// output element in the same order (k ascending in groups of 4, the same
+// MFMAs per group and component, the same epilogue), so the choice never
+// changes a result.
+enum { TILE_AUTO = 0, TILE_NARROW = 1, TILE_WIDE = 2, TILE_W8 = 3 };
 int tile_forced() {
     static const int v = [] {
         const char* e = getenv("QT_GEMM_TILE");
         if (e && std::string(e) == "64x64") return (int)TILE_NARROW;
         if (e && std::string(e) == "128x64") return (int)TILE_WIDE;
+        if (e && std::string(e) == "128x64w8") return (int)TILE_W8;
         return (int)TILE_AUTO;
     }();
     return v;
@@ -497,14 +679,14 @@ void launch_narrow(hipStream_t s, const ZGemmArgs& a) {
 }
 
 // Autotuner: the first product of each shape (op pair, M, N, K, batch) runs
-// both kernels twice into a scratch output (the operands and the addend are
+// the three kernels twice into a scratch output (the operands and the addend are
 // only read; at most 512 MiB of scratch, fewer members where the full batch
 // needs more) after the work already queued on the stream, and the faster
 // kernel is kept for the shape for the life of the process.  Where the
 // scratch cannot be allocated the row-waste rule decides.
 struct TuneState {
     std::mutex mu;
-    std::map<std::tuple<int, int, int, int, int>, bool> wide;
+    std::map<std::tuple<int, int, int, int, int>, int> tile;
 };
 TuneState& tune_state() {
     static TuneState t;
@@ -512,13 +694,23 @@ TuneState& tune_state() {
 }
 
 template <int OPA, int OPB, bool G3>
-bool tuned_wide(hipStream_t s, const ZGemmArgs& a) {
+void launch_tile(hipStream_t s, const ZGemmArgs& a, int tile) {
+    if (tile == TILE_W8)
+        launch_w8<OPA, OPB, G3>(s, a);
+    else if (tile == TILE_WIDE)
+        launch_wide<OPA, OPB, G3>(s, a);
+    else
+        launch_narrow<OPA, OPB, G3>(s, a);
+}
+
+template <int OPA, int OPB, bool G3>
+int tuned_tile(hipStream_t s, const ZGemmArgs& a) {
     const auto key = std::make_tuple((OPA * 3 + OPB) * 2 + (G3 ? 1 : 0), a.M, a.N, a.K, a.batch);
     TuneState& ts = tune_state();
     {
         std::lock_guard<std::mutex> lk(ts.mu);
-        auto it = ts.wide.find(key);
-        if (it != ts.wide.end()) return it->second;
+        auto it = ts.tile.find(key);
+        if (it != ts.tile.end()) return it->second;
     }
     const size_t per = sizeof(double2) * (size_t)a.M * (size_t)a.N;
     const size_t cap = (size_t)512 << 20;
@@ -531,20 +723,20 @@ bool tuned_wide(hipStream_t s, const ZGemmArgs& a) {
     t.skip_lo = t.skip_hi = 0;
     void* scratch = nullptr;
     hipEvent_t ev[3];
-    float ms[2] = {1e30f, 1e30f};
+    float ms[3] = {1e30f, 1e30f, 1e30f};  // 64x64, 128x64, 128x64 with 8 waves
     if (hipMalloc(&scratch, per * (size_t)members) != hipSuccess) {
         (void)hipGetLastError();  // no room for the scratch output: the row-waste rule, kept
-        const bool wide = use_wide(a.M);
+        const int tile = use_wide(a.M) ? TILE_WIDE : TILE_NARROW;
         std::lock_guard<std::mutex> lk(ts.mu);
-        ts.wide[key] = wide;
-        return wide;
+        ts.tile[key] = tile;
+        return tile;
     }
     for (auto& e : ev) QT_HIP(hipEventCreate(&e));
     t.C = (double2*)scratch;
     for (int rep = 0; rep < 2; ++rep)
-        for (int v = 0; v < 2; ++v) {
+        for (int v = 0; v < 3; ++v) {
             QT_HIP(hipEventRecord(ev[0], s));
-            if (v) launch_wide<OPA, OPB, G3>(s, t); else launch_narrow<OPA, OPB, G3>(s, t);
+            launch_tile<OPA, OPB, G3>(s, t, v + 1);
             QT_HIP(hipEventRecord(ev[1], s));
             QT_HIP(hipEventSynchronize(ev[1]));
             float m = 0.0f;
@@ -553,31 +745,32 @@ bool tuned_wide(hipStream_t s, const ZGemmArgs& a) {
         }
     for (auto& e : ev) QT_HIP(hipEventDestroy(e));
     QT_HIP(hipFree(scratch));
-    const bool wide = ms[1] < ms[0];
+    int best = 0;
+    for (int v = 1; v < 3; ++v)
+        if (ms[v] < ms[best]) best = v;
+    const int tile = best + 1;
     if (tune_log())
         fprintf(stderr,
-                "qt gemm tune: ops %d%s M %d N %d K %d batch %d (%d timed): 64x64 %.4f ms, 128x64 %.4f ms -> %s\n",
-                OPA * 3 + OPB, G3 ? " (3m)" : "", a.M, a.N, a.K, a.batch, members, ms[0], ms[1],
-                wide ? "128x64" : "64x64");
+                "qt gemm tune: ops %d%s M %d N %d K %d batch %d (%d timed): 64x64 %.4f ms, 128x64 %.4f ms, "
+                "128x64w8 %.4f ms -> %s\n",
+                OPA * 3 + OPB, G3 ? " (3m)" : "", a.M, a.N, a.K, a.batch, members, ms[0], ms[1], ms[2],
+                tile == TILE_W8 ? "128x64w8" : (tile == TILE_WIDE ? "128x64" : "64x64"));
     std::lock_guard<std::mutex> lk(ts.mu);
-    ts.wide[key] = wide;
-    return wide;
+    ts.tile[key] = tile;
+    return tile;
 }
 
 template <int OPA, int OPB, bool G3>
 void launch_g(hipStream_t s, const ZGemmArgs& a) {
     const int forced = tile_forced();
-    bool wide;
+    int tile;
     if (forced != TILE_AUTO)
-        wide = forced == TILE_WIDE;
+        tile = forced;
     else if (tune_enabled())
-        wide = tuned_wide<OPA, OPB, G3>(s, a);
+        tile = tuned_tile<OPA, OPB, G3>(s, a);
     else
-        wide = use_wide(a.M);
-    if (wide)
-        launch_wide<OPA, OPB, G3>(s, a);
-    else
-        launch_narrow<OPA, OPB, G3>(s, a);
+        tile = use_wide(a.M) ? TILE_WIDE : TILE_NARROW;
+    launch_tile<OPA, OPB, G3>(s, a, tile);
 }
 
 template <int OPA, int OPB>

@@ -1,6 +1,7 @@
-"""The GEMM tile autotuner may pick either kernel per shape; the 64x64 and
-128x64 kernels accumulate every output element in the same order, so the
-products are bit-identical whichever is used: the same batched products
+"""The GEMM tile autotuner may pick any of three kernels per shape; the 64x64,
+128x64 and 128x64-with-8-waves kernels accumulate every output element in
+the same order, so the products are bit-identical whichever is used: the
+same batched products
 (NN and NC, shapes that the row-waste rule and the tuner may split either
 way) with each kernel forced (QT_GEMM_TILE, read once per process, hence one
 short child process per setting) and with the tuner; likewise for the
@@ -40,7 +41,7 @@ print("gemm child done", flush=True)
 def test_tile_choice_does_not_change_products(tmp_path):
     res = {}
     for name, env in (("narrow", {"QT_GEMM_TILE": "64x64"}), ("wide", {"QT_GEMM_TILE": "128x64"}),
-                      ("tuned", {"QT_GEMM_TUNE_LOG": "1"})):
+                      ("w8", {"QT_GEMM_TILE": "128x64w8"}), ("tuned", {"QT_GEMM_TUNE_LOG": "1"})):
         print("gemm child", name, flush=True)
         e = dict(os.environ)
         e.pop("QT_GEMM_TILE", None)
@@ -54,6 +55,6 @@ def test_tile_choice_does_not_change_products(tmp_path):
             assert "qt gemm tune" in r.stderr
         res[name] = np.load(path)
     for key in res["narrow"].files:
-        a, b, c = res["narrow"][key], res["wide"][key], res["tuned"][key]
-        assert np.array_equal(a, b), key
-        assert np.array_equal(a, c), key
+        a = res["narrow"][key]
+        for other in ("wide", "w8", "tuned"):
+            assert np.array_equal(a, res[other][key]), (key, other)
