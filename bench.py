@@ -503,7 +503,7 @@ def run_leads(args, hm, rgf) -> dict:
                                f"{B} points per step", "nb": nb, "points_per_step": B},
         "roofline": {"bound": "mfma", "achieved": round(gemm_tf, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": round(gemm_tf / PEAK_FP64, 4), "traffic": None,
-                     "kernel": "zgemm_mfma_kernel (batched complex128 GEMM, decimation products)"},
+                     "kernel": "zgemm_mfma_*_kernel (batched complex128 GEMM, autotuned tile; decimation products)"},
         "path": {"iterations_avg": round(float(it.mean()), 2), "iterations_max": int(it.max()),
                  "not_converged": int((~cv).sum()),
                  "algorithmic_tflops": round(flops / elapsed / 1e12, 3),
@@ -636,7 +636,7 @@ def run_uneven(args, hm, rgf) -> dict:
                    "slabs": 80, "sum_n": nd, "points_per_step": B, "precision": "fp64"},
         "roofline": {"bound": "mfma", "achieved": round(gemm_tf, 3), "peak": PEAK_FP64,
                      "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4), "traffic": None,
-                     "kernel": "zgemm_mfma_kernel (batched complex128 GEMM)",
+                     "kernel": "zgemm_mfma_*_kernel (batched complex128 GEMM, autotuned tile)",
                      "launches": acc["gemm_launches"]},
         "sample": acc["sample"],
         "path": {"lead_iterations_avg": round(K, 2),
@@ -694,7 +694,7 @@ def run_small(args, hm, rgf) -> dict:
                    "nb": nb, "slabs": nx, "energies": nen, "points_per_step": nen, "precision": "fp64"},
         "roofline": {"bound": "mfma", "achieved": round(gemm_tf, 3), "peak": PEAK_FP64,
                      "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4), "traffic": None,
-                     "kernel": "zgemm_mfma_kernel (batched complex128 GEMM)",
+                     "kernel": "zgemm_mfma_*_kernel (batched complex128 GEMM, autotuned tile)",
                      "launches": acc["gemm_launches"]},
         "path": {"lead_iterations_avg": round(K, 2),
                  "algorithmic_tflops": round(points * path_flops / elapsed / 1e12, 3),
@@ -888,7 +888,7 @@ def main() -> int:
         "roofline": {"bound": "mfma", "achieved": round(gemm_tf, 3), "peak": PEAK_FP64,
                      "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4),
                      "traffic": (GEMM_BYTES_PER_MEMBER_512 * B) if nb == 512 else None,
-                     "kernel": "zgemm_mfma_kernel (batched complex128 GEMM)",
+                     "kernel": "zgemm_mfma_*_kernel (batched complex128 GEMM, autotuned tile)",
                      "launches": acc["gemm_launches"]},
         "sample": acc["sample"],
         "path": {"lead_iterations_avg": round(K, 2),
